@@ -1,0 +1,175 @@
+"""Headline benchmark: batched CR-PCR, FP32, 65,536 systems x 1,024 unknowns.
+
+One step = one solve of the whole batch (inputs already resident in HBM,
+written by the device generator, seed 1, diagonally dominant kind). Prints
+ONE JSON line with unknowns solved per second, the HBM roofline of the solve
+kernel (algorithmic bytes 5 * n * 4 per system: read a, b, c, d, write x)
+and a CPU baseline (the FP32 Thomas recurrence of the oracle, OpenMP over all
+host cores) timed in the same run.
+
+    python bench.py [--steps K] [--warmup W] [--method crpcr] [--n 1024]
+                    [--systems 65536] [--rest 64] [--dtype f32]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E, AMD published peak
+METRIC = ("unknowns solved/s + % HBM roofline; served p50/p99 ms vs plain, "
+          "same accuracy")
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--method", default="crpcr")
+    p.add_argument("--n", type=int, default=1024)
+    p.add_argument("--systems", type=int, default=65536)
+    p.add_argument("--rest", type=int, default=64)
+    p.add_argument("--dtype", default="f32", choices=["f32", "f64"])
+    p.add_argument("--lanes", type=int, default=0)
+    p.add_argument("--per-lane", type=int, default=0)
+    p.add_argument("--io", type=int, default=0)
+    p.add_argument("--cpu-seconds", type=float, default=10.0)
+    p.add_argument("--no-cpu", action="store_true")
+    return p.parse_args()
+
+
+def traffic_from_profiles(workload_key: str):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by a copy kernel of known size), if committed."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(workload_key, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(args, x_gpu_np, n, N):
+    """FP32 Thomas (same recurrence as tridiagonal_gold.h:26-44) with OpenMP
+    over all host cores on the same inputs, repeated for about
+    --cpu-seconds; also reports the GPU result's max relative error against
+    the FP64 Thomas oracle on those inputs."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    dt = np.float32 if args.dtype == "f32" else np.float64
+    a, b, c, d = O.generate("dominant", 1, n, N, dt)
+    threads = O.max_threads()
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        if dt == np.float32:
+            O.thomas_f32(a, b, c, d, n, threads)
+        else:
+            O.thomas(a, b, c, d, n=n, threads=threads)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= args.cpu_seconds:
+            break
+    xref = O.thomas(a, b, c, d, n=n, threads=threads)
+    rel, rms, fails = O.compare(x_gpu_np, xref, n=n, rms_eps=1e-3)
+    return {
+        "value": reps * n * N / el,
+        "unit": "unknowns/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle Thomas ({args.dtype} arithmetic), full {N}x{n} batch "
+                   f"x {reps} passes, {el:.1f} s, OpenMP {threads} threads"),
+        "gpu_max_rel_err_vs_fp64_thomas": float(np.max(rel)),
+        "gpu_reference_rms_failures": int(fails),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: multi-GPU runs are not part of this build "
+              "(independent systems shard by system range with no exchange).",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import tridiagonalsolvers_amd as T
+
+    n, N = args.n, args.systems
+    tdt = torch.float32 if args.dtype == "f32" else torch.float64
+    esize = 4 if args.dtype == "f32" else 8
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    a, b, c, d = (torch.empty(n * N, dtype=tdt, device=dev) for _ in range(4))
+    x = torch.empty_like(a)
+    stream = torch.cuda.current_stream()
+    T.generate(0, 1, a, b, c, d, n, N, stream=stream)
+    h = T.Handle()
+    opts = T.make_opts(rest=args.rest if args.method in ("crpcr", "crpcrnbc") else 0,
+                       gaussian=args.rest if args.method == "pcrgauss" else 0,
+                       lanes=args.lanes, per_lane=args.per_lane, io_mode=args.io)
+    plan = T.plan_query(args.method, n, N, esize, opts)
+
+    def step():
+        h.solve(args.method, a, b, c, d, x, n, N, opts, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # per-launch kernel time: HIP events on the launch stream
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e0, e1 in evs:
+        e0.record(stream)
+        step()
+        e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = sorted(e0.elapsed_time(e1) for e0, e1 in evs)
+    avg_ms = sum(kms) / len(kms)
+    med_ms = kms[len(kms) // 2]
+
+    unknowns = n * N
+    alg_bytes = 5 * n * esize * N
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    workload = f"{args.method} {args.dtype} {N} systems x {n} unknowns"
+    traffic = traffic_from_profiles(workload)
+    out = {
+        "metric": METRIC,
+        "value": unknowns * args.steps / wall,
+        "unit": "unknowns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic (seeded device generator: a,c~U[-1,1], b=|a|+|c|+U[1,2], d~U[-1,1])",
+        "config": {"workload": workload, "method": args.method, "systems": N,
+                   "system_size": n, "rest": plan["rest"], "lanes": plan["lanes"],
+                   "per_lane": plan["per_lane"], "family": plan["family"]},
+        "kernel_ms": {"avg": avg_ms, "median": med_ms, "min": kms[0]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": traffic},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args, x.cpu().numpy(), n, N)
+    h.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

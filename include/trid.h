@@ -1,0 +1,197 @@
+/*
+ * trid.h -- C ABI of the MI355X (gfx950) batched tridiagonal solver library
+ * (libtrid.so).
+ *
+ * A batch is numSystems independent systems T x = d of systemSize unknowns,
+ * stored system-major in five separate arrays: element j of system s is at
+ * s*systemSize + j in a (sub-diagonal), b (diagonal), c (super-diagonal),
+ * d (right-hand side) and x (solution). a[first] and c[last] of each system
+ * are ignored (treated as 0). No pivoting is done: the methods are meant for
+ * diagonally dominant systems, as in the reference.
+ *
+ * Every function returns an int status (trid_status); none of them exits
+ * the process.
+ *
+ * Reference: zhangyaobit/tridiagonalsolvers, directory tridiagonalsolvers/.
+ * The reference declares its solver API only as forward declarations in
+ * test_tridiagonal.cpp:17-26 (C++ linkage, fixed element type T, host
+ * pointers, void return).
+ */
+#ifndef TRID_H
+#define TRID_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* status, methods, options                                                 */
+/* ---------------------------------------------------------------------- */
+
+typedef enum trid_status {
+    TRID_OK = 0,
+    TRID_EINVAL = 1,  /* bad size, method parameter or null pointer */
+    TRID_ENOMEM = 2,  /* device or host allocation failed */
+    TRID_EHIP = 3,    /* a HIP runtime call or kernel launch failed */
+    TRID_ENOTSUP = 4, /* valid request that this build cannot run */
+    TRID_ENODEV = 5   /* no usable gfx950 device */
+} trid_status;
+
+typedef enum trid_method {
+    TRID_CR = 0,        /* cyclic reduction               (cr_kernel.cu)      */
+    TRID_PCR = 1,       /* parallel cyclic reduction      (pcr_kernel.cu)     */
+    TRID_CRPCR = 2,     /* CR down to `rest`, PCR, CR back (crpcr_kernel.cu)  */
+    TRID_CRNBC = 3,     /* CR, LDS-resident, padded LDS   (crNBC.cu)          */
+    TRID_CRPCRNBC = 4,  /* CR-PCR, LDS-resident, padded   (crNBC.cu)          */
+    TRID_CRNBC32 = 5,   /* CR, LDS-resident, unpadded     (crNBC32banks.cu)   */
+    TRID_PCRGAUSS = 6,  /* PCR to G subsystems, Thomas    (pcrGauss_*.cu)     */
+    TRID_AUTO = 7       /* library chooses (plan / autotune table)            */
+} trid_method;
+
+/* Per-call options. Zero-initialise and set what you need; 0 = default. */
+typedef struct trid_opts {
+    int32_t rest;       /* CRPCR/CRPCRNBC: size of the system handed to PCR,
+                           power of two in [2, n/2] (crpcr_app.cu:5)          */
+    int32_t gaussian;   /* PCRGAUSS: number G of Thomas subsystems, power of
+                           two; the PCR depth is exactly log2(G)
+                           (pcrGauss_app.cu:164)                              */
+    int32_t lds_pad;    /* LDS-resident kernels: one pad element per lds_pad
+                           elements (power of two), -1 = no padding           */
+    int32_t lanes;      /* register kernels: lanes per system (power of two,
+                           2..64), 0 = plan                                   */
+    int32_t per_lane;   /* register kernels: unknowns per lane (1..16)        */
+    int32_t io_mode;    /* register kernels, dense batches: 0 = plan,
+                           1 per-lane non-temporal, 2 per-lane cached,
+                           3 wave-coalesced through LDS, 4 as 3 non-temporal  */
+    int32_t flags;      /* reserved, must be 0                                */
+} trid_opts;
+
+/* Human-readable text for a status code (static storage). */
+const char* trid_status_string(int status);
+
+/* Status of the most recent call made by this host thread through any entry
+ * point (used by the void drop-ins of trid_compat.h). */
+int trid_last_status(void);
+
+/* Number of gfx950 devices visible, or a negative status. */
+int trid_device_count(void);
+
+/* ---------------------------------------------------------------------- */
+/* Host-pointer drop-ins: one per reference wrapper                         */
+/* ---------------------------------------------------------------------- */
+/*
+ * All pointers are host memory. Each call copies a,b,c,d to the device,
+ * runs exactly one solve on device 0 (current device) and copies x back.
+ * a,b,c,d are not modified. Unlike the reference wrappers there is no
+ * built-in 50-launch timing loop and no printf.
+ */
+
+/* replaces void cr(T*,T*,T*,T*,T*,int,int)           cr_app.cu:4         */
+int trid_cr_f32(const float* a, const float* b, const float* c, const float* d,
+                float* x, int systemSize, int numSystems);
+int trid_cr_f64(const double* a, const double* b, const double* c, const double* d,
+                double* x, int systemSize, int numSystems);
+
+/* replaces void pcr(T*,T*,T*,T*,T*,int,int)          pcr_app.cu:5        */
+int trid_pcr_f32(const float* a, const float* b, const float* c, const float* d,
+                 float* x, int systemSize, int numSystems);
+int trid_pcr_f64(const double* a, const double* b, const double* c, const double* d,
+                 double* x, int systemSize, int numSystems);
+
+/* replaces void crpcr(T*,...,int,int,int restSystemSize) crpcr_app.cu:5  */
+int trid_crpcr_f32(const float* a, const float* b, const float* c, const float* d,
+                   float* x, int systemSize, int numSystems, int restSystemSize);
+int trid_crpcr_f64(const double* a, const double* b, const double* c, const double* d,
+                   double* x, int systemSize, int numSystems, int restSystemSize);
+
+/* replaces void crNBC(T*,T*,T*,T*,T*,int,int)        crNBC.cu:123        */
+int trid_crnbc_f32(const float* a, const float* b, const float* c, const float* d,
+                   float* x, int systemSize, int numSystems);
+int trid_crnbc_f64(const double* a, const double* b, const double* c, const double* d,
+                   double* x, int systemSize, int numSystems);
+
+/* replaces void crpcrNBC(T*,...,int,int,int restSystemSize) crNBC.cu:344 */
+int trid_crpcrnbc_f32(const float* a, const float* b, const float* c, const float* d,
+                      float* x, int systemSize, int numSystems, int restSystemSize);
+int trid_crpcrnbc_f64(const double* a, const double* b, const double* c, const double* d,
+                      double* x, int systemSize, int numSystems, int restSystemSize);
+
+/* replaces void crNBC32(T*,T*,T*,T*,T*,int,int)      crNBC32banks.cu:123 */
+int trid_crnbc32_f32(const float* a, const float* b, const float* c, const float* d,
+                     float* x, int systemSize, int numSystems);
+int trid_crnbc32_f64(const double* a, const double* b, const double* c, const double* d,
+                     double* x, int systemSize, int numSystems);
+
+/* replaces void pcrGaussWrapper(T*,...,int,int,int gaussian) pcrGauss_app.cu:164 */
+int trid_pcrgauss_f32(const float* a, const float* b, const float* c, const float* d,
+                      float* x, int systemSize, int numSystems, int gaussian);
+int trid_pcrgauss_f64(const double* a, const double* b, const double* c, const double* d,
+                      double* x, int systemSize, int numSystems, int gaussian);
+
+/* ---------------------------------------------------------------------- */
+/* Device-pointer, stream-ordered API                                       */
+/* ---------------------------------------------------------------------- */
+/*
+ * No counterpart in the reference (its wrappers allocate and copy on every
+ * call, cr_app.cu:20-65). All array pointers are device memory; `stream` is
+ * a hipStream_t (NULL = default stream). The solve is enqueued and the call
+ * returns without synchronising. A handle owns the workspace some methods
+ * need; use one handle from one host thread at a time.
+ */
+typedef struct trid_context* trid_handle;
+
+int trid_create(trid_handle* handle);
+int trid_destroy(trid_handle handle);
+
+int trid_solve_f32(trid_handle handle, int method,
+                   const float* a, const float* b, const float* c, const float* d,
+                   float* x, int64_t systemSize, int64_t numSystems,
+                   const trid_opts* opts, void* stream);
+int trid_solve_f64(trid_handle handle, int method,
+                   const double* a, const double* b, const double* c, const double* d,
+                   double* x, int64_t systemSize, int64_t numSystems,
+                   const trid_opts* opts, void* stream);
+
+/*
+ * Seeded synthetic batch written straight into device memory (same values as
+ * the reference-shaped generator kinds below, bit for bit, on every run):
+ *   kind 0  a,c ~ U[-1,1], b = |a|+|c|+U[1,2],      d ~ U[-1,1]
+ *   kind 1  a,c ~ U[-1,1], b = |a|+|c|+U[0.01,0.1], d ~ U[-1,1]
+ *   kind 2  b = 8+U, a = 3+U, c = 2+U, d = 5+U (tridiagonal_gold.h:81-94)
+ * with a[first] = c[last] = 0 in every system.
+ */
+int trid_generate_f32(int kind, uint64_t seed, float* a, float* b, float* c, float* d,
+                      int64_t systemSize, int64_t numSystems, void* stream);
+int trid_generate_f64(int kind, uint64_t seed, double* a, double* b, double* c, double* d,
+                      int64_t systemSize, int64_t numSystems, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Plan introspection (host only; no device needed)                         */
+/* ---------------------------------------------------------------------- */
+/*
+ * No counterpart in the reference, whose launch shapes are implied by
+ * systemSize (one block of n/2 or n threads per system, cr_app.cu:37-38).
+ * Validates a request exactly as trid_solve_* would and reports the kernel
+ * family and launch shape it would use: family 0 = register-resident
+ * CR/CR-PCR/PCR, 1 = LDS-resident CR/CR-PCR with pad map, 2 = PCR + Thomas.
+ */
+typedef struct trid_plan_info {
+    int32_t family;
+    int32_t per_lane;    /* family 0: unknowns per lane             */
+    int32_t lanes;       /* family 0: lanes per system              */
+    int32_t rest;        /* families 0/1: PCR system size (2 = CR)  */
+    int32_t lds_pad;     /* family 1: pad period, 0 = unpadded      */
+    int32_t gaussian;    /* family 2: Thomas subsystems             */
+    int64_t padded_size; /* systemSize rounded up to a power of two */
+} trid_plan_info;
+
+int trid_plan_query(int method, int64_t systemSize, int64_t numSystems, int elemSize,
+                    const trid_opts* opts, trid_plan_info* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TRID_H */

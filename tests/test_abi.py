@@ -1,0 +1,108 @@
+"""CPU tests of the C ABI: the library loads, exports every symbol that
+include/*.h declares, and validates arguments before touching a device
+(no solve is run here)."""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        # only the extern "C" header declares C symbols; compat headers hold
+        # inline C++ wrappers
+        if 'extern "C"' not in text:
+            continue
+        for m in re.finditer(r"\b(trid_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol(trid):
+    names = declared_symbols()
+    assert len(names) >= 24
+    lib = trid.lib()
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_status_strings(trid):
+    lib = trid.lib()
+    for s in range(6):
+        assert lib.trid_status_string(s).decode().startswith("TRID_")
+    assert b"unknown" in lib.trid_status_string(99)
+
+
+def test_host_dropins_reject_bad_arguments_without_a_device(trid):
+    lib = trid.lib()
+    a = np.ones(16, dtype=np.float32)
+    p = ctypes.c_void_p(a.ctypes.data)
+    null = ctypes.c_void_p(0)
+    EINVAL = 1
+    assert lib.trid_cr_f32(null, p, p, p, p, 16, 1) == EINVAL
+    assert lib.trid_last_status() == EINVAL
+    assert lib.trid_cr_f32(p, p, p, p, p, 1, 1) == EINVAL        # n < 2
+    assert lib.trid_cr_f32(p, p, p, p, p, 16, -1) == EINVAL      # N < 0
+    assert lib.trid_crpcr_f32(p, p, p, p, p, 16, 1, 3) == EINVAL   # rest not 2^k
+    assert lib.trid_crpcr_f32(p, p, p, p, p, 16, 1, 16) == EINVAL  # rest > n/2
+    assert lib.trid_crpcr_f32(p, p, p, p, p, 16, 1, 0) == EINVAL
+    assert lib.trid_pcrgauss_f32(p, p, p, p, p, 16, 1, 3) == EINVAL
+    assert lib.trid_pcrgauss_f32(p, p, p, p, p, 16, 1, 16) == EINVAL
+    assert lib.trid_crpcrnbc_f64(p, p, p, p, p, 16, 1, 5) == EINVAL
+
+
+def test_device_api_rejects_bad_arguments_without_a_device(trid):
+    h = trid.Handle()
+    with pytest.raises(trid.TridError) as e:
+        h.solve("crpcr", 0, 0, 0, 0, 0, 1024, 1)
+    assert e.value.status == 1
+    with pytest.raises(trid.TridError):
+        h.solve("crpcr", 16, 16, 16, 16, 16, 1, 1)
+    with pytest.raises(trid.TridError):
+        h.solve("crpcr", 16, 16, 16, 16, 16, 64, 1, opts=trid.make_opts(flags=1))
+    with pytest.raises(trid.TridError):
+        h.solve(42, 16, 16, 16, 16, 16, 64, 1)
+    h.close()
+
+
+def test_plan_query_register_family(trid):
+    q = trid.plan_query("crpcr", 1024, 65536, 4)
+    assert q["family"] == 0 and q["per_lane"] * q["lanes"] == 1024
+    assert q["per_lane"] == 16 and q["lanes"] == 64 and q["rest"] == 64
+    q = trid.plan_query("cr", 512, 1, 4)
+    assert q["rest"] == 2 and q["per_lane"] * q["lanes"] == 512
+    q = trid.plan_query("pcr", 256, 1, 8)
+    assert q["rest"] == 256
+    q = trid.plan_query("crpcr", 1000, 3, 4)   # padded to 1024
+    assert q["padded_size"] == 1024
+    q = trid.plan_query("crpcr", 2, 1, 4)
+    assert q["lanes"] == 2 and q["per_lane"] == 1 and q["rest"] == 2
+    q = trid.plan_query("crpcr", 1024, 1, 4, trid.make_opts(rest=256))
+    assert q["rest"] == 256
+    q = trid.plan_query("crpcr", 64, 1, 4, trid.make_opts(lanes=8, per_lane=8))
+    assert q["lanes"] == 8 and q["per_lane"] == 8
+    with pytest.raises(trid.TridError):
+        trid.plan_query("crpcr", 64, 1, 4, trid.make_opts(lanes=8, per_lane=4))
+    with pytest.raises(trid.TridError):
+        trid.plan_query("crpcr", 64, 1, 4, trid.make_opts(rest=64))
+    with pytest.raises(trid.TridError):
+        trid.plan_query("crpcr", 64, 1, 3)
+
+
+def test_plan_query_nbc_and_gauss(trid):
+    q = trid.plan_query("crnbc", 1024, 1, 4)
+    assert q["family"] == 1 and q["lds_pad"] > 0
+    q = trid.plan_query("crnbc32", 1024, 1, 4)
+    assert q["family"] == 1 and q["lds_pad"] == 0
+    q = trid.plan_query("crpcrnbc", 1024, 1, 4, trid.make_opts(rest=128))
+    assert q["family"] == 1 and q["rest"] == 128
+    q = trid.plan_query("pcrgauss", 1024, 1, 4, trid.make_opts(gaussian=64))
+    assert q["family"] == 2 and q["gaussian"] == 64

@@ -1,0 +1,110 @@
+"""ctypes binding of libtrid.so (the C ABI declared in include/trid.h).
+
+The library is loaded from this package directory. There is no fallback:
+if libtrid.so is missing or fails to load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtrid.so")
+
+# trid_status
+OK, EINVAL, ENOMEM, EHIP, ENOTSUP, ENODEV = range(6)
+# trid_method
+CR, PCR, CRPCR, CRNBC, CRPCRNBC, CRNBC32, PCRGAUSS, AUTO = range(8)
+METHODS = {"cr": CR, "pcr": PCR, "crpcr": CRPCR, "crnbc": CRNBC, "crpcrnbc": CRPCRNBC,
+           "crnbc32": CRNBC32, "pcrgauss": PCRGAUSS, "auto": AUTO}
+
+# symbols declared in include/trid.h, checked at load time
+EXPORTS = (
+    "trid_status_string", "trid_last_status", "trid_device_count",
+    "trid_cr_f32", "trid_cr_f64", "trid_pcr_f32", "trid_pcr_f64",
+    "trid_crpcr_f32", "trid_crpcr_f64", "trid_crnbc_f32", "trid_crnbc_f64",
+    "trid_crpcrnbc_f32", "trid_crpcrnbc_f64", "trid_crnbc32_f32", "trid_crnbc32_f64",
+    "trid_pcrgauss_f32", "trid_pcrgauss_f64",
+    "trid_create", "trid_destroy", "trid_solve_f32", "trid_solve_f64",
+    "trid_generate_f32", "trid_generate_f64", "trid_plan_query",
+)
+
+
+class TridError(RuntimeError):
+    def __init__(self, status: int, where: str = ""):
+        self.status = int(status)
+        msg = string(status) if _lib is not None else f"status {status}"
+        super().__init__(f"{where}: {msg}" if where else msg)
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("rest", ctypes.c_int32), ("gaussian", ctypes.c_int32),
+                ("lds_pad", ctypes.c_int32), ("lanes", ctypes.c_int32),
+                ("per_lane", ctypes.c_int32), ("io_mode", ctypes.c_int32),
+                ("flags", ctypes.c_int32)]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [("family", ctypes.c_int32), ("per_lane", ctypes.c_int32),
+                ("lanes", ctypes.c_int32), ("rest", ctypes.c_int32),
+                ("lds_pad", ctypes.c_int32), ("gaussian", ctypes.c_int32),
+                ("padded_size", ctypes.c_int64)]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libtrid.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run __graft_entry__.build() "
+            "(hipcc --offload-arch=gfx950). There is no CPU fallback.")
+    L = ctypes.CDLL(LIB_PATH)
+    missing = [s for s in EXPORTS if not hasattr(L, s)]
+    if missing:
+        raise ImportError(f"libtrid.so lacks symbols: {missing}")
+    P = ctypes.c_void_p
+    i32, i64 = ctypes.c_int, ctypes.c_int64
+    L.trid_status_string.argtypes = [i32]
+    L.trid_status_string.restype = ctypes.c_char_p
+    L.trid_last_status.restype = i32
+    L.trid_device_count.restype = i32
+    for name in ("cr", "pcr", "crnbc", "crnbc32"):
+        for t in ("f32", "f64"):
+            f = getattr(L, f"trid_{name}_{t}")
+            f.argtypes = [P, P, P, P, P, i32, i32]
+            f.restype = i32
+    for name in ("crpcr", "crpcrnbc", "pcrgauss"):
+        for t in ("f32", "f64"):
+            f = getattr(L, f"trid_{name}_{t}")
+            f.argtypes = [P, P, P, P, P, i32, i32, i32]
+            f.restype = i32
+    L.trid_create.argtypes = [ctypes.POINTER(P)]
+    L.trid_create.restype = i32
+    L.trid_destroy.argtypes = [P]
+    L.trid_destroy.restype = i32
+    for t in ("f32", "f64"):
+        f = getattr(L, f"trid_solve_{t}")
+        f.argtypes = [P, i32, P, P, P, P, P, i64, i64, ctypes.POINTER(Opts), P]
+        f.restype = i32
+        g = getattr(L, f"trid_generate_{t}")
+        g.argtypes = [i32, ctypes.c_uint64, P, P, P, P, i64, i64, P]
+        g.restype = i32
+    L.trid_plan_query.argtypes = [i32, i64, i64, i32, ctypes.POINTER(Opts),
+                                  ctypes.POINTER(PlanInfo)]
+    L.trid_plan_query.restype = i32
+    _lib = L
+    return L
+
+
+def string(status: int) -> str:
+    return lib().trid_status_string(int(status)).decode()
+
+
+def check(status: int, where: str = "") -> None:
+    if status != OK:
+        raise TridError(status, where)

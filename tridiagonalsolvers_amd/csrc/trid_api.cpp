@@ -1,0 +1,308 @@
+// trid_api.cpp -- the C ABI of libtrid.so (declared in include/trid.h).
+//
+// Validation, planning (which kernel family and launch shape serves a
+// request), the stream-ordered device API and the host-pointer drop-ins that
+// replace the reference wrappers (cr_app.cu:4, pcr_app.cu:5, crpcr_app.cu:5,
+// crNBC.cu:123,344, crNBC32banks.cu:123, pcrGauss_app.cu:164).
+//
+// Every entry point returns a trid_status and records it as the calling
+// thread's last status; nothing here exits the process or prints.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <new>
+
+#include "trid.h"
+#include "trid_launch.hpp"
+#include "trid_plan.hpp"
+
+namespace {
+
+thread_local int g_last_status = TRID_OK;
+
+int set_status(int s)
+{
+    g_last_status = s;
+    return s;
+}
+
+int from_hip(hipError_t e)
+{
+    if (e == hipSuccess) return TRID_OK;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return TRID_ENOMEM;
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return TRID_ENODEV;
+    return TRID_EHIP;
+}
+
+// Only gfx950 code objects are built; check the current device once.
+std::mutex g_dev_mu;
+int g_dev_checked[64];  // 0 unknown, 1 ok, 2 not gfx950
+
+int check_device()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return TRID_ENODEV;
+    if (dev < 0 || dev >= 64) return TRID_ENODEV;
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    if (g_dev_checked[dev] == 0) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return TRID_ENODEV;
+        g_dev_checked[dev] = (std::strncmp(prop.gcnArchName, "gfx950", 6) == 0) ? 1 : 2;
+    }
+    return g_dev_checked[dev] == 1 ? TRID_OK : TRID_ENODEV;
+}
+
+bool aligned16(const void* p)
+{
+    return (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
+}
+
+template <typename T>
+int solve_impl(trid_handle h, int method, const T* a, const T* b, const T* c, const T* d, T* x,
+               int64_t n, int64_t N, const trid_opts* opts, hipStream_t stream)
+{
+    (void)h;
+    trid_opts o{};
+    if (opts) o = *opts;
+    if (!a || !b || !c || !d || !x) return TRID_EINVAL;
+    if (n < 2 || N < 0) return TRID_EINVAL;
+    if (o.flags != 0) return TRID_EINVAL;
+    if (N == 0) return TRID_OK;
+
+    trid::Plan p;
+    int st = trid::make_plan(method, n, N, (int)sizeof(T), o, &p);
+    if (st != TRID_OK) return st;
+    st = check_device();
+    if (st != TRID_OK) return st;
+
+    switch (p.family) {
+        case trid::Family::Reg: {
+            trid::RegArgs<T> g{};
+            g.a = a;
+            g.b = b;
+            g.c = c;
+            g.d = d;
+            g.x = x;
+            g.n = n;
+            g.N = N;
+            g.M = p.M;
+            g.L = p.L;
+            g.rest = p.rest;
+            g.io = p.io;
+            g.vec = (p.dense && aligned16(a) && aligned16(b) && aligned16(c) && aligned16(d) &&
+                     aligned16(x))
+                        ? 1
+                        : 0;
+            g.stream = stream;
+            return from_hip(trid::launch_reg_crpcr<T>(g));
+        }
+        default:
+            return TRID_ENOTSUP;
+    }
+}
+
+// Host-pointer drop-in: copy in, one solve on the null stream, copy out.
+template <typename T>
+int host_solve(int method, const T* a, const T* b, const T* c, const T* d, T* x, int n, int N,
+               int rest, int gaussian)
+{
+    if (!a || !b || !c || !d || !x) return TRID_EINVAL;
+    if (n < 2 || N < 0) return TRID_EINVAL;
+    if (N == 0) return TRID_OK;
+    trid_opts o{};
+    o.rest = rest;
+    o.gaussian = gaussian;
+    trid::Plan p;
+    int st = trid::make_plan(method, n, N, (int)sizeof(T), o, &p);
+    if (st != TRID_OK) return st;
+    st = check_device();
+    if (st != TRID_OK) return st;
+
+    const size_t elems = (size_t)n * (size_t)N;
+    const size_t bytes = elems * sizeof(T);
+    // one allocation, five 256-byte aligned arrays
+    const size_t slot = (bytes + 255) & ~(size_t)255;
+    char* base = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&base), slot * 5);
+    if (e != hipSuccess) return from_hip(e);
+    T* da = reinterpret_cast<T*>(base);
+    T* db = reinterpret_cast<T*>(base + slot);
+    T* dc = reinterpret_cast<T*>(base + 2 * slot);
+    T* dd = reinterpret_cast<T*>(base + 3 * slot);
+    T* dx = reinterpret_cast<T*>(base + 4 * slot);
+    int rc = TRID_OK;
+    if ((e = hipMemcpy(da, a, bytes, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(db, b, bytes, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(dc, c, bytes, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(dd, d, bytes, hipMemcpyHostToDevice)) != hipSuccess) {
+        rc = from_hip(e);
+    }
+    if (rc == TRID_OK) rc = solve_impl<T>(nullptr, method, da, db, dc, dd, dx, n, N, &o, nullptr);
+    if (rc == TRID_OK) {
+        e = hipStreamSynchronize(nullptr);
+        if (e == hipSuccess) e = hipMemcpy(x, dx, bytes, hipMemcpyDeviceToHost);
+        rc = from_hip(e);
+    }
+    (void)hipFree(base);
+    return rc;
+}
+
+}  // namespace
+
+struct trid_context {
+    int device;
+};
+
+extern "C" {
+
+const char* trid_status_string(int status)
+{
+    switch (status) {
+        case TRID_OK: return "TRID_OK: success";
+        case TRID_EINVAL: return "TRID_EINVAL: invalid argument";
+        case TRID_ENOMEM: return "TRID_ENOMEM: out of memory";
+        case TRID_EHIP: return "TRID_EHIP: HIP runtime error";
+        case TRID_ENOTSUP: return "TRID_ENOTSUP: not supported by this build";
+        case TRID_ENODEV: return "TRID_ENODEV: no gfx950 device";
+        default: return "unknown trid status";
+    }
+}
+
+int trid_last_status(void) { return g_last_status; }
+
+int trid_device_count(void)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess) return 0;
+    int good = 0;
+    for (int i = 0; i < count; ++i) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, i) == hipSuccess &&
+            std::strncmp(prop.gcnArchName, "gfx950", 6) == 0)
+            ++good;
+    }
+    return good;
+}
+
+int trid_create(trid_handle* handle)
+{
+    if (!handle) return set_status(TRID_EINVAL);
+    trid_context* h = new (std::nothrow) trid_context();
+    if (!h) return set_status(TRID_ENOMEM);
+    h->device = 0;
+    (void)hipGetDevice(&h->device);
+    *handle = h;
+    return set_status(TRID_OK);
+}
+
+int trid_destroy(trid_handle handle)
+{
+    if (!handle) return set_status(TRID_EINVAL);
+    delete handle;
+    return set_status(TRID_OK);
+}
+
+int trid_solve_f32(trid_handle handle, int method, const float* a, const float* b,
+                   const float* c, const float* d, float* x, int64_t systemSize,
+                   int64_t numSystems, const trid_opts* opts, void* stream)
+{
+    return set_status(solve_impl<float>(handle, method, a, b, c, d, x, systemSize, numSystems,
+                                        opts, static_cast<hipStream_t>(stream)));
+}
+
+int trid_solve_f64(trid_handle handle, int method, const double* a, const double* b,
+                   const double* c, const double* d, double* x, int64_t systemSize,
+                   int64_t numSystems, const trid_opts* opts, void* stream)
+{
+    return set_status(solve_impl<double>(handle, method, a, b, c, d, x, systemSize, numSystems,
+                                         opts, static_cast<hipStream_t>(stream)));
+}
+
+int trid_generate_f32(int kind, uint64_t seed, float* a, float* b, float* c, float* d,
+                      int64_t systemSize, int64_t numSystems, void* stream)
+{
+    if (!a || !b || !c || !d || systemSize < 1 || numSystems < 0 || kind < 0 || kind > 2)
+        return set_status(TRID_EINVAL);
+    int st = check_device();
+    if (st != TRID_OK) return set_status(st);
+    return set_status(from_hip(trid::launch_generate<float>(
+        kind, seed, a, b, c, d, systemSize, numSystems, static_cast<hipStream_t>(stream))));
+}
+
+int trid_generate_f64(int kind, uint64_t seed, double* a, double* b, double* c, double* d,
+                      int64_t systemSize, int64_t numSystems, void* stream)
+{
+    if (!a || !b || !c || !d || systemSize < 1 || numSystems < 0 || kind < 0 || kind > 2)
+        return set_status(TRID_EINVAL);
+    int st = check_device();
+    if (st != TRID_OK) return set_status(st);
+    return set_status(from_hip(trid::launch_generate<double>(
+        kind, seed, a, b, c, d, systemSize, numSystems, static_cast<hipStream_t>(stream))));
+}
+
+#define TRID_DROPIN(NAME, METHOD)                                                              \
+    int NAME##_f32(const float* a, const float* b, const float* c, const float* d, float* x,   \
+                   int systemSize, int numSystems)                                             \
+    {                                                                                          \
+        return set_status(host_solve<float>(METHOD, a, b, c, d, x, systemSize, numSystems, 0,  \
+                                            0));                                               \
+    }                                                                                          \
+    int NAME##_f64(const double* a, const double* b, const double* c, const double* d,         \
+                   double* x, int systemSize, int numSystems)                                  \
+    {                                                                                          \
+        return set_status(host_solve<double>(METHOD, a, b, c, d, x, systemSize, numSystems, 0, \
+                                             0));                                              \
+    }
+
+TRID_DROPIN(trid_cr, TRID_CR)
+TRID_DROPIN(trid_pcr, TRID_PCR)
+TRID_DROPIN(trid_crnbc, TRID_CRNBC)
+TRID_DROPIN(trid_crnbc32, TRID_CRNBC32)
+#undef TRID_DROPIN
+
+int trid_crpcr_f32(const float* a, const float* b, const float* c, const float* d, float* x,
+                   int systemSize, int numSystems, int restSystemSize)
+{
+    if (restSystemSize <= 0) return set_status(TRID_EINVAL);
+    return set_status(host_solve<float>(TRID_CRPCR, a, b, c, d, x, systemSize, numSystems,
+                                        restSystemSize, 0));
+}
+int trid_crpcr_f64(const double* a, const double* b, const double* c, const double* d,
+                   double* x, int systemSize, int numSystems, int restSystemSize)
+{
+    if (restSystemSize <= 0) return set_status(TRID_EINVAL);
+    return set_status(host_solve<double>(TRID_CRPCR, a, b, c, d, x, systemSize, numSystems,
+                                         restSystemSize, 0));
+}
+int trid_crpcrnbc_f32(const float* a, const float* b, const float* c, const float* d,
+                      float* x, int systemSize, int numSystems, int restSystemSize)
+{
+    if (restSystemSize <= 0) return set_status(TRID_EINVAL);
+    return set_status(host_solve<float>(TRID_CRPCRNBC, a, b, c, d, x, systemSize, numSystems,
+                                        restSystemSize, 0));
+}
+int trid_crpcrnbc_f64(const double* a, const double* b, const double* c, const double* d,
+                      double* x, int systemSize, int numSystems, int restSystemSize)
+{
+    if (restSystemSize <= 0) return set_status(TRID_EINVAL);
+    return set_status(host_solve<double>(TRID_CRPCRNBC, a, b, c, d, x, systemSize, numSystems,
+                                         restSystemSize, 0));
+}
+int trid_pcrgauss_f32(const float* a, const float* b, const float* c, const float* d, float* x,
+                      int systemSize, int numSystems, int gaussian)
+{
+    if (gaussian <= 0) return set_status(TRID_EINVAL);
+    return set_status(host_solve<float>(TRID_PCRGAUSS, a, b, c, d, x, systemSize, numSystems,
+                                        0, gaussian));
+}
+int trid_pcrgauss_f64(const double* a, const double* b, const double* c, const double* d,
+                      double* x, int systemSize, int numSystems, int gaussian)
+{
+    if (gaussian <= 0) return set_status(TRID_EINVAL);
+    return set_status(host_solve<double>(TRID_PCRGAUSS, a, b, c, d, x, systemSize, numSystems,
+                                         0, gaussian));
+}
+
+}  // extern "C"

@@ -1,0 +1,122 @@
+// trid_device.hpp -- device-side building blocks shared by the gfx950 kernels.
+//
+// One tridiagonal equation i reads  a_i x_{i-1} + b_i x_i + c_i x_{i+1} = d_i.
+// After reduction levels it couples x_i to x_{i-delta} and x_{i+delta} with
+// the same four coefficients. A neighbour that lies outside the system is an
+// identity row (a = c = d = 0, b = 1): it contributes nothing. This is what
+// the reference obtains by clamping neighbour indices and relying on
+// a[0] = c[n-1] = 0 (cr_kernel.cu:52, pcr_kernel.cu:33-37); an explicit
+// identity row also makes padding to a power of two exact.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace trid {
+
+constexpr int kWave = 64;
+
+template <typename T>
+struct Eq {
+    T a, b, c, d;
+};
+
+template <typename T>
+__device__ __forceinline__ Eq<T> identity_eq()
+{
+    return Eq<T>{T(0), T(1), T(0), T(0)};
+}
+
+// Quotient p / q. FP32 uses the hardware reciprocal (1 ulp) and one
+// multiply; FP64 uses IEEE division.
+__device__ __forceinline__ float tdiv(float p, float q)
+{
+    return p * __builtin_amdgcn_rcpf(q);
+}
+__device__ __forceinline__ double tdiv(double p, double q)
+{
+    return p / q;
+}
+
+// One CR/PCR reduction step (cr_kernel.cu:53-58, pcr_kernel.cu:39-56):
+//   k1 = a_i / b_L, k2 = c_i / b_R
+//   b_i <- b_i - c_L k1 - a_R k2,  d_i <- d_i - d_L k1 - d_R k2
+//   a_i <- -a_L k1,                c_i <- -c_R k2
+template <typename T>
+__device__ __forceinline__ Eq<T> reduce_eq(const Eq<T>& e, const Eq<T>& L, const Eq<T>& R)
+{
+    T k1 = tdiv(e.a, L.b);
+    T k2 = tdiv(e.c, R.b);
+    Eq<T> r;
+    r.b = e.b - L.c * k1 - R.a * k2;
+    r.d = e.d - L.d * k1 - R.d * k2;
+    r.a = -L.a * k1;
+    r.c = -R.c * k2;
+    return r;
+}
+
+// 2x2 solve by Cramer's rule (cr_kernel.cu:65-69) for the pair
+// (e1 at the lower index, e2 at the upper index).
+template <typename T>
+__device__ __forceinline__ void solve2(const Eq<T>& e1, const Eq<T>& e2, T& x1, T& x2)
+{
+    T det = e2.b * e1.b - e1.c * e2.a;
+    x1 = tdiv(e2.b * e1.d - e1.c * e2.d, det);
+    x2 = tdiv(e2.d * e1.b - e1.d * e2.a, det);
+}
+
+// Back-substitution of one unknown (cr_kernel.cu:80-84):
+//   x_i = (d_i - a_i x_{i-delta} - c_i x_{i+delta}) / b_i
+template <typename T>
+__device__ __forceinline__ T backsub(const Eq<T>& e, T xl, T xr)
+{
+    return tdiv(e.d - e.a * xl - e.c * xr, e.b);
+}
+
+// ---------------------------------------------------------------------------
+// cross-lane moves (ds_bpermute through the LDS crossbar; no LDS storage)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float lane_read(float v, int src)
+{
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
+}
+__device__ __forceinline__ double lane_read(double v, int src)
+{
+    int lo = __builtin_amdgcn_ds_bpermute(src << 2, __double2loint(v));
+    int hi = __builtin_amdgcn_ds_bpermute(src << 2, __double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+// Equation held by lane `src` (all lanes must execute this), replaced by the
+// identity row when `valid` is false for the reading lane.
+template <typename T>
+__device__ __forceinline__ Eq<T> lane_read_eq(const Eq<T>& e, int src, bool valid)
+{
+    src &= (kWave - 1);
+    Eq<T> r;
+    r.a = lane_read(e.a, src);
+    r.b = lane_read(e.b, src);
+    r.c = lane_read(e.c, src);
+    r.d = lane_read(e.d, src);
+    if (!valid) r = identity_eq<T>();
+    return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T lane_read_or_zero(T v, int src, bool valid)
+{
+    T r = lane_read(v, src & (kWave - 1));
+    return valid ? r : T(0);
+}
+
+template <int V>
+struct Log2 {
+    static constexpr int value = 1 + Log2<V / 2>::value;
+};
+template <>
+struct Log2<1> {
+    static constexpr int value = 0;
+};
+
+}  // namespace trid

@@ -1,0 +1,166 @@
+// trid_plan.cpp -- request validation and kernel planning (host only, no HIP
+// calls, so the rules are testable without a GPU through trid_plan_query).
+#include "trid_plan.hpp"
+
+namespace trid {
+
+namespace {
+
+bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
+
+int64_t next_pow2(int64_t v)
+{
+    int64_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// Largest LDS footprint one workgroup may use on gfx950 (160 KiB).
+constexpr int64_t kLdsBytes = 160 * 1024;
+
+// Dense-path I/O mode of the register kernels when the caller does not pick
+// one (RegArgs::io numbering).
+constexpr int kDefaultIo = 0;
+
+// Register family: unknowns per lane M and lanes per system L with
+// L * M = n_pad, 2 <= L <= 64, M <= 16.
+int plan_reg(int64_t n_pad, int elem_size, const trid_opts& o, Plan* p)
+{
+    int M, L;
+    if (o.per_lane != 0 || o.lanes != 0) {
+        if (o.per_lane == 0 || o.lanes == 0) return TRID_EINVAL;
+        M = o.per_lane;
+        L = o.lanes;
+        if (!is_pow2(M) || M > 16 || !is_pow2(L) || L < 2 || L > 64) return TRID_EINVAL;
+        if ((int64_t)M * L != n_pad) return TRID_EINVAL;
+    } else {
+        if (n_pad > 64 * 16) return TRID_ENOTSUP;
+        // Up to 16 unknowns per lane, at least two lanes per system.
+        int64_t m = n_pad / 2;
+        if (m > 16) m = 16;
+        if (n_pad / m > 64) m = n_pad / 64;
+        M = (int)m;
+        L = (int)(n_pad / m);
+        (void)elem_size;
+    }
+    if (o.io_mode < 0 || o.io_mode > 4) return TRID_EINVAL;
+    p->family = Family::Reg;
+    p->M = M;
+    p->L = L;
+    p->io = o.io_mode > 0 ? o.io_mode - 1 : kDefaultIo;
+    return TRID_OK;
+}
+
+// LDS-resident family: four arrays of n_pad elements (x overwrites d) plus
+// one pad element per `pad` elements must fit in one workgroup's LDS.
+int plan_lds(int64_t n_pad, int elem_size, int pad, Plan* p)
+{
+    const int64_t padded = pad > 0 ? n_pad + n_pad / pad : n_pad;
+    if (4 * padded * elem_size > kLdsBytes) return TRID_ENOTSUP;
+    p->family = Family::Lds;
+    p->pad = pad;
+    return TRID_OK;
+}
+
+}  // namespace
+
+int make_plan(int method, int64_t n, int64_t N, int elem_size, const trid_opts& o, Plan* p)
+{
+    if (n < 2 || N < 0) return TRID_EINVAL;
+    if (elem_size != 4 && elem_size != 8) return TRID_EINVAL;
+    if (n > (int64_t(1) << 30)) return TRID_ENOTSUP;
+    *p = Plan{};
+    p->n_pad = next_pow2(n);
+    p->dense = (p->n_pad == n) ? 1 : 0;
+    const int64_t n_pad = p->n_pad;
+    if (method == TRID_AUTO) method = TRID_CRPCR;
+
+    switch (method) {
+        case TRID_CR:
+        case TRID_PCR:
+        case TRID_CRPCR: {
+            int rest;
+            if (method == TRID_CR) {
+                rest = 2;
+            } else if (method == TRID_PCR) {
+                rest = (int)(n_pad > (1 << 30) ? (1 << 30) : n_pad);
+            } else if (o.rest == 0) {
+                rest = -1;  // chosen below
+            } else {
+                if (!is_pow2(o.rest) || o.rest < 2 || o.rest > n_pad / 2) return TRID_EINVAL;
+                rest = o.rest;
+            }
+            int st = plan_reg(n_pad, elem_size, o, p);
+            if (st == TRID_EINVAL) return st;
+            if (st == TRID_OK) {
+                if (rest < 0) rest = (int)(p->L < n_pad / 2 ? p->L : n_pad / 2);
+                if (rest < 2) rest = 2;
+                p->rest = rest;
+                return TRID_OK;
+            }
+            // Too long for one wave: LDS-resident kernel, unpadded map.
+            if (rest < 0) rest = (int)(n_pad / 2 < 64 ? n_pad / 2 : 64);
+            p->rest = rest;
+            const int pad = o.lds_pad > 0 ? o.lds_pad : 0;
+            return plan_lds(n_pad, elem_size, pad, p);
+        }
+        case TRID_CRNBC:
+        case TRID_CRNBC32:
+        case TRID_CRPCRNBC: {
+            int rest = 2;
+            if (method == TRID_CRPCRNBC) {
+                if (o.rest == 0) {
+                    rest = (int)(n_pad / 2 < 64 ? n_pad / 2 : 64);
+                } else {
+                    if (!is_pow2(o.rest) || o.rest < 2 || o.rest > n_pad / 2) return TRID_EINVAL;
+                    rest = o.rest;
+                }
+            }
+            int pad;
+            if (method == TRID_CRNBC32) {
+                pad = 0;  // the unpadded control (crNBC32banks.cu's map is the identity)
+            } else if (o.lds_pad == 0) {
+                pad = 64 * 4 / elem_size;  // one element per 64 banks of 4 bytes
+            } else if (o.lds_pad < 0) {
+                pad = 0;
+            } else {
+                if (!is_pow2(o.lds_pad) || o.lds_pad < 2) return TRID_EINVAL;
+                pad = o.lds_pad;
+            }
+            p->rest = rest;
+            return plan_lds(n_pad, elem_size, pad, p);
+        }
+        case TRID_PCRGAUSS: {
+            int G = o.gaussian;
+            if (G == 0) G = (int)(n_pad / 2 < 64 ? n_pad / 2 : 64);
+            if (!is_pow2(G) || G > n_pad / 2) return TRID_EINVAL;
+            p->gaussian = G;
+            p->family = Family::PcrThomas;
+            return TRID_OK;
+        }
+        default:
+            return TRID_EINVAL;
+    }
+}
+
+}  // namespace trid
+
+extern "C" int trid_plan_query(int method, int64_t systemSize, int64_t numSystems,
+                               int elemSize, const trid_opts* opts, trid_plan_info* out)
+{
+    if (!out) return TRID_EINVAL;
+    trid_opts o{};
+    if (opts) o = *opts;
+    if (o.flags != 0) return TRID_EINVAL;
+    trid::Plan p;
+    int st = trid::make_plan(method, systemSize, numSystems, elemSize, o, &p);
+    if (st != TRID_OK) return st;
+    out->family = (int32_t)p.family;
+    out->per_lane = p.M;
+    out->lanes = p.L;
+    out->rest = p.rest;
+    out->lds_pad = p.pad;
+    out->gaussian = p.gaussian;
+    out->padded_size = p.n_pad;
+    return TRID_OK;
+}

@@ -1,0 +1,178 @@
+"""Host-side mirror of the reference solver API, over the C ABI of libtrid.so.
+
+The reference exposes seven free functions with host pointers, a
+compile-time element type and a void return (forward-declared in
+test_tridiagonal.cpp:17-26). This module keeps their names, argument order
+and meaning -- ``cr(a, b, c, d, x, systemSize, numSystems)`` and so on --
+for numpy arrays (float32 or float64 picks the _f32 or _f64 entry point).
+``x`` is overwritten; ``a, b, c, d`` are not modified. Errors raise
+``TridError`` instead of exiting the process.
+
+``Handle.solve`` is the stream-ordered device API (``trid_solve_*``) for
+device pointers, e.g. ``torch.Tensor.data_ptr()`` of tensors on ``cuda:0``.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import METHODS, Opts, PlanInfo, TridError, check, lib
+
+
+def _suffix(arr: np.ndarray) -> str:
+    if arr.dtype == np.float32:
+        return "f32"
+    if arr.dtype == np.float64:
+        return "f64"
+    raise TypeError(f"unsupported dtype {arr.dtype}; use float32 or float64")
+
+
+def _host_ptrs(a, b, c, d, x, systemSize: int, numSystems: int):
+    arrs = (a, b, c, d, x)
+    for v in arrs:
+        if not isinstance(v, np.ndarray):
+            raise TypeError("host arrays must be numpy arrays")
+        if not v.flags["C_CONTIGUOUS"]:
+            raise ValueError("host arrays must be C-contiguous")
+        if v.dtype != a.dtype:
+            raise TypeError("a, b, c, d, x must share one dtype")
+        if v.size < int(systemSize) * int(numSystems):
+            raise ValueError("array shorter than systemSize * numSystems")
+    if not x.flags["WRITEABLE"]:
+        raise ValueError("x must be writeable")
+    return [ctypes.c_void_p(v.ctypes.data) for v in arrs]
+
+
+def _call7(name: str, a, b, c, d, x, systemSize, numSystems):
+    p = _host_ptrs(a, b, c, d, x, systemSize, numSystems)
+    f = getattr(lib(), f"trid_{name}_{_suffix(a)}")
+    check(f(*p, int(systemSize), int(numSystems)), name)
+
+
+def _call8(name: str, a, b, c, d, x, systemSize, numSystems, extra):
+    p = _host_ptrs(a, b, c, d, x, systemSize, numSystems)
+    f = getattr(lib(), f"trid_{name}_{_suffix(a)}")
+    check(f(*p, int(systemSize), int(numSystems), int(extra)), name)
+
+
+# --- reference names (cr_app.cu:4, pcr_app.cu:5, crpcr_app.cu:5, crNBC.cu:123,
+#     crNBC.cu:344, crNBC32banks.cu:123, pcrGauss_app.cu:164) -----------------
+
+def cr(a, b, c, d, x, systemSize, numSystems):
+    _call7("cr", a, b, c, d, x, systemSize, numSystems)
+
+
+def pcr(a, b, c, d, x, systemSize, numSystems):
+    _call7("pcr", a, b, c, d, x, systemSize, numSystems)
+
+
+def crpcr(a, b, c, d, x, systemSize, numSystems, restSystemSize):
+    _call8("crpcr", a, b, c, d, x, systemSize, numSystems, restSystemSize)
+
+
+def crNBC(a, b, c, d, x, systemSize, numSystems):
+    _call7("crnbc", a, b, c, d, x, systemSize, numSystems)
+
+
+def crpcrNBC(a, b, c, d, x, systemSize, numSystems, restSystemSize):
+    _call8("crpcrnbc", a, b, c, d, x, systemSize, numSystems, restSystemSize)
+
+
+def crNBC32(a, b, c, d, x, systemSize, numSystems):
+    _call7("crnbc32", a, b, c, d, x, systemSize, numSystems)
+
+
+def pcrGaussWrapper(a, b, c, d, x, systemSize, numSystems, gaussian):
+    _call8("pcrgauss", a, b, c, d, x, systemSize, numSystems, gaussian)
+
+
+# --- options, plans, device API ----------------------------------------------
+
+def make_opts(rest=0, gaussian=0, lds_pad=0, lanes=0, per_lane=0, io_mode=0,
+              flags=0) -> Opts:
+    return Opts(int(rest), int(gaussian), int(lds_pad), int(lanes), int(per_lane),
+                int(io_mode), int(flags))
+
+
+def method_id(method) -> int:
+    return METHODS[method] if isinstance(method, str) else int(method)
+
+
+def plan_query(method, systemSize: int, numSystems: int, elem_size: int = 4,
+               opts: Opts | None = None) -> dict:
+    """Validate a request and report the kernel plan (no device needed)."""
+    info = PlanInfo()
+    o = opts if opts is not None else Opts()
+    st = lib().trid_plan_query(method_id(method), int(systemSize), int(numSystems),
+                               int(elem_size), ctypes.byref(o), ctypes.byref(info))
+    check(st, "plan_query")
+    return {k: getattr(info, k) for k, _ in PlanInfo._fields_}
+
+
+def _ptr(v) -> int:
+    if isinstance(v, int):
+        return v
+    if hasattr(v, "data_ptr"):
+        return int(v.data_ptr())
+    raise TypeError("device arrays must be torch tensors or integer addresses")
+
+
+def _stream_ptr(stream) -> int | None:
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return int(stream.cuda_stream)
+
+
+class Handle:
+    """Owns a trid_handle (per-thread workspace) for the device API."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        check(lib().trid_create(ctypes.byref(h)), "trid_create")
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().trid_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, method, a, b, c, d, x, systemSize: int, numSystems: int,
+              opts: Opts | None = None, stream=None, dtype: str | None = None) -> None:
+        if dtype is None:
+            dtype = "f64" if str(getattr(a, "dtype", "")) == "torch.float64" else "f32"
+        f = getattr(lib(), f"trid_solve_{dtype}")
+        o = opts if opts is not None else Opts()
+        st = f(self._h, method_id(method), *(ctypes.c_void_p(_ptr(v)) for v in (a, b, c, d, x)),
+               int(systemSize), int(numSystems), ctypes.byref(o),
+               ctypes.c_void_p(_stream_ptr(stream)))
+        check(st, "trid_solve")
+
+
+def generate(kind: int, seed: int, a, b, c, d, systemSize: int, numSystems: int,
+             stream=None, dtype: str | None = None) -> None:
+    """Seeded synthetic batch written on the device (trid_generate_*)."""
+    if dtype is None:
+        dtype = "f64" if str(getattr(a, "dtype", "")) == "torch.float64" else "f32"
+    f = getattr(lib(), f"trid_generate_{dtype}")
+    st = f(int(kind), ctypes.c_uint64(seed), *(ctypes.c_void_p(_ptr(v)) for v in (a, b, c, d)),
+           int(systemSize), int(numSystems), ctypes.c_void_p(_stream_ptr(stream)))
+    check(st, "trid_generate")
+
+
+def device_count() -> int:
+    return int(lib().trid_device_count())
+
+
+__all__ = ["cr", "pcr", "crpcr", "crNBC", "crpcrNBC", "crNBC32", "pcrGaussWrapper",
+           "Handle", "generate", "plan_query", "make_opts", "device_count", "TridError",
+           "METHODS", "_lib"]
