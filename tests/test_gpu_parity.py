@@ -218,3 +218,23 @@ def test_headline_full_size(oracle, trid, dtype):
     worst = check(oracle, x.cpu().numpy(), a, b, c, d, n, dtype)
     print(f"headline {np.dtype(dtype).name}: max rel err {worst:.3e}", flush=True)
     h.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_io_modes_bitwise_identical(oracle, trid, dtype):
+    # the dense-path I/O modes move the same values: results are bit-identical
+    import torch
+    for n, N in ((1024, 301), (512, 77), (64, 1000), (16, 129)):
+        a, b, c, d = oracle.generate("dominant", 31, n, N, dtype)
+        dev = [torch.from_numpy(v).cuda() for v in (a, b, c, d)]
+        h = trid.Handle()
+        outs = []
+        for io in (1, 2, 3, 4):
+            x = torch.full_like(dev[0], float("nan"))
+            h.solve("crpcr", *dev, x, n, N, trid.make_opts(rest=min(64, n // 2), io_mode=io))
+            torch.cuda.synchronize()
+            outs.append(x.cpu().numpy())
+        h.close()
+        check(oracle, outs[0], a, b, c, d, n, dtype)
+        for o in outs[1:]:
+            assert np.array_equal(o, outs[0]), (n, N)
