@@ -41,6 +41,9 @@ def parse():
     p.add_argument("--io", type=int, default=0)
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--copy-probe", action="store_true",
+                   help="after timing, copy the four input arrays once with a plain "
+                        "device copy (known byte count) to calibrate HBM counters")
     return p.parse_args()
 
 
@@ -165,6 +168,13 @@ def main():
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": traffic},
     }
+    if args.copy_probe:
+        # 4 * n * N * esize bytes read and written by a plain copy kernel
+        dst = torch.empty_like(a)
+        for src in (a, b, c, d):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_probe_bytes_each_way"] = 4 * n * N * esize
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, x.cpu().numpy(), n, N)
     h.close()

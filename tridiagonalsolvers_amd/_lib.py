@@ -59,6 +59,13 @@ def lib() -> ctypes.CDLL:
     global _lib
     if _lib is not None:
         return _lib
+    try:
+        # libtrid.so and torch both bind libamdhip64.so.7 by soname. Loading
+        # torch first makes the process use one HIP runtime (torch's bundled
+        # copy) for both; the other order leaves torch unable to initialise.
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is not built; run __graft_entry__.build() "

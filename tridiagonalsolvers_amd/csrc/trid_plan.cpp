@@ -20,7 +20,7 @@ constexpr int64_t kLdsBytes = 160 * 1024;
 
 // Dense-path I/O mode of the register kernels when the caller does not pick
 // one (RegArgs::io numbering).
-constexpr int kDefaultIo = 0;
+constexpr int kDefaultIo = 3;  // wave-coalesced through LDS, non-temporal
 
 // Register family: unknowns per lane M and lanes per system L with
 // L * M = n_pad, 2 <= L <= 64, M <= 16.
