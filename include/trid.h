@@ -60,13 +60,20 @@ typedef struct trid_opts {
     int32_t lds_pad;    /* LDS-resident kernels: one pad element per lds_pad
                            elements (power of two), -1 = no padding           */
     int32_t lanes;      /* register kernels: lanes per system (power of two,
-                           2..64), 0 = plan                                   */
-    int32_t per_lane;   /* register kernels: unknowns per lane (1..16)        */
+                           2..1024; above 64 a workgroup of lanes/64 waves
+                           holds one system), 0 = plan                        */
+    int32_t per_lane;   /* register kernels: unknowns per lane (1..16; 2..16
+                           FP32 / 2..8 FP64 when lanes > 64)                  */
     int32_t io_mode;    /* register kernels, dense batches: 0 = plan,
                            1 per-lane non-temporal, 2 per-lane cached,
                            3 wave-coalesced through LDS, 4 as 3 non-temporal  */
-    int32_t flags;      /* reserved, must be 0                                */
+    int32_t flags;      /* bit set of TRID_FLAG_*                             */
 } trid_opts;
+
+/* PCRGAUSS: use the two-pass form (global PCR pass, then strided Thomas pass
+ * through a handle workspace of 4*n*N elements) even when the system fits
+ * one workgroup's LDS. It is always used when it does not fit. */
+#define TRID_FLAG_TWO_PASS 1
 
 /* Human-readable text for a status code (static storage). */
 const char* trid_status_string(int status);

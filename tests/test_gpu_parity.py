@@ -238,3 +238,69 @@ def test_io_modes_bitwise_identical(oracle, trid, dtype):
         check(oracle, outs[0], a, b, c, d, n, dtype)
         for o in outs[1:]:
             assert np.array_equal(o, outs[0]), (n, N)
+
+
+def _mw_sizes(dtype):
+    return [2048, 3000, 4096, 8192] + ([16384] if dtype == np.float32 else [5000])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("method", ["cr", "crpcr"])
+def test_multiwave_methods(oracle, trid, method, dtype):
+    # systems longer than one wave: W = 2..16 waves per workgroup
+    for n in _mw_sizes(dtype):
+        N = 19
+        a, b, c, d = oracle.generate("dominant", 200 + n, n, N, dtype)
+        print(f"{method} {np.dtype(dtype).name} n={n}", flush=True)
+        x = solve_host(trid, method, a, b, c, d, n, N, 64)
+        check(oracle, x, a, b, c, d, n, dtype)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_multiwave_rest_sweep(oracle, trid, dtype):
+    import torch
+    n, N = 8192, 24
+    a, b, c, d = oracle.generate("dominant", 17, n, N, dtype)
+    dev = [torch.from_numpy(v).cuda() for v in (a, b, c, d)]
+    h = trid.Handle()
+    lanes = trid.plan_query("crpcr", n, N, np.dtype(dtype).itemsize)["lanes"]
+    outs = []
+    rest = 2
+    while rest <= lanes:
+        x = torch.full_like(dev[0], float("nan"))
+        h.solve("crpcr", *dev, x, n, N, trid.make_opts(rest=rest))
+        torch.cuda.synchronize()
+        print(f"rest={rest}", flush=True)
+        outs.append(x.cpu().numpy())
+        check(oracle, outs[-1], a, b, c, d, n, dtype)
+        rest *= 2
+    h.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_multiwave_shapes_and_io(oracle, trid, dtype):
+    # the same system split over 2..16 waves, every I/O mode
+    import torch
+    n, N = 2048, 33
+    a, b, c, d = oracle.generate("weak", 8, n, N, dtype)
+    dev = [torch.from_numpy(v).cuda() for v in (a, b, c, d)]
+    h = trid.Handle()
+    mmax = 16 if dtype == np.float32 else 8
+    for M in (2, 4, 8, 16):
+        if M > mmax or n // M > 1024:
+            continue
+        for io in (1, 2, 3, 4):
+            x = torch.full_like(dev[0], float("nan"))
+            h.solve("crpcr", *dev, x, n, N,
+                    trid.make_opts(rest=32, lanes=n // M, per_lane=M, io_mode=io))
+            torch.cuda.synchronize()
+            check(oracle, x.cpu().numpy(), a, b, c, d, n, dtype, TOL[dtype] * 10)
+    # FP64 n=1024 over two waves
+    if dtype == np.float64:
+        a, b, c, d = oracle.generate("dominant", 9, 1024, N, dtype)
+        dev = [torch.from_numpy(v).cuda() for v in (a, b, c, d)]
+        x = torch.full_like(dev[0], float("nan"))
+        h.solve("crpcr", *dev, x, 1024, N, trid.make_opts(rest=64, lanes=128, per_lane=8))
+        torch.cuda.synchronize()
+        check(oracle, x.cpu().numpy(), a, b, c, d, 1024, dtype)
+    h.close()

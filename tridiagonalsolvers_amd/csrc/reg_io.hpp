@@ -1,0 +1,149 @@
+// reg_io.hpp -- global <-> register movement of the register-resident
+// kernels (kern_reg.hip, kern_reg_mw.hip).
+#pragma once
+
+#include "trid_device.hpp"
+
+namespace trid {
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// I/O modes of the dense path (see RegArgs::io):
+//   0  per-lane 16-byte accesses, non-temporal
+//   1  per-lane 16-byte accesses, default cache policy
+//   2  wave-coalesced 16-byte accesses staged through LDS (padded rows)
+//   3  as 2, non-temporal
+template <bool NT>
+static __device__ __forceinline__ u32x4 gload16(const void* p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+    else return *reinterpret_cast<const u32x4*>(p);
+}
+template <bool NT>
+static __device__ __forceinline__ void gstore16(void* p, u32x4 v)
+{
+    if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(p));
+    else *reinterpret_cast<u32x4*>(p) = v;
+}
+
+template <typename T, int M>
+static __device__ __forceinline__ void set_identity(Eq<T> (&e)[M])
+{
+#pragma unroll
+    for (int s = 0; s < M; ++s) e[s] = identity_eq<T>();
+}
+
+// Per-lane 16-byte loads of the lane's M consecutive unknowns.
+template <typename T, int M, bool NT>
+static __device__ __forceinline__ void load_direct(const T* __restrict__ A, const T* __restrict__ B,
+                                            const T* __restrict__ C, const T* __restrict__ D,
+                                            int64_t off, Eq<T> (&e)[M])
+{
+    constexpr int E = 16 / sizeof(T);
+    T ta[M], tb[M], tc[M], td[M];
+#pragma unroll
+    for (int q = 0; q < M / E; ++q) {
+        u32x4 va = gload16<NT>(A + off + q * E);
+        u32x4 vb = gload16<NT>(B + off + q * E);
+        u32x4 vc = gload16<NT>(C + off + q * E);
+        u32x4 vd = gload16<NT>(D + off + q * E);
+        __builtin_memcpy(&ta[q * E], &va, 16);
+        __builtin_memcpy(&tb[q * E], &vb, 16);
+        __builtin_memcpy(&tc[q * E], &vc, 16);
+        __builtin_memcpy(&td[q * E], &vd, 16);
+    }
+#pragma unroll
+    for (int s = 0; s < M; ++s) e[s] = Eq<T>{ta[s], tb[s], tc[s], td[s]};
+}
+
+template <typename T, int M, bool NT>
+static __device__ __forceinline__ void store_direct(T* __restrict__ X, int64_t off, const T (&xv)[M])
+{
+    constexpr int E = 16 / sizeof(T);
+#pragma unroll
+    for (int q = 0; q < M / E; ++q) {
+        u32x4 v;
+        __builtin_memcpy(&v, &xv[q * E], 16);
+        gstore16<NT>(X + off + q * E, v);
+    }
+}
+
+// LDS row of one lane: M elements plus 16 bytes of padding. With 64 banks of
+// 4 bytes and 16 lanes served per pass of a 16-byte access, a row stride of
+// M*sizeof(T)+16 bytes puts the 16 rows of a pass on disjoint banks for the
+// row-wise reads (M*sizeof(T) in {32, 64, 128}).
+template <typename T, int M>
+struct Stage {
+    static constexpr int E = 16 / sizeof(T);
+    static constexpr int Q = M / E;                        // 16-byte chunks per row
+    static constexpr int kRowBytes = M * (int)sizeof(T) + 16;
+    static constexpr int kWaveBytes = kWave * kRowBytes;
+    // chunk k of the wave block (k = 64 q + lane) lives in row k / Q
+    __device__ static __forceinline__ int chunk_addr(int k)
+    {
+        return (k / Q) * kRowBytes + (k % Q) * 16;
+    }
+};
+
+// Wave-coalesced loads: instruction q of the wave reads the q-th contiguous
+// 1 KiB of the wave's block; the chunks are transposed through LDS so that
+// lane l ends up with unknowns [l*M, l*M+M) of the block.
+template <typename T, int M, bool NT>
+static __device__ __forceinline__ void load_staged(const T* __restrict__ A, const T* __restrict__ B,
+                                            const T* __restrict__ C, const T* __restrict__ D,
+                                            int64_t wave_base, int64_t total, int lane,
+                                            char* lds, Eq<T> (&e)[M])
+{
+    using S = Stage<T, M>;
+    constexpr int E = S::E, Q = S::Q;
+    u32x4 r[4][Q];
+    const T* src[4] = {A, B, C, D};
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int64_t el = wave_base + (int64_t)(kWave * q + lane) * E;
+        const bool ok = el < total;
+#pragma unroll
+        for (int arr = 0; arr < 4; ++arr)
+            r[arr][q] = ok ? gload16<NT>(src[arr] + el) : u32x4{0u, 0u, 0u, 0u};
+    }
+    T v[4][M];
+#pragma unroll
+    for (int arr = 0; arr < 4; ++arr) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            *reinterpret_cast<u32x4*>(lds + S::chunk_addr(kWave * q + lane)) = r[arr][q];
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            u32x4 t = *reinterpret_cast<const u32x4*>(lds + lane * S::kRowBytes + q * 16);
+            __builtin_memcpy(&v[arr][q * E], &t, 16);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+#pragma unroll
+    for (int s = 0; s < M; ++s) e[s] = Eq<T>{v[0][s], v[1][s], v[2][s], v[3][s]};
+}
+
+template <typename T, int M, bool NT>
+static __device__ __forceinline__ void store_staged(T* __restrict__ X, int64_t wave_base, int64_t total,
+                                             int lane, char* lds, const T (&xv)[M])
+{
+    using S = Stage<T, M>;
+    constexpr int E = S::E, Q = S::Q;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        u32x4 t;
+        __builtin_memcpy(&t, &xv[q * E], 16);
+        *reinterpret_cast<u32x4*>(lds + lane * S::kRowBytes + q * 16) = t;
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const u32x4 t = *reinterpret_cast<const u32x4*>(lds + S::chunk_addr(kWave * q + lane));
+        const int64_t el = wave_base + (int64_t)(kWave * q + lane) * E;
+        if (el < total) gstore16<NT>(X + el, t);
+    }
+}
+
+
+}  // namespace trid

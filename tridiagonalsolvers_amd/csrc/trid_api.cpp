@@ -18,9 +18,35 @@
 #include "trid_launch.hpp"
 #include "trid_plan.hpp"
 
+struct trid_context {
+    int device;
+    void* ws;         // workspace of the two-pass pcrGauss form
+    size_t ws_bytes;
+};
+
 namespace {
 
 thread_local int g_last_status = TRID_OK;
+
+// Grow the handle's workspace to at least `bytes` (device memory; freeing
+// the old buffer waits for work that may still use it).
+int ensure_ws(trid_context* h, size_t bytes)
+{
+    if (!h) return TRID_EINVAL;
+    if (h->ws_bytes >= bytes) return TRID_OK;
+    if (h->ws) {
+        (void)hipFree(h->ws);
+        h->ws = nullptr;
+        h->ws_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&h->ws, bytes);
+    if (e != hipSuccess) {
+        h->ws = nullptr;
+        return e == hipErrorOutOfMemory ? TRID_ENOMEM : TRID_EHIP;
+    }
+    h->ws_bytes = bytes;
+    return TRID_OK;
+}
 
 int set_status(int s)
 {
@@ -63,12 +89,12 @@ template <typename T>
 int solve_impl(trid_handle h, int method, const T* a, const T* b, const T* c, const T* d, T* x,
                int64_t n, int64_t N, const trid_opts* opts, hipStream_t stream)
 {
-    (void)h;
     trid_opts o{};
     if (opts) o = *opts;
+    if (!h) return TRID_EINVAL;
     if (!a || !b || !c || !d || !x) return TRID_EINVAL;
     if (n < 2 || N < 0) return TRID_EINVAL;
-    if (o.flags != 0) return TRID_EINVAL;
+    if (o.flags & ~TRID_FLAG_TWO_PASS) return TRID_EINVAL;
     if (N == 0) return TRID_OK;
 
     trid::Plan p;
@@ -96,7 +122,43 @@ int solve_impl(trid_handle h, int method, const T* a, const T* b, const T* c, co
                         ? 1
                         : 0;
             g.stream = stream;
-            return from_hip(trid::launch_reg_crpcr<T>(g));
+            return from_hip(g.L <= 64 ? trid::launch_reg_crpcr<T>(g) : trid::launch_reg_mw<T>(g));
+        }
+        case trid::Family::Lds: {
+            trid::LdsArgs<T> g{};
+            g.a = a;
+            g.b = b;
+            g.c = c;
+            g.d = d;
+            g.x = x;
+            g.n = n;
+            g.N = N;
+            g.n_pad = (int)p.n_pad;
+            g.rest = p.rest;
+            g.pad = p.pad;
+            g.stream = stream;
+            return from_hip(trid::launch_lds_crpcr<T>(g));
+        }
+        case trid::Family::PcrThomas: {
+            trid::PcrThomasArgs<T> g{};
+            g.a = a;
+            g.b = b;
+            g.c = c;
+            g.d = d;
+            g.x = x;
+            g.n = n;
+            g.N = N;
+            g.n_pad = (int)p.n_pad;
+            g.G = p.gaussian;
+            g.pad = p.pad;
+            g.two_pass = p.two_pass;
+            if (p.two_pass) {
+                st = ensure_ws(h, (size_t)4 * (size_t)n * (size_t)N * sizeof(T));
+                if (st != TRID_OK) return st;
+                g.ws = static_cast<T*>(h->ws);
+            }
+            g.stream = stream;
+            return from_hip(trid::launch_pcr_thomas<T>(g));
         }
         default:
             return TRID_ENOTSUP;
@@ -139,21 +201,19 @@ int host_solve(int method, const T* a, const T* b, const T* c, const T* d, T* x,
         (e = hipMemcpy(dd, d, bytes, hipMemcpyHostToDevice)) != hipSuccess) {
         rc = from_hip(e);
     }
-    if (rc == TRID_OK) rc = solve_impl<T>(nullptr, method, da, db, dc, dd, dx, n, N, &o, nullptr);
+    trid_context ctx{};
+    if (rc == TRID_OK) rc = solve_impl<T>(&ctx, method, da, db, dc, dd, dx, n, N, &o, nullptr);
     if (rc == TRID_OK) {
         e = hipStreamSynchronize(nullptr);
         if (e == hipSuccess) e = hipMemcpy(x, dx, bytes, hipMemcpyDeviceToHost);
         rc = from_hip(e);
     }
+    if (ctx.ws) (void)hipFree(ctx.ws);
     (void)hipFree(base);
     return rc;
 }
 
 }  // namespace
-
-struct trid_context {
-    int device;
-};
 
 extern "C" {
 
@@ -192,6 +252,8 @@ int trid_create(trid_handle* handle)
     trid_context* h = new (std::nothrow) trid_context();
     if (!h) return set_status(TRID_ENOMEM);
     h->device = 0;
+    h->ws = nullptr;
+    h->ws_bytes = 0;
     (void)hipGetDevice(&h->device);
     *handle = h;
     return set_status(TRID_OK);
@@ -200,6 +262,7 @@ int trid_create(trid_handle* handle)
 int trid_destroy(trid_handle handle)
 {
     if (!handle) return set_status(TRID_EINVAL);
+    if (handle->ws) (void)hipFree(handle->ws);
     delete handle;
     return set_status(TRID_OK);
 }

@@ -110,6 +110,24 @@ __device__ __forceinline__ T lane_read_or_zero(T v, int src, bool valid)
     return valid ? r : T(0);
 }
 
+// Value of `v` in lane `src` (wave-uniform index), broadcast to all lanes.
+__device__ __forceinline__ float lane_bcast(float v, int src)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ __forceinline__ double lane_bcast(double v, int src)
+{
+    int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+    return __hiloint2double(hi, lo);
+}
+template <typename T>
+__device__ __forceinline__ Eq<T> lane_bcast_eq(const Eq<T>& e, int src)
+{
+    return Eq<T>{lane_bcast(e.a, src), lane_bcast(e.b, src), lane_bcast(e.c, src),
+                 lane_bcast(e.d, src)};
+}
+
 template <int V>
 struct Log2 {
     static constexpr int value = 1 + Log2<V / 2>::value;

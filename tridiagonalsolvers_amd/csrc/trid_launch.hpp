@@ -17,7 +17,7 @@ struct RegArgs {
     int64_t n;       // unknowns per system
     int64_t N;       // systems
     int M;           // unknowns per lane (1, 2, 4, 8, 16)
-    int L;           // lanes per system (power of two, 2..64), L*M >= n
+    int L;           // lanes per system (power of two, 2..1024), L*M >= n
     int rest;        // size of the PCR system (power of two, 2..L*M)
     int vec;         // 1: dense 16-byte vector path (n == L*M, aligned)
     int io;          // dense-path I/O mode: 0 per-lane non-temporal, 1 per-lane,
@@ -27,6 +27,50 @@ struct RegArgs {
 
 template <typename T>
 hipError_t launch_reg_crpcr(const RegArgs<T>& g);
+
+// Multi-wave register-resident CR / CR-PCR (kern_reg_mw.hip): g.L = 64 * W.
+template <typename T>
+hipError_t launch_reg_mw(const RegArgs<T>& g);
+
+// LDS-resident CR / CR-PCR with a pad map (kern_lds.hip).
+template <typename T>
+struct LdsArgs {
+    const T* a;
+    const T* b;
+    const T* c;
+    const T* d;
+    T* x;
+    int64_t n;
+    int64_t N;
+    int n_pad;   // power of two >= n
+    int rest;    // PCR system size (2 = CR, n_pad = PCR)
+    int pad;     // pad period in elements, 0 = none
+    hipStream_t stream;
+};
+
+template <typename T>
+hipError_t launch_lds_crpcr(const LdsArgs<T>& g);
+
+// PCR to G subsystems + Thomas (kern_pcrgauss.hip).
+template <typename T>
+struct PcrThomasArgs {
+    const T* a;
+    const T* b;
+    const T* c;
+    const T* d;
+    T* x;
+    int64_t n;
+    int64_t N;
+    int n_pad;
+    int G;          // subsystems; PCR depth log2(G)
+    int pad;        // in-LDS form: pad period in elements, 0 = none
+    int two_pass;   // 1: global PCR pass + strided Thomas pass through `ws`
+    T* ws;          // two-pass workspace: 4 * n * N elements
+    hipStream_t stream;
+};
+
+template <typename T>
+hipError_t launch_pcr_thomas(const PcrThomasArgs<T>& g);
 
 // Seeded synthetic batch (kern_util.hip).
 template <typename T>

@@ -23,25 +23,31 @@ constexpr int64_t kLdsBytes = 160 * 1024;
 constexpr int kDefaultIo = 3;  // wave-coalesced through LDS, non-temporal
 
 // Register family: unknowns per lane M and lanes per system L with
-// L * M = n_pad, 2 <= L <= 64, M <= 16.
+// L * M = n_pad. L <= 64: one wave holds a system (several systems share a
+// wave when L < 64). L = 64 * W, W = 2..16: a workgroup of W waves holds one
+// system (M <= 16 for FP32, M <= 8 for FP64, to fit 1024 threads' registers).
 int plan_reg(int64_t n_pad, int elem_size, const trid_opts& o, Plan* p)
 {
     int M, L;
+    const int mmax_mw = elem_size == 4 ? 16 : 8;
     if (o.per_lane != 0 || o.lanes != 0) {
         if (o.per_lane == 0 || o.lanes == 0) return TRID_EINVAL;
         M = o.per_lane;
         L = o.lanes;
-        if (!is_pow2(M) || M > 16 || !is_pow2(L) || L < 2 || L > 64) return TRID_EINVAL;
+        if (!is_pow2(M) || M > 16 || !is_pow2(L) || L < 2 || L > 1024) return TRID_EINVAL;
         if ((int64_t)M * L != n_pad) return TRID_EINVAL;
-    } else {
-        if (n_pad > 64 * 16) return TRID_ENOTSUP;
+        if (L > 64 && (M < 2 || M > mmax_mw)) return TRID_EINVAL;
+    } else if (n_pad <= 64 * 16) {
         // Up to 16 unknowns per lane, at least two lanes per system.
         int64_t m = n_pad / 2;
         if (m > 16) m = 16;
         if (n_pad / m > 64) m = n_pad / 64;
         M = (int)m;
         L = (int)(n_pad / m);
-        (void)elem_size;
+    } else {
+        M = mmax_mw;
+        if (n_pad / M > 1024) return TRID_ENOTSUP;
+        L = (int)(n_pad / M);
     }
     if (o.io_mode < 0 || o.io_mode > 4) return TRID_EINVAL;
     p->family = Family::Reg;
@@ -57,6 +63,8 @@ int plan_lds(int64_t n_pad, int elem_size, int pad, Plan* p)
 {
     const int64_t padded = pad > 0 ? n_pad + n_pad / pad : n_pad;
     if (4 * padded * elem_size > kLdsBytes) return TRID_ENOTSUP;
+    // PCR levels in LDS: at most 16 equations per thread of 256
+    if (p->rest > 16 * 256) return TRID_ENOTSUP;
     p->family = Family::Lds;
     p->pad = pad;
     return TRID_OK;
@@ -93,10 +101,21 @@ int make_plan(int method, int64_t n, int64_t N, int elem_size, const trid_opts& 
             int st = plan_reg(n_pad, elem_size, o, p);
             if (st == TRID_EINVAL) return st;
             if (st == TRID_OK) {
-                if (rest < 0) rest = (int)(p->L < n_pad / 2 ? p->L : n_pad / 2);
-                if (rest < 2) rest = 2;
-                p->rest = rest;
-                return TRID_OK;
+                if (rest < 0) {
+                    // default switch size: one PCR equation per lane of a wave
+                    const int64_t r = p->L <= 64 ? p->L : 64;
+                    rest = (int)(r < n_pad / 2 ? r : n_pad / 2);
+                    if (rest < 2) rest = 2;
+                }
+                // multi-wave kernels hand PCR at most one equation per lane
+                if (p->L <= 64 || rest <= p->L) {
+                    p->rest = rest;
+                    return TRID_OK;
+                }
+                if (o.lanes != 0) return TRID_EINVAL;
+                *p = Plan{};
+                p->n_pad = n_pad;
+                p->dense = (n_pad == n) ? 1 : 0;
             }
             // Too long for one wave: LDS-resident kernel, unpadded map.
             if (rest < 0) rest = (int)(n_pad / 2 < 64 ? n_pad / 2 : 64);
@@ -136,6 +155,14 @@ int make_plan(int method, int64_t n, int64_t N, int elem_size, const trid_opts& 
             if (!is_pow2(G) || G > n_pad / 2) return TRID_EINVAL;
             p->gaussian = G;
             p->family = Family::PcrThomas;
+            // in-LDS form: four padded arrays in 160 KiB, <= 8 rows per thread
+            const int pad = 64 * 4 / elem_size;
+            const int64_t padded = n_pad + n_pad / pad;
+            const bool fits = 4 * padded * elem_size <= kLdsBytes && n_pad <= 8 * 1024;
+            p->two_pass = (!fits || (o.flags & TRID_FLAG_TWO_PASS)) ? 1 : 0;
+            p->pad = p->two_pass ? 0 : pad;
+            if (p->two_pass && G > (elem_size == 4 ? 1024 : 512)) return TRID_ENOTSUP;
+            if (p->two_pass && N > 65535) return TRID_ENOTSUP;
             return TRID_OK;
         }
         default:
@@ -151,7 +178,7 @@ extern "C" int trid_plan_query(int method, int64_t systemSize, int64_t numSystem
     if (!out) return TRID_EINVAL;
     trid_opts o{};
     if (opts) o = *opts;
-    if (o.flags != 0) return TRID_EINVAL;
+    if (o.flags & ~TRID_FLAG_TWO_PASS) return TRID_EINVAL;
     trid::Plan p;
     int st = trid::make_plan(method, systemSize, numSystems, elemSize, o, &p);
     if (st != TRID_OK) return st;

@@ -28,6 +28,7 @@ struct Plan {
     int gaussian;   // PcrThomas: number of Thomas subsystems
     int dense;      // 1 if n == n_pad (vector path possible)
     int io;         // Reg: dense-path I/O mode (RegArgs::io)
+    int two_pass;   // PcrThomas: global PCR pass + strided Thomas pass
 };
 
 // Returns TRID_OK and fills *p, or TRID_EINVAL / TRID_ENOTSUP.
