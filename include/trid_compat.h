@@ -1,0 +1,53 @@
+/*
+ * trid_compat.h -- the reference's solver names as inline C++ overloads.
+ *
+ * zhangyaobit/tridiagonalsolvers declares its whole solver API as forward
+ * declarations (test_tridiagonal.cpp:17-26): free C++ functions over host
+ * pointers with a compile-time element type T and a void return. Including
+ * this header gives code written against those declarations the same names,
+ * argument order and meaning, for both float and double, forwarding to the
+ * C ABI of libtrid.so (trid.h). Like the reference they return nothing; the
+ * status of the call is available from trid_last_status(). Unlike the
+ * reference nothing exits the process, prints, or runs a timing loop.
+ */
+#ifndef TRID_COMPAT_H
+#define TRID_COMPAT_H
+
+#include "trid.h"
+
+#define TRID_COMPAT_7(NAME, CFN)                                                      \
+    inline void NAME(float* a, float* b, float* c, float* d, float* x, int systemSize, \
+                     int numSystems)                                                  \
+    {                                                                                 \
+        (void)CFN##_f32(a, b, c, d, x, systemSize, numSystems);                       \
+    }                                                                                 \
+    inline void NAME(double* a, double* b, double* c, double* d, double* x,           \
+                     int systemSize, int numSystems)                                  \
+    {                                                                                 \
+        (void)CFN##_f64(a, b, c, d, x, systemSize, numSystems);                       \
+    }
+
+#define TRID_COMPAT_8(NAME, CFN, EXTRA)                                                \
+    inline void NAME(float* a, float* b, float* c, float* d, float* x, int systemSize, \
+                     int numSystems, int EXTRA)                                       \
+    {                                                                                 \
+        (void)CFN##_f32(a, b, c, d, x, systemSize, numSystems, EXTRA);                \
+    }                                                                                 \
+    inline void NAME(double* a, double* b, double* c, double* d, double* x,           \
+                     int systemSize, int numSystems, int EXTRA)                       \
+    {                                                                                 \
+        (void)CFN##_f64(a, b, c, d, x, systemSize, numSystems, EXTRA);                \
+    }
+
+TRID_COMPAT_7(cr, trid_cr)                                   /* cr_app.cu:4          */
+TRID_COMPAT_7(pcr, trid_pcr)                                 /* pcr_app.cu:5         */
+TRID_COMPAT_8(crpcr, trid_crpcr, restSystemSize)             /* crpcr_app.cu:5       */
+TRID_COMPAT_7(crNBC, trid_crnbc)                             /* crNBC.cu:123         */
+TRID_COMPAT_8(crpcrNBC, trid_crpcrnbc, restSystemSize)       /* crNBC.cu:344         */
+TRID_COMPAT_7(crNBC32, trid_crnbc32)                         /* crNBC32banks.cu:123  */
+TRID_COMPAT_8(pcrGaussWrapper, trid_pcrgauss, gaussian)      /* pcrGauss_app.cu:164  */
+
+#undef TRID_COMPAT_7
+#undef TRID_COMPAT_8
+
+#endif /* TRID_COMPAT_H */
