@@ -162,6 +162,19 @@ int trid_solve_f64(trid_handle handle, int method,
                    const trid_opts* opts, void* stream);
 
 /*
+ * FP32 solve with FP64 iterative refinement. No counterpart in the reference
+ * (its element type is fixed at build time, common.h:5-12). a,b,c,d,x are
+ * FP64 device arrays; the solves run in FP32 with `method` (TRID_CR,
+ * TRID_CRPCR, TRID_PCR or TRID_AUTO); then `iters` times (0..8):
+ * r = d - T x in FP64, solve T e = r in FP32, x += e in FP64. One kernel per
+ * call; x is written once. Systems up to 8192 unknowns.
+ */
+int trid_solve_refined_f64(trid_handle handle, int method,
+                           const double* a, const double* b, const double* c, const double* d,
+                           double* x, int64_t systemSize, int64_t numSystems, int iters,
+                           const trid_opts* opts, void* stream);
+
+/*
  * Seeded synthetic batch written straight into device memory (same values as
  * the reference-shaped generator kinds below, bit for bit, on every run):
  *   kind 0  a,c ~ U[-1,1], b = |a|+|c|+U[1,2],      d ~ U[-1,1]

@@ -27,6 +27,7 @@ EXPORTS = (
     "trid_pcrgauss_f32", "trid_pcrgauss_f64",
     "trid_create", "trid_destroy", "trid_solve_f32", "trid_solve_f64",
     "trid_generate_f32", "trid_generate_f64", "trid_plan_query",
+    "trid_solve_refined_f64",
 )
 
 
@@ -101,6 +102,9 @@ def lib() -> ctypes.CDLL:
         g = getattr(L, f"trid_generate_{t}")
         g.argtypes = [i32, ctypes.c_uint64, P, P, P, P, i64, i64, P]
         g.restype = i32
+    L.trid_solve_refined_f64.argtypes = [P, i32, P, P, P, P, P, i64, i64, i32,
+                                         ctypes.POINTER(Opts), P]
+    L.trid_solve_refined_f64.restype = i32
     L.trid_plan_query.argtypes = [i32, i64, i64, i32, ctypes.POINTER(Opts),
                                   ctypes.POINTER(PlanInfo)]
     L.trid_plan_query.restype = i32

@@ -23,6 +23,7 @@
 // Loads and stores: each lane moves its M consecutive elements with 16-byte
 // vector accesses when the batch is dense (n == L*M, 16-byte aligned);
 // otherwise element-wise with identity rows past the end (any n >= 2).
+#include "reg_core.hpp"
 #include "reg_io.hpp"
 #include "trid_device.hpp"
 #include "trid_launch.hpp"
@@ -42,8 +43,6 @@ __global__ __launch_bounds__(256) void reg_crpcr_kernel(const T* __restrict__ A,
                                                         int64_t N, int L, int logL, int rest,
                                                         int vec)
 {
-    constexpr int S = M / R;                // in-lane stride of the PCR equations
-    constexpr int KIN = Log2<S>::value;     // in-lane CR levels
     const int lane = threadIdx.x & (kWave - 1);
     const int ll = lane & (L - 1);
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -85,131 +84,8 @@ __global__ __launch_bounds__(256) void reg_crpcr_kernel(const T* __restrict__ A,
     for (int s = 0; s < M; ++s)
         if (j0 + s == n - 1) e[s].c = T(0);
 
-    // ---- 1. in-lane CR forward levels -----------------------------------
-#pragma unroll
-    for (int k = 1; k <= KIN; ++k) {
-        const int SS = 1 << k, H = SS >> 1;
-        // The top equation of this lane needs the right lane's equation at
-        // slot H-1, which that lane finished at level k-1.
-        const Eq<T> nb = lane_read_eq(e[H - 1], lane + 1, ll + 1 < L);
-#pragma unroll
-        for (int j = 0; j < M / SS; ++j) {
-            const int i = SS * j + SS - 1;
-            const int ir = (i + H < M) ? i + H : i;
-            const Eq<T> rgt = (i + H < M) ? e[ir] : nb;
-            e[i] = reduce_eq(e[i], e[i - H], rgt);
-        }
-    }
-
     T xv[M];
-#pragma unroll
-    for (int s = 0; s < M; ++s) xv[s] = T(0);
-
-    if constexpr (R == 1) {
-        // ---- 2. cross-lane CR levels on slot M-1 while count > rest ------
-        int hs = 1;
-        for (int count = L; count > rest; count >>= 1, hs <<= 1) {
-            const bool active = ((ll + 1) & (2 * hs - 1)) == 0;
-            const Eq<T> lft = lane_read_eq(e[M - 1], lane - hs, ll - hs >= 0);
-            const Eq<T> rgt = lane_read_eq(e[M - 1], lane + hs, ll + hs < L);
-            const Eq<T> r = reduce_eq(e[M - 1], lft, rgt);
-            if (active) e[M - 1] = r;
-        }
-        // hs = L / rest: spacing of the lanes that hold the rest system
-        const int LS = hs;
-        const bool pact = ((ll + 1) & (LS - 1)) == 0;
-        // ---- 3. PCR: log2(rest/2) levels, lane distance LS .. L/4 ----------
-        for (int dist = LS; dist < (L >> 1); dist <<= 1) {
-            const Eq<T> lft = lane_read_eq(e[M - 1], lane - dist, ll - dist >= 0);
-            const Eq<T> rgt = lane_read_eq(e[M - 1], lane + dist, ll + dist < L);
-            const Eq<T> r = reduce_eq(e[M - 1], lft, rgt);
-            if (pact) e[M - 1] = r;
-        }
-        // ---- 4. 2x2 solves at lane distance L/2 --------------------------
-        const int half = L >> 1;
-        const bool lower = ll < half;
-        const Eq<T> partner = lane_read_eq(e[M - 1], lower ? lane + half : lane - half, true);
-        {
-            const Eq<T> lo = lower ? e[M - 1] : partner;
-            const Eq<T> hi = lower ? partner : e[M - 1];
-            T x1, x2;
-            solve2(lo, hi, x1, x2);
-            xv[M - 1] = lower ? x1 : x2;
-        }
-        // ---- 5a. cross-lane CR back-substitution -------------------------
-        for (int h = LS >> 1; h >= 1; h >>= 1) {
-            const T xl = lane_read_or_zero(xv[M - 1], lane - h, ll - h >= 0);
-            const T xr = lane_read_or_zero(xv[M - 1], lane + h, ll + h < L);
-            const bool solve = ((ll + 1) & (2 * h - 1)) == h;
-            const T v = backsub(e[M - 1], xl, xr);
-            if (solve) xv[M - 1] = v;
-        }
-    } else {
-        // rest = R * L: lane ll holds PCR equations q = ll*R + r at slots
-        // S*r + S - 1, r = 0..R-1.
-        // ---- 3a. PCR levels with distance < R (partly inside the lane) ---
-#pragma unroll
-        for (int p = 0; (1 << p) < R; ++p) {
-            const int dl = 1 << p;
-            Eq<T> fromL[R], fromR[R];
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-                if (q < dl) {
-                    fromL[q] = lane_read_eq(e[S * (R - dl + q) + S - 1], lane - 1, ll >= 1);
-                    fromR[q] = lane_read_eq(e[S * q + S - 1], lane + 1, ll + 1 < L);
-                }
-            }
-            Eq<T> nw[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const Eq<T> lft = (r - dl >= 0) ? e[S * ((r - dl >= 0) ? r - dl : 0) + S - 1]
-                                                : fromL[(r - dl >= 0) ? 0 : r];
-                const Eq<T> rgt = (r + dl < R) ? e[S * ((r + dl < R) ? r + dl : 0) + S - 1]
-                                               : fromR[(r + dl < R) ? 0 : r + dl - R];
-                nw[r] = reduce_eq(e[S * r + S - 1], lft, rgt);
-            }
-#pragma unroll
-            for (int r = 0; r < R; ++r) e[S * r + S - 1] = nw[r];
-        }
-        // ---- 3b. PCR levels with distance >= R: lane distance 1 .. L/4 ---
-        for (int ld = 1; ld < (L >> 1); ld <<= 1) {
-            Eq<T> nw[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const Eq<T> lft = lane_read_eq(e[S * r + S - 1], lane - ld, ll - ld >= 0);
-                const Eq<T> rgt = lane_read_eq(e[S * r + S - 1], lane + ld, ll + ld < L);
-                nw[r] = reduce_eq(e[S * r + S - 1], lft, rgt);
-            }
-#pragma unroll
-            for (int r = 0; r < R; ++r) e[S * r + S - 1] = nw[r];
-        }
-        // ---- 4. 2x2 solves at lane distance L/2 --------------------------
-        const int half = L >> 1;
-        const bool lower = ll < half;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const Eq<T> own = e[S * r + S - 1];
-            const Eq<T> partner = lane_read_eq(own, lower ? lane + half : lane - half, true);
-            const Eq<T> lo = lower ? own : partner;
-            const Eq<T> hi = lower ? partner : own;
-            T x1, x2;
-            solve2(lo, hi, x1, x2);
-            xv[S * r + S - 1] = lower ? x1 : x2;
-        }
-    }
-
-    // ---- 5b. in-lane CR back-substitution --------------------------------
-    const T xprev = lane_read_or_zero(xv[M - 1], lane - 1, ll >= 1);
-#pragma unroll
-    for (int k = KIN; k >= 1; --k) {
-        const int SS = 1 << k, H = SS >> 1;
-#pragma unroll
-        for (int j = 0; j < M / SS; ++j) {
-            const int i = SS * j + H - 1;
-            const T xl = (i - H >= 0) ? xv[(i - H >= 0) ? i - H : 0] : xprev;
-            xv[i] = backsub(e[i], xl, xv[i + H]);
-        }
-    }
+    wave_core<T, M, R>(e, xv, lane, ll, L, rest);
 
     if constexpr (M % E == 0 && IO >= 2) {
         if (vec) {

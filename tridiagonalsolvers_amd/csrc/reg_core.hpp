@@ -1,0 +1,336 @@
+// reg_core.hpp -- the register-resident CR / CR-PCR / PCR solve of one
+// system held by a segment of L lanes (M consecutive unknowns per lane),
+// shared by the batch kernel (kern_reg.hip), the refinement kernels
+// (kern_refine.hip) and the ragged kernel (kern_ragged.hip).
+//
+//   1. in-lane CR levels      (stride 2..M; the top equation of a lane reads
+//                              the right lane's "prefix" equation by bpermute)
+//   2. cross-lane CR levels   (while more than `rest` equations remain)
+//   3. PCR on the `rest` system, cross-lane, all reads before writes because
+//      every level works on registers
+//   4. 2x2 Cramer solves at lane distance L/2
+//   5. CR back-substitution across lanes, then inside lanes
+//
+// R: equations per lane handed to PCR (R = 1: rest <= L; R > 1: rest = R*L).
+// On return xv holds the lane's M unknowns; e holds the reduced equations.
+#pragma once
+
+#include "trid_device.hpp"
+
+namespace trid {
+
+template <typename T, int M, int R>
+__device__ __forceinline__ void wave_core(Eq<T> (&e)[M], T (&xv)[M], int lane, int ll, int L,
+                                          int rest)
+{
+    constexpr int S = M / R;             // in-lane stride of the PCR equations
+    constexpr int KIN = Log2<S>::value;  // in-lane CR levels
+
+    // ---- 1. in-lane CR forward levels -----------------------------------
+#pragma unroll
+    for (int k = 1; k <= KIN; ++k) {
+        const int SS = 1 << k, H = SS >> 1;
+        // The top equation of this lane needs the right lane's equation at
+        // slot H-1, which that lane finished at level k-1.
+        const Eq<T> nb = lane_read_eq(e[H - 1], lane + 1, ll + 1 < L);
+#pragma unroll
+        for (int j = 0; j < M / SS; ++j) {
+            const int i = SS * j + SS - 1;
+            const int ir = (i + H < M) ? i + H : i;
+            const Eq<T> rgt = (i + H < M) ? e[ir] : nb;
+            e[i] = reduce_eq(e[i], e[i - H], rgt);
+        }
+    }
+
+#pragma unroll
+    for (int s = 0; s < M; ++s) xv[s] = T(0);
+
+    if constexpr (R == 1) {
+        // ---- 2. cross-lane CR levels on slot M-1 while count > rest ------
+        int hs = 1;
+        for (int count = L; count > rest; count >>= 1, hs <<= 1) {
+            const bool active = ((ll + 1) & (2 * hs - 1)) == 0;
+            const Eq<T> lft = lane_read_eq(e[M - 1], lane - hs, ll - hs >= 0);
+            const Eq<T> rgt = lane_read_eq(e[M - 1], lane + hs, ll + hs < L);
+            const Eq<T> r = reduce_eq(e[M - 1], lft, rgt);
+            if (active) e[M - 1] = r;
+        }
+        // hs = L / rest: spacing of the lanes that hold the rest system
+        const int LS = hs;
+        const bool pact = ((ll + 1) & (LS - 1)) == 0;
+        // ---- 3. PCR: log2(rest/2) levels, lane distance LS .. L/4 ----------
+        for (int dist = LS; dist < (L >> 1); dist <<= 1) {
+            const Eq<T> lft = lane_read_eq(e[M - 1], lane - dist, ll - dist >= 0);
+            const Eq<T> rgt = lane_read_eq(e[M - 1], lane + dist, ll + dist < L);
+            const Eq<T> r = reduce_eq(e[M - 1], lft, rgt);
+            if (pact) e[M - 1] = r;
+        }
+        // ---- 4. 2x2 solves at lane distance L/2 --------------------------
+        const int half = L >> 1;
+        const bool lower = ll < half;
+        const Eq<T> partner = lane_read_eq(e[M - 1], lower ? lane + half : lane - half, true);
+        {
+            const Eq<T> lo = lower ? e[M - 1] : partner;
+            const Eq<T> hi = lower ? partner : e[M - 1];
+            T x1, x2;
+            solve2(lo, hi, x1, x2);
+            xv[M - 1] = lower ? x1 : x2;
+        }
+        // ---- 5a. cross-lane CR back-substitution -------------------------
+        for (int h = LS >> 1; h >= 1; h >>= 1) {
+            const T xl = lane_read_or_zero(xv[M - 1], lane - h, ll - h >= 0);
+            const T xr = lane_read_or_zero(xv[M - 1], lane + h, ll + h < L);
+            const bool solve = ((ll + 1) & (2 * h - 1)) == h;
+            const T v = backsub(e[M - 1], xl, xr);
+            if (solve) xv[M - 1] = v;
+        }
+    } else {
+        // rest = R * L: lane ll holds PCR equations q = ll*R + r at slots
+        // S*r + S - 1, r = 0..R-1.
+        // ---- 3a. PCR levels with distance < R (partly inside the lane) ---
+#pragma unroll
+        for (int p = 0; (1 << p) < R; ++p) {
+            const int dl = 1 << p;
+            Eq<T> fromL[R], fromR[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                if (q < dl) {
+                    fromL[q] = lane_read_eq(e[S * (R - dl + q) + S - 1], lane - 1, ll >= 1);
+                    fromR[q] = lane_read_eq(e[S * q + S - 1], lane + 1, ll + 1 < L);
+                }
+            }
+            Eq<T> nw[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const Eq<T> lft = (r - dl >= 0) ? e[S * ((r - dl >= 0) ? r - dl : 0) + S - 1]
+                                                : fromL[(r - dl >= 0) ? 0 : r];
+                const Eq<T> rgt = (r + dl < R) ? e[S * ((r + dl < R) ? r + dl : 0) + S - 1]
+                                               : fromR[(r + dl < R) ? 0 : r + dl - R];
+                nw[r] = reduce_eq(e[S * r + S - 1], lft, rgt);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) e[S * r + S - 1] = nw[r];
+        }
+        // ---- 3b. PCR levels with distance >= R: lane distance 1 .. L/4 ---
+        for (int ld = 1; ld < (L >> 1); ld <<= 1) {
+            Eq<T> nw[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const Eq<T> lft = lane_read_eq(e[S * r + S - 1], lane - ld, ll - ld >= 0);
+                const Eq<T> rgt = lane_read_eq(e[S * r + S - 1], lane + ld, ll + ld < L);
+                nw[r] = reduce_eq(e[S * r + S - 1], lft, rgt);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) e[S * r + S - 1] = nw[r];
+        }
+        // ---- 4. 2x2 solves at lane distance L/2 --------------------------
+        const int half = L >> 1;
+        const bool lower = ll < half;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const Eq<T> own = e[S * r + S - 1];
+            const Eq<T> partner = lane_read_eq(own, lower ? lane + half : lane - half, true);
+            const Eq<T> lo = lower ? own : partner;
+            const Eq<T> hi = lower ? partner : own;
+            T x1, x2;
+            solve2(lo, hi, x1, x2);
+            xv[S * r + S - 1] = lower ? x1 : x2;
+        }
+    }
+
+    // ---- 5b. in-lane CR back-substitution --------------------------------
+    const T xprev = lane_read_or_zero(xv[M - 1], lane - 1, ll >= 1);
+#pragma unroll
+    for (int k = KIN; k >= 1; --k) {
+        const int SS = 1 << k, H = SS >> 1;
+#pragma unroll
+        for (int j = 0; j < M / SS; ++j) {
+            const int i = SS * j + H - 1;
+            const T xl = (i - H >= 0) ? xv[(i - H >= 0) ? i - H : 0] : xprev;
+            xv[i] = backsub(e[i], xl, xv[i + H]);
+        }
+    }
+
+}
+
+// Multi-wave form: W waves of one workgroup hold one system, wave w owning
+// unknowns [w*64*M, (w+1)*64*M) (see kern_reg_mw.hip). Every thread of the
+// workgroup must call it (it contains barriers). `xchg` is LDS for
+// W*(log2(M)+6) + 64*W equations and 64*W values; the caller's staging area
+// may alias it (a barrier is taken before first use).
+template <typename T, int M>
+__device__ __forceinline__ void wg_core(Eq<T> (&e)[M], T (&xv)[M], int w, int W, int lane,
+                                        int rest, char* xchg)
+{
+    constexpr int LOGM = Log2<M>::value;
+    constexpr int NPRE = LOGM + 6;  // published prefix equations per wave
+    Eq<T>* pre = reinterpret_cast<Eq<T>*>(xchg);   // [W][NPRE]
+    Eq<T>* red = pre + W * NPRE;                     // [64*W]
+    T* xs = reinterpret_cast<T*>(red + kWave * W);   // [64*W]
+
+    const Eq<T> top0 = e[M - 1];  // level-0 top equation (lane 63's chain restarts here)
+    __syncthreads();              // staging rows become the exchange area
+
+    // ---- in-lane CR levels (lane 63's top slot is provisional) ------------
+#pragma unroll
+    for (int k = 1; k <= LOGM; ++k) {
+        const int SS = 1 << k, H = SS >> 1;
+        const Eq<T> nb = lane_read_eq(e[H - 1], lane + 1, lane + 1 < kWave);
+#pragma unroll
+        for (int j = 0; j < M / SS; ++j) {
+            const int i = SS * j + SS - 1;
+            const int ir = (i + H < M) ? i + H : i;
+            const Eq<T> rgt = (i + H < M) ? e[ir] : nb;
+            e[i] = reduce_eq(e[i], e[i - H], rgt);
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 1; k <= LOGM; ++k) pre[w * NPRE + k - 1] = e[(1 << (k - 1)) - 1];
+    }
+
+    // ---- cross-lane CR levels while more than max(rest, W) remain ---------
+    int hs = 1;
+    for (int count = kWave * W; count > rest && hs < kWave; count >>= 1, hs <<= 1) {
+        const bool active = ((lane + 1) & (2 * hs - 1)) == 0;
+        const Eq<T> lft = lane_read_eq(e[M - 1], lane - hs, lane - hs >= 0);
+        const Eq<T> rgt = lane_read_eq(e[M - 1], lane + hs, lane + hs < kWave);
+        const Eq<T> r = reduce_eq(e[M - 1], lft, rgt);
+        if (active) e[M - 1] = r;
+    }
+    const int klane = __builtin_ctz((unsigned)hs);  // cross-lane levels done
+    // prefix of cross-lane level j: lane 2^(j-1)-1 after level j-1 (final)
+    if (((lane + 1) & lane) == 0 && lane + 1 <= (hs >> 1))
+        pre[w * NPRE + LOGM + __builtin_ctz((unsigned)(lane + 1))] = e[M - 1];
+    __syncthreads();
+
+    // ---- redo lane 63's chain with the next wave's prefixes ---------------
+    if (w + 1 < W) {
+        const Eq<T>* nxt = pre + (w + 1) * NPRE;
+        Eq<T> t = lane_bcast_eq(top0, kWave - 1);
+#pragma unroll
+        for (int k = 1; k <= LOGM; ++k) {
+            const int H = 1 << (k - 1);
+            t = reduce_eq(t, lane_bcast_eq(e[M - 1 - H], kWave - 1), nxt[k - 1]);
+        }
+        for (int j = 1; j <= klane; ++j) {
+            const int h = 1 << (j - 1);
+            t = reduce_eq(t, lane_bcast_eq(e[M - 1], kWave - 1 - h), nxt[LOGM + j - 1]);
+        }
+        if (lane == kWave - 1) e[M - 1] = t;
+    }
+
+#pragma unroll
+    for (int s = 0; s < M; ++s) xv[s] = T(0);
+    T xprevw;  // top unknown of the previous wave (0 for wave 0)
+
+    if (rest <= W) {
+        // ---- wave-top system: W equations, solved by wave 0 ---------------
+        if (lane == kWave - 1) red[w] = e[M - 1];
+        __syncthreads();
+        if (w == 0) {
+            const int L = W;
+            const int ll = lane & (L - 1);
+            Eq<T> t = red[ll];
+            int h2 = 1;
+            for (int count = L; count > rest; count >>= 1, h2 <<= 1) {
+                const bool active = ((ll + 1) & (2 * h2 - 1)) == 0;
+                const Eq<T> lft = lane_read_eq(t, lane - h2, ll - h2 >= 0);
+                const Eq<T> rgt = lane_read_eq(t, lane + h2, ll + h2 < L);
+                const Eq<T> r = reduce_eq(t, lft, rgt);
+                if (active) t = r;
+            }
+            const int LS = h2;
+            const bool pact = ((ll + 1) & (LS - 1)) == 0;
+            for (int dist = LS; dist < (L >> 1); dist <<= 1) {
+                const Eq<T> lft = lane_read_eq(t, lane - dist, ll - dist >= 0);
+                const Eq<T> rgt = lane_read_eq(t, lane + dist, ll + dist < L);
+                const Eq<T> r = reduce_eq(t, lft, rgt);
+                if (pact) t = r;
+            }
+            const int half = L >> 1;
+            const bool lower = ll < half;
+            const Eq<T> partner = lane_read_eq(t, lower ? lane + half : lane - half, true);
+            T x1, x2;
+            solve2(lower ? t : partner, lower ? partner : t, x1, x2);
+            T xt = lower ? x1 : x2;
+            for (int h = LS >> 1; h >= 1; h >>= 1) {
+                const T xl = lane_read_or_zero(xt, lane - h, ll - h >= 0);
+                const T xr = lane_read_or_zero(xt, lane + h, ll + h < L);
+                const bool solve = ((ll + 1) & (2 * h - 1)) == h;
+                const T v = backsub(t, xl, xr);
+                if (solve) xt = v;
+            }
+            if (lane < W) xs[lane] = xt;
+        }
+        __syncthreads();
+        if (lane == kWave - 1) xv[M - 1] = xs[w];
+        xprevw = w > 0 ? xs[w - 1] : T(0);
+    } else {
+        // ---- PCR on `rest` equations spread over the workgroup ------------
+        const int LS = hs;  // lanes between consecutive PCR equations
+        const bool pact = ((lane + 1) & (LS - 1)) == 0;
+        const int q = (w * kWave + lane + 1) / LS - 1;
+        if (pact) red[q] = e[M - 1];
+        __syncthreads();
+        for (int delta = 1; delta < (rest >> 1); delta <<= 1) {
+            Eq<T> nw = e[M - 1];
+            if (pact) {
+                const Eq<T> lft = q - delta >= 0 ? red[q - delta] : identity_eq<T>();
+                const Eq<T> rgt = q + delta < rest ? red[q + delta] : identity_eq<T>();
+                nw = reduce_eq(e[M - 1], lft, rgt);
+            }
+            __syncthreads();
+            if (pact) {
+                red[q] = nw;
+                e[M - 1] = nw;
+            }
+            __syncthreads();
+        }
+        if (pact) {
+            const int half = rest >> 1;
+            const bool lower = q < half;
+            const Eq<T> partner = red[lower ? q + half : q - half];
+            T x1, x2;
+            solve2(lower ? e[M - 1] : partner, lower ? partner : e[M - 1], x1, x2);
+            xv[M - 1] = lower ? x1 : x2;
+            xs[q] = xv[M - 1];
+        }
+        __syncthreads();
+        xprevw = w > 0 ? xs[(w * kWave) / LS - 1] : T(0);
+    }
+
+    // ---- cross-lane CR back-substitution ----------------------------------
+    for (int h = hs >> 1; h >= 1; h >>= 1) {
+        const T xl0 = lane_read(xv[M - 1], (lane - h) & (kWave - 1));
+        const T xl = lane - h >= 0 ? xl0 : xprevw;
+        const T xr = lane_read(xv[M - 1], (lane + h) & (kWave - 1));
+        const bool solve = ((lane + 1) & (2 * h - 1)) == h;
+        const T v = backsub(e[M - 1], xl, xr);
+        if (solve) xv[M - 1] = v;
+    }
+
+    // ---- in-lane back-substitution ------------------------------------------
+    const T xp0 = lane_read(xv[M - 1], (lane - 1) & (kWave - 1));
+    const T xprev = lane >= 1 ? xp0 : xprevw;
+#pragma unroll
+    for (int k = LOGM; k >= 1; --k) {
+        const int SS = 1 << k, H = SS >> 1;
+#pragma unroll
+        for (int j = 0; j < M / SS; ++j) {
+            const int i = SS * j + H - 1;
+            const T xl = (i - H >= 0) ? xv[(i - H >= 0) ? i - H : 0] : xprev;
+            xv[i] = backsub(e[i], xl, xv[i + H]);
+        }
+    }
+
+}
+
+template <typename T, int M>
+constexpr size_t wg_core_lds_bytes(int W)
+{
+    return (size_t)W * (Log2<M>::value + 6) * sizeof(Eq<T>) + (size_t)kWave * W * sizeof(Eq<T>) +
+           (size_t)kWave * W * sizeof(T);
+}
+
+}  // namespace trid

@@ -283,6 +283,38 @@ int trid_solve_f64(trid_handle handle, int method, const double* a, const double
                                          opts, static_cast<hipStream_t>(stream)));
 }
 
+int trid_solve_refined_f64(trid_handle handle, int method, const double* a, const double* b,
+                           const double* c, const double* d, double* x, int64_t systemSize,
+                           int64_t numSystems, int iters, const trid_opts* opts, void* stream)
+{
+    trid_opts o{};
+    if (opts) o = *opts;
+    if (!handle || !a || !b || !c || !d || !x) return set_status(TRID_EINVAL);
+    if (systemSize < 2 || numSystems < 0 || iters < 0 || iters > 8) return set_status(TRID_EINVAL);
+    if (o.flags != 0) return set_status(TRID_EINVAL);
+    if (numSystems == 0) return set_status(TRID_OK);
+    trid::Plan p;
+    int st = trid::make_refine_plan(method, systemSize, numSystems, o, &p);
+    if (st != TRID_OK) return set_status(st);
+    st = check_device();
+    if (st != TRID_OK) return set_status(st);
+    trid::RefineArgs g{};
+    g.a = a;
+    g.b = b;
+    g.c = c;
+    g.d = d;
+    g.x = x;
+    g.n = systemSize;
+    g.N = numSystems;
+    g.M = p.M;
+    g.L = p.L;
+    g.rest = p.rest;
+    g.iters = iters;
+    g.vec = (p.dense && aligned16(a) && aligned16(b) && aligned16(c) && aligned16(d)) ? 1 : 0;
+    g.stream = static_cast<hipStream_t>(stream);
+    return set_status(from_hip(trid::launch_refine(g)));
+}
+
 int trid_generate_f32(int kind, uint64_t seed, float* a, float* b, float* c, float* d,
                       int64_t systemSize, int64_t numSystems, void* stream)
 {

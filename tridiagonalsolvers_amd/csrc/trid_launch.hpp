@@ -72,6 +72,25 @@ struct PcrThomasArgs {
 template <typename T>
 hipError_t launch_pcr_thomas(const PcrThomasArgs<T>& g);
 
+// FP32 solve + FP64 residual refinement (kern_refine.hip).
+struct RefineArgs {
+    const double* a;
+    const double* b;
+    const double* c;
+    const double* d;
+    double* x;
+    int64_t n;
+    int64_t N;
+    int M;       // unknowns per lane
+    int L;       // lanes per system (<= 64: one wave; 64*W: W waves)
+    int rest;    // PCR system size of the FP32 solves
+    int iters;   // refinement steps
+    int vec;     // 1: dense and 16-byte aligned
+    hipStream_t stream;
+};
+
+hipError_t launch_refine(const RefineArgs& g);
+
 // Seeded synthetic batch (kern_util.hip).
 template <typename T>
 hipError_t launch_generate(int kind, uint64_t seed, T* a, T* b, T* c, T* d, int64_t n,

@@ -170,6 +170,27 @@ int make_plan(int method, int64_t n, int64_t N, int elem_size, const trid_opts& 
     }
 }
 
+int make_refine_plan(int method, int64_t n, int64_t N, const trid_opts& o, Plan* p)
+{
+    if (method == TRID_AUTO) method = TRID_CRPCR;
+    if (method != TRID_CR && method != TRID_CRPCR && method != TRID_PCR) return TRID_ENOTSUP;
+    trid_opts q = o;
+    const int64_t n_pad = next_pow2(n < 2 ? 2 : n);
+    if (n_pad > 8192) return TRID_ENOTSUP;
+    if (q.lanes == 0 && q.per_lane == 0) {
+        // FP64 unknowns live in registers beside the FP32 equations: at most
+        // 8 unknowns per lane (one wave up to n = 512, W waves above)
+        const int64_t m = n_pad / 2 < 8 ? n_pad / 2 : 8;
+        q.per_lane = (int32_t)m;
+        q.lanes = (int32_t)(n_pad / m);
+    }
+    int st = make_plan(method, n, N, 4, q, p);
+    if (st != TRID_OK) return st;
+    if (p->family != Family::Reg) return TRID_ENOTSUP;
+    if (p->L > 64 && p->M > 8) return TRID_EINVAL;
+    return TRID_OK;
+}
+
 }  // namespace trid
 
 extern "C" int trid_plan_query(int method, int64_t systemSize, int64_t numSystems,

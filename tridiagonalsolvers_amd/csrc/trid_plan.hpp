@@ -34,4 +34,8 @@ struct Plan {
 // Returns TRID_OK and fills *p, or TRID_EINVAL / TRID_ENOTSUP.
 int make_plan(int method, int64_t n, int64_t N, int elem_size, const trid_opts& o, Plan* p);
 
+// Plan of the FP32-solve / FP64-refinement kernels (register family only:
+// CR, CR-PCR, PCR up to n = 1024, CR / CR-PCR up to n = 8192).
+int make_refine_plan(int method, int64_t n, int64_t N, const trid_opts& o, Plan* p);
+
 }  // namespace trid

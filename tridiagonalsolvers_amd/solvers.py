@@ -157,6 +157,16 @@ class Handle:
                ctypes.c_void_p(_stream_ptr(stream)))
         check(st, "trid_solve")
 
+    def solve_refined(self, method, a, b, c, d, x, systemSize: int, numSystems: int,
+                      iters: int = 2, opts: Opts | None = None, stream=None) -> None:
+        """FP64 arrays; FP32 solves plus `iters` FP64 residual corrections."""
+        o = opts if opts is not None else Opts()
+        st = lib().trid_solve_refined_f64(
+            self._h, method_id(method), *(ctypes.c_void_p(_ptr(v)) for v in (a, b, c, d, x)),
+            int(systemSize), int(numSystems), int(iters), ctypes.byref(o),
+            ctypes.c_void_p(_stream_ptr(stream)))
+        check(st, "trid_solve_refined_f64")
+
 
 def generate(kind: int, seed: int, a, b, c, d, systemSize: int, numSystems: int,
              stream=None, dtype: str | None = None) -> None:
