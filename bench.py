@@ -29,6 +29,9 @@ METRIC = ("unknowns solved/s + % HBM roofline; served p50/p99 ms vs plain, "
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--workload", default="uniform", choices=["uniform", "ragged"],
+                   help="uniform: the headline batch; ragged: 100,000 systems of "
+                        "log-uniform sizes in [16, 8192], FP32 solve + FP64 refinement")
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--method", default="crpcr")
@@ -94,6 +97,123 @@ def cpu_baseline(args, x_gpu_np, n, N):
     }
 
 
+def ragged_cpu_baseline(args, host, offs, x_gpu):
+    """FP64 Thomas of the oracle on the same ragged batch, OpenMP over all host
+    cores, repeated for about --cpu-seconds."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    a, b, c, d = host
+    threads = O.max_threads()
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        xref = O.thomas(a, b, c, d, offsets=offs, threads=threads)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= args.cpu_seconds:
+            break
+    rel, _, fails = O.compare(x_gpu, xref, offsets=offs)
+    res = O.residual(a, b, c, d, x_gpu, offsets=offs)
+    return {"value": reps * int(offs[-1]) / el, "unit": "unknowns/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle FP64 Thomas, full ragged batch x {reps} passes, {el:.1f} s, "
+                      f"OpenMP {threads} threads",
+            "gpu_max_rel_err_vs_fp64_thomas": float(np.max(rel)),
+            "gpu_max_rel_residual_host_long_double": float(np.max(res)),
+            "gpu_reference_rms_failures": int(fails)}
+
+
+def bench_ragged(args):
+    """C4: 100,000 systems, sizes round(exp(U[ln 16, ln 8192])) (seed 1), FP64
+    data; one graph replay of the persistent ragged kernel (FP32 solves + 2
+    FP64 refinement steps) against plain FP64 launches, one per distinct
+    size, on the same data (sorted by size so each size is contiguous)."""
+    import numpy as np
+    import torch
+
+    import tridiagonalsolvers_amd as T
+    import tridiagonalsolvers_amd.ragged as R
+
+    sizes = np.sort(R.log_uniform_sizes(100000, 16, 8192, seed=1))
+    offs = R.offsets_of(sizes)
+    total = int(offs[-1])
+    N = len(sizes)
+    dev = torch.device("cuda:0")
+    a, b, c, d = (torch.empty(total, dtype=torch.float64, device=dev) for _ in range(4))
+    x = torch.empty_like(a)
+    x_plain = torch.empty_like(a)
+    stream = torch.cuda.current_stream()
+    # every row strongly dominant: a,c ~ U[-1,1], b = |a|+|c|+U[1,2]; the
+    # first/last rows' a/c are ignored by the solvers
+    T.generate(0, 1, a, b, c, d, total, 1, stream=stream)
+    plan = R.RaggedPlan(offs)
+    info = plan.info()
+    h = T.Handle()
+    uniq, first = np.unique(sizes, return_index=True)
+    runs = list(zip(uniq.tolist(), first.tolist(),
+                    np.diff(np.append(first, N)).tolist()))
+
+    def ragged_step():
+        plan.solve(a, b, c, d, x, iters=2, stream=stream)
+
+    def plain_step():
+        for n, f, cnt in runs:
+            o = int(offs[f]) * 8
+            h.solve("crpcr", a.data_ptr() + o, b.data_ptr() + o, c.data_ptr() + o,
+                    d.data_ptr() + o, x_plain.data_ptr() + o, n, cnt, stream=stream,
+                    dtype="f64")
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps, e0.elapsed_time(e1) / steps
+
+    wall_r, gpu_r = timed(ragged_step, args.steps, args.warmup)
+    wall_p, gpu_p = timed(plain_step, max(1, args.steps // 5), 1)
+    d_offs = torch.from_numpy(offs).to(dev)
+    rr = torch.empty(N, dtype=torch.float64, device=dev)
+    R.residual(a, b, c, d, x, d_offs, N, rr, stream=stream)
+    rr_r = float(rr.max())
+    R.residual(a, b, c, d, x_plain, d_offs, N, rr, stream=stream)
+    rr_p = float(rr.max())
+    alg = 5 * 8 * total
+    out = {
+        "metric": METRIC, "value": total / wall_r, "unit": "unknowns/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall_r * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded device generator, FP64; sizes log-uniform in [16, 8192], seed 1)",
+        "config": {"workload": f"ragged f64 {N} systems, sizes log-uniform [16, 8192], "
+                               f"FP32 solve + 2 FP64 refinements, one graph launch",
+                   "systems": N, "unknowns": total, "plan": info},
+        "kernel_ms": {"ragged_graph": gpu_r, "plain_per_size": gpu_p},
+        "roofline": {"bound": "hbm", "achieved": alg / (gpu_r * 1e-3) / 1e9,
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": alg / (gpu_r * 1e-3) / 1e9 / PEAK_HBM_GBS, "traffic": None,
+                     "note": "algorithmic 5*8 B per unknown (FP64 in/out); refinement "
+                             "re-reads are not credited"},
+        "plain_per_size": {"value": total / wall_p, "launches": len(runs),
+                           "ms_per_step": wall_p * 1e3, "max_rel_residual": rr_p},
+        "speedup_vs_plain_per_size": wall_p / wall_r,
+        "max_rel_residual": rr_r,
+    }
+    if not args.no_cpu:
+        host = [v.cpu().numpy() for v in (a, b, c, d)]
+        out["cpu_baseline"] = ragged_cpu_baseline(args, host, offs, x.cpu().numpy())
+    plan.close()
+    h.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -101,6 +221,9 @@ def main():
               "(independent systems shard by system range with no exchange).",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ragged":
+        bench_ragged(args)
+        return
     import torch
 
     import tridiagonalsolvers_amd as T

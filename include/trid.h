@@ -175,6 +175,17 @@ int trid_solve_refined_f64(trid_handle handle, int method,
                            const trid_opts* opts, void* stream);
 
 /*
+ * Relative residual of every system, in FP64 on the device:
+ * relres[s] = max_i |(d - T x)_i| / max_i |d_i| over system s =
+ * [offsets[s], offsets[s+1]) (device arrays; a[first], c[last] ignored).
+ * No counterpart in the reference (its check is an RMS against the CPU
+ * solution, tridiagonal_gold.h:97-148).
+ */
+int trid_residual_f64(const double* a, const double* b, const double* c, const double* d,
+                      const double* x, const int64_t* offsets, int64_t numSystems,
+                      double* relres, void* stream);
+
+/*
  * Seeded synthetic batch written straight into device memory (same values as
  * the reference-shaped generator kinds below, bit for bit, on every run):
  *   kind 0  a,c ~ U[-1,1], b = |a|+|c|+U[1,2],      d ~ U[-1,1]
@@ -186,6 +197,55 @@ int trid_generate_f32(int kind, uint64_t seed, float* a, float* b, float* c, flo
                       int64_t systemSize, int64_t numSystems, void* stream);
 int trid_generate_f64(int kind, uint64_t seed, double* a, double* b, double* c, double* d,
                       int64_t systemSize, int64_t numSystems, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Ragged batches: one persistent, graph-captured launch                    */
+/* ---------------------------------------------------------------------- */
+/*
+ * No counterpart in the reference (one systemSize per call). System s of a
+ * ragged batch occupies [offsets[s], offsets[s+1]) of flat FP64 device
+ * arrays a, b, c, d, x; sizes 2..8192. A plan buckets the systems by padded
+ * size (power of two) and builds the work queue of one persistent kernel
+ * (workgroup items for 1024..8192, wave items below); each bucket solves in
+ * FP32 with its (per_lane, lanes, rest) configuration and refines in FP64
+ * `iters` times. The plan depends only on the sizes and the table.
+ */
+typedef struct trid_bucket_cfg {
+    int32_t size;      /* padded system size (power of two, 2..8192)            */
+    int32_t per_lane;  /* unknowns per lane: <= 8                               */
+    int32_t lanes;     /* lanes per system: size/per_lane (<= 64 below 1024)    */
+    int32_t rest;      /* CR -> PCR switch size, power of two <= lanes          */
+} trid_bucket_cfg;
+
+typedef struct trid_ragged_info {
+    int64_t num_systems;
+    int64_t total_unknowns;
+    int32_t num_buckets;
+    int32_t workgroup_items;
+    int32_t wave_items;
+    int32_t grid;        /* persistent workgroups launched */
+} trid_ragged_info;
+
+typedef struct trid_ragged_plan_s* trid_ragged_plan;
+
+/* Do not capture/replay a hipGraph (a plain launch); also used when stream is
+ * NULL, which cannot be captured. */
+#define TRID_RAGGED_NO_GRAPH 1
+
+/* offsets: host array of numSystems+1 element offsets. table: optional
+ * per-bucket configurations (e.g. from the autotuner); buckets not listed use
+ * the library default. */
+int trid_ragged_plan_create(trid_ragged_plan* plan, const int64_t* offsets, int64_t numSystems,
+                            const trid_bucket_cfg* table, int tableSize);
+int trid_ragged_plan_destroy(trid_ragged_plan plan);
+int trid_ragged_plan_info(trid_ragged_plan plan, trid_ragged_info* info,
+                          trid_bucket_cfg* buckets, int64_t* bucketSystems, int capacity);
+/* Enqueue the ragged solve on `stream`. The first call for a given set of
+ * pointers and `iters` captures the reset + persistent launch into a
+ * hipGraph; later calls replay it. */
+int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* b,
+                          const double* c, const double* d, double* x, int iters, int flags,
+                          void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* Plan introspection (host only; no device needed)                         */

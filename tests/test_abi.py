@@ -106,3 +106,46 @@ def test_plan_query_nbc_and_gauss(trid):
     assert q["family"] == 1 and q["rest"] == 128
     q = trid.plan_query("pcrgauss", 1024, 1, 4, trid.make_opts(gaussian=64))
     assert q["family"] == 2 and q["gaussian"] == 64
+
+
+def test_ragged_plan_rejects_bad_sizes_without_a_device(trid):
+    import tridiagonalsolvers_amd.ragged as R
+    with pytest.raises(trid.TridError) as e:
+        R.RaggedPlan(np.array([0, 5, 6, 10], dtype=np.int64), table=[])   # a size of 1
+    assert e.value.status == 1
+    with pytest.raises(trid.TridError) as e:
+        R.RaggedPlan(np.array([0, 5, 3], dtype=np.int64), table=[])       # negative size
+    assert e.value.status == 1
+    with pytest.raises(trid.TridError) as e:
+        R.RaggedPlan(np.array([0, 10000], dtype=np.int64), table=[])      # above 8192
+    assert e.value.status == 4
+    bad = [dict(size=64, per_lane=16, lanes=4, rest=4)]                    # per_lane > 8
+    with pytest.raises(trid.TridError):
+        R.RaggedPlan(np.array([0, 64], dtype=np.int64), table=bad)
+    bad = [dict(size=64, per_lane=8, lanes=8, rest=16)]                    # rest > lanes
+    with pytest.raises(trid.TridError):
+        R.RaggedPlan(np.array([0, 64], dtype=np.int64), table=bad)
+
+
+def test_autotune_candidates_are_valid_configs():
+    from tridiagonalsolvers_amd.autotune import candidates
+    for k in range(1, 14):
+        size = 1 << k
+        cs = candidates(size)
+        assert cs, size
+        for c in cs:
+            assert c["per_lane"] * c["lanes"] == size
+            assert c["per_lane"] <= 8 and 2 <= c["rest"] <= c["lanes"]
+            if size <= 512:
+                assert c["lanes"] <= 64
+
+
+def test_log_uniform_sizes_are_seeded_and_in_range():
+    import tridiagonalsolvers_amd.ragged as R
+    s1 = R.log_uniform_sizes(10000, 16, 8192, seed=1)
+    s2 = R.log_uniform_sizes(10000, 16, 8192, seed=1)
+    assert np.array_equal(s1, s2)
+    assert s1.min() >= 16 and s1.max() <= 8192
+    # log-uniform: about a third of the systems at or above 1024
+    frac = float(np.mean(s1 >= 1024))
+    assert 0.28 < frac < 0.39

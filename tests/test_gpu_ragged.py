@@ -1,0 +1,127 @@
+"""GPU tests of the ragged persistent solve (trid_ragged_*): every system of a
+mixed-size batch against the FP64 Thomas oracle, FP64-level residuals after
+refinement, bit-identical results alone and in the mix, graph replays equal
+to the eager launch, and every autotuner candidate within tolerance."""
+import numpy as np
+import pytest
+
+from test_gpu_parity import check
+
+pytestmark = pytest.mark.gpu
+
+EDGE_SIZES = [2, 3, 5, 16, 17, 100, 511, 512, 513, 1000, 1024, 1025, 2048, 3000, 4096,
+              5000, 8191, 8192]
+
+
+def _batch(oracle, trid_ragged, sizes, seed=11, kind="dominant"):
+    import torch
+    offs = trid_ragged.offsets_of(sizes)
+    a, b, c, d = oracle.generate(kind, seed, 0, 0, np.float64, offsets=offs)
+    dev = [torch.from_numpy(v).cuda() for v in (a, b, c, d)]
+    return offs, (a, b, c, d), dev
+
+
+@pytest.fixture(scope="module")
+def rg():
+    import tridiagonalsolvers_amd.ragged as R
+    return R
+
+
+def _check_ragged(oracle, offs, host, x, tol):
+    a, b, c, d = host
+    xref = oracle.thomas(a, b, c, d, offsets=offs)
+    rel, rms, fails = oracle.compare(x, xref, offsets=offs)
+    res = oracle.residual(a, b, c, d, x, offsets=offs)
+    assert fails == 0
+    assert float(np.max(rel)) <= tol, float(np.max(rel))
+    assert float(np.max(res)) <= 1e-12, float(np.max(res))
+    return float(np.max(rel)), float(np.max(res))
+
+
+@pytest.mark.parametrize("kind", ["dominant", "weak"])
+def test_ragged_matches_oracle(oracle, trid, rg, kind):
+    import torch
+    sizes = np.concatenate([EDGE_SIZES, rg.log_uniform_sizes(1500, 16, 8192, seed=5)])
+    offs, host, dev = _batch(oracle, rg, sizes, kind=kind)
+    x = torch.full_like(dev[0], float("nan"))
+    plan = rg.RaggedPlan(offs)
+    info = plan.info()
+    print(info, flush=True)
+    assert info["num_systems"] == len(sizes)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        plan.solve(*dev, x, iters=2, stream=s)
+    s.synchronize()
+    worst = _check_ragged(oracle, offs, host, x.cpu().numpy(),
+                          1e-11 if kind == "dominant" else 1e-10)
+    print(f"{kind}: max rel err {worst[0]:.2e}, max rel residual {worst[1]:.2e}")
+    # the device residual kernel agrees with the oracle's
+    d_offs = torch.from_numpy(offs).cuda()
+    rr = torch.empty(len(sizes), dtype=torch.float64, device="cuda")
+    rg.residual(*dev, x, d_offs, len(sizes), rr)
+    torch.cuda.synchronize()
+    ref = oracle.residual(*host, x.cpu().numpy(), offsets=offs)
+    assert np.allclose(rr.cpu().numpy(), ref, rtol=1e-3, atol=1e-18)
+    plan.close()
+
+
+def test_ragged_alone_equals_in_the_mix(oracle, trid, rg):
+    import torch
+    sizes = np.concatenate([EDGE_SIZES, rg.log_uniform_sizes(300, 2, 8192, seed=9)])
+    offs, host, dev = _batch(oracle, rg, sizes, seed=4)
+    x = torch.full_like(dev[0], float("nan"))
+    plan = rg.RaggedPlan(offs)
+    plan.solve(*dev, x, iters=2, graph=False)
+    torch.cuda.synchronize()
+    xm = x.cpu().numpy()
+    plan.close()
+    for s in list(range(len(EDGE_SIZES))) + [40, 77, 150]:
+        o, e = int(offs[s]), int(offs[s + 1])
+        one = [v[o:e].contiguous() for v in dev]
+        xs = torch.full_like(one[0], float("nan"))
+        p1 = rg.RaggedPlan(np.array([0, e - o], dtype=np.int64))
+        p1.solve(*one, xs, iters=2, graph=False)
+        torch.cuda.synchronize()
+        p1.close()
+        assert np.array_equal(xs.cpu().numpy(), xm[o:e]), (s, e - o)
+
+
+def test_ragged_graph_replays_match_eager(oracle, trid, rg):
+    import torch
+    sizes = rg.log_uniform_sizes(2000, 16, 8192, seed=21)
+    offs, host, dev = _batch(oracle, rg, sizes, seed=8)
+    plan = rg.RaggedPlan(offs)
+    x_eager = torch.full_like(dev[0], float("nan"))
+    plan.solve(*dev, x_eager, iters=2, graph=False)
+    torch.cuda.synchronize()
+    ref = x_eager.cpu().numpy()
+    x = torch.full_like(dev[0], float("nan"))
+    s = torch.cuda.Stream()
+    for rep in range(4):
+        x.fill_(float("nan"))
+        junk = torch.randn(1 << 20, device="cuda").sum()  # eager work between replays
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            plan.solve(*dev, x, iters=2, stream=s)
+        s.synchronize()
+        assert np.array_equal(x.cpu().numpy(), ref), rep
+        del junk
+    plan.close()
+    _check_ragged(oracle, offs, host, ref, 1e-11)
+
+
+def test_every_tuner_candidate_is_accurate(oracle, trid, rg):
+    import torch
+    from tridiagonalsolvers_amd.autotune import candidates
+    for size in (4, 64, 512, 1024, 8192):
+        n = size - (size // 5)   # not a power of two: padded
+        n = max(n, 2)
+        sizes = [n] * 12
+        offs, host, dev = _batch(oracle, rg, sizes, seed=size)
+        for cand in candidates(size):
+            x = torch.full_like(dev[0], float("nan"))
+            plan = rg.RaggedPlan(offs, table=[cand])
+            plan.solve(*dev, x, iters=2, graph=False)
+            torch.cuda.synchronize()
+            plan.close()
+            _check_ragged(oracle, offs, host, x.cpu().numpy(), 1e-11)

@@ -27,7 +27,9 @@ EXPORTS = (
     "trid_pcrgauss_f32", "trid_pcrgauss_f64",
     "trid_create", "trid_destroy", "trid_solve_f32", "trid_solve_f64",
     "trid_generate_f32", "trid_generate_f64", "trid_plan_query",
-    "trid_solve_refined_f64",
+    "trid_solve_refined_f64", "trid_residual_f64",
+    "trid_ragged_plan_create", "trid_ragged_plan_destroy", "trid_ragged_plan_info",
+    "trid_ragged_solve_f64",
 )
 
 
@@ -43,6 +45,17 @@ class Opts(ctypes.Structure):
                 ("lds_pad", ctypes.c_int32), ("lanes", ctypes.c_int32),
                 ("per_lane", ctypes.c_int32), ("io_mode", ctypes.c_int32),
                 ("flags", ctypes.c_int32)]
+
+
+class BucketCfg(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_int32), ("per_lane", ctypes.c_int32),
+                ("lanes", ctypes.c_int32), ("rest", ctypes.c_int32)]
+
+
+class RaggedInfo(ctypes.Structure):
+    _fields_ = [("num_systems", ctypes.c_int64), ("total_unknowns", ctypes.c_int64),
+                ("num_buckets", ctypes.c_int32), ("workgroup_items", ctypes.c_int32),
+                ("wave_items", ctypes.c_int32), ("grid", ctypes.c_int32)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -105,6 +118,16 @@ def lib() -> ctypes.CDLL:
     L.trid_solve_refined_f64.argtypes = [P, i32, P, P, P, P, P, i64, i64, i32,
                                          ctypes.POINTER(Opts), P]
     L.trid_solve_refined_f64.restype = i32
+    L.trid_residual_f64.argtypes = [P, P, P, P, P, P, i64, P, P]
+    L.trid_residual_f64.restype = i32
+    L.trid_ragged_plan_create.argtypes = [ctypes.POINTER(P), P, i64, P, i32]
+    L.trid_ragged_plan_create.restype = i32
+    L.trid_ragged_plan_destroy.argtypes = [P]
+    L.trid_ragged_plan_destroy.restype = i32
+    L.trid_ragged_plan_info.argtypes = [P, ctypes.POINTER(RaggedInfo), P, P, i32]
+    L.trid_ragged_plan_info.restype = i32
+    L.trid_ragged_solve_f64.argtypes = [P, P, P, P, P, P, i32, i32, P]
+    L.trid_ragged_solve_f64.restype = i32
     L.trid_plan_query.argtypes = [i32, i64, i64, i32, ctypes.POINTER(Opts),
                                   ctypes.POINTER(PlanInfo)]
     L.trid_plan_query.restype = i32

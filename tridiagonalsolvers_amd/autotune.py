@@ -1,0 +1,97 @@
+"""Per-bucket autotuner of the ragged solve.
+
+For every padded size 2..8192 it times the candidate configurations of the
+persistent ragged kernel -- unknowns per lane (threads per system), the
+CR -> PCR switch size (`rest`, with rest = 2 being plain CR) -- on a batch of
+that size alone, with the same FP32 solve + FP64 refinement the ragged solve
+runs, and keeps the fastest. The result is a JSON table
+(tune/ragged_f64_refine.json) that RaggedPlan passes to
+trid_ragged_plan_create. Needs a gfx950 device.
+
+    python -m tridiagonalsolvers_amd.autotune [--out PATH] [--unknowns 4194304]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import time
+
+
+def candidates(size: int):
+    out = []
+    if size <= 512:
+        m = 1
+        while m <= 8:
+            lanes = size // m
+            if 2 <= lanes <= 64:
+                r = 2
+                while r <= min(lanes, max(2, size // 2)):
+                    out.append(dict(size=size, per_lane=m, lanes=lanes, rest=r))
+                    r *= 2
+            m *= 2
+    else:
+        lanes = size // 8
+        w = lanes // 64
+        for r in sorted({2, max(2, w // 2), w, 4 * w, 16 * w}):
+            if 2 <= r <= lanes:
+                out.append(dict(size=size, per_lane=8, lanes=lanes, rest=r))
+    return out
+
+
+def tune(sizes=None, unknowns: int = 1 << 22, iters: int = 2, reps: int = 5, log=print):
+    import torch
+
+    from .ragged import RaggedPlan, offsets_of
+    from .solvers import generate
+
+    sizes = sizes or [1 << k for k in range(1, 14)]
+    table = []
+    for size in sizes:
+        N = max(64, unknowns // size)
+        offs = offsets_of([size] * N)
+        arrs = [torch.empty(N * size, dtype=torch.float64, device="cuda") for _ in range(5)]
+        generate(0, 7, *arrs[:4], N * size, 1)
+        best = None
+        for cand in candidates(size):
+            plan = RaggedPlan(offs, table=[cand])
+            plan.solve(*arrs, iters=iters, graph=False)
+            torch.cuda.synchronize()
+            times = []
+            for _ in range(reps):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                plan.solve(*arrs, iters=iters, graph=False)
+                e1.record()
+                torch.cuda.synchronize()
+                times.append(e0.elapsed_time(e1))
+            plan.close()
+            ms = sorted(times)[len(times) // 2]
+            if best is None or ms < best["ms"]:
+                best = dict(cand, ms=ms, unknowns_per_s=N * size / (ms * 1e-3))
+        log(f"size {size}: {best}")
+        table.append(best)
+    return table
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "tune",
+                                                  "ragged_f64_refine.json"))
+    ap.add_argument("--unknowns", type=int, default=1 << 22)
+    ap.add_argument("--iters", type=int, default=2)
+    args = ap.parse_args()
+    import torch
+    t0 = time.time()
+    table = tune(unknowns=args.unknowns, iters=args.iters)
+    os.makedirs(os.path.dirname(args.out), exist_ok=True)
+    with open(args.out, "w") as f:
+        json.dump({"device": torch.cuda.get_device_name(0), "iters": args.iters,
+                   "unknowns_per_bucket": args.unknowns, "seconds": round(time.time() - t0, 1),
+                   "buckets": table}, f, indent=1)
+    print(args.out)
+
+
+if __name__ == "__main__":
+    main()

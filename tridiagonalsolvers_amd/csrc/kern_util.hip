@@ -87,6 +87,48 @@ hipError_t launch_generate(int kind, uint64_t seed, T* a, T* b, T* c, T* d, int6
     return hipGetLastError();
 }
 
+// Relative residual of every system: max_i |d - T x|_i / max_i |d_i|, FP64,
+// one wave per system (a[first] and c[last] ignored).
+__global__ __launch_bounds__(256) void residual_kernel(const double* __restrict__ A,
+                                                      const double* __restrict__ B,
+                                                      const double* __restrict__ C,
+                                                      const double* __restrict__ D,
+                                                      const double* __restrict__ X,
+                                                      const int64_t* __restrict__ offsets,
+                                                      int64_t N, double* __restrict__ out)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    if (s >= N) return;  // whole wave leaves together
+    const int64_t o = offsets[s], n = offsets[s + 1] - o;
+    double rmax = 0.0, dmax = 0.0;
+    for (int64_t j = lane; j < n; j += kWave) {
+        double r = D[o + j] - B[o + j] * X[o + j];
+        if (j > 0) r -= A[o + j] * X[o + j - 1];
+        if (j < n - 1) r -= C[o + j] * X[o + j + 1];
+        rmax = (r != r) ? r : fmax(rmax, fabs(r));
+        dmax = fmax(dmax, fabs(D[o + j]));
+    }
+    for (int k = kWave / 2; k >= 1; k >>= 1) {
+        const double ro = __shfl_xor(rmax, k);
+        rmax = (ro != ro) ? ro : fmax(rmax, ro);
+        dmax = fmax(dmax, __shfl_xor(dmax, k));
+    }
+    if (lane == 0) out[s] = dmax > 0.0 ? rmax / dmax : rmax;
+}
+
+hipError_t launch_residual(const double* a, const double* b, const double* c, const double* d,
+                           const double* x, const int64_t* offsets, int64_t N, double* out,
+                           hipStream_t stream)
+{
+    if (N <= 0) return hipSuccess;
+    const int64_t blocks = (N * kWave + 255) / 256;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL(residual_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, a, b, c, d,
+                       x, offsets, N, out);
+    return hipGetLastError();
+}
+
 template hipError_t launch_generate<float>(int, uint64_t, float*, float*, float*, float*,
                                            int64_t, int64_t, hipStream_t);
 template hipError_t launch_generate<double>(int, uint64_t, double*, double*, double*, double*,

@@ -315,6 +315,18 @@ int trid_solve_refined_f64(trid_handle handle, int method, const double* a, cons
     return set_status(from_hip(trid::launch_refine(g)));
 }
 
+int trid_residual_f64(const double* a, const double* b, const double* c, const double* d,
+                      const double* x, const int64_t* offsets, int64_t numSystems,
+                      double* relres, void* stream)
+{
+    if (!a || !b || !c || !d || !x || !offsets || !relres || numSystems < 0)
+        return set_status(TRID_EINVAL);
+    int st = check_device();
+    if (st != TRID_OK) return set_status(st);
+    return set_status(from_hip(trid::launch_residual(a, b, c, d, x, offsets, numSystems, relres,
+                                                     static_cast<hipStream_t>(stream))));
+}
+
 int trid_generate_f32(int kind, uint64_t seed, float* a, float* b, float* c, float* d,
                       int64_t systemSize, int64_t numSystems, void* stream)
 {

@@ -91,9 +91,46 @@ struct RefineArgs {
 
 hipError_t launch_refine(const RefineArgs& g);
 
+// Ragged persistent solve (kern_ragged.hip).
+struct RaggedItem {
+    int32_t bucket;  // index into the bucket table
+    int32_t first;   // first position in `order`
+    int32_t count;   // systems in the item
+    int32_t pad;
+};
+struct RaggedBucket {
+    int32_t n_pad;     // padded size (power of two)
+    int32_t per_lane;  // unknowns per lane
+    int32_t lanes;     // lanes per system (64*W for workgroup items)
+    int32_t rest;      // CR -> PCR switch size
+};
+struct RaggedArgs {
+    const double* a;
+    const double* b;
+    const double* c;
+    const double* d;
+    double* x;
+    const int64_t* offsets;   // device, numSystems + 1
+    const int32_t* order;     // device, systems grouped by bucket
+    const RaggedItem* items;  // device, workgroup items then wave items
+    const RaggedBucket* buckets;
+    int* counters;            // device, 2 queue counters
+    int n_wg_items;
+    int n_wave_items;
+    int iters;
+};
+
+hipError_t launch_ragged(const RaggedArgs& g, int grid, hipStream_t stream);
+int ragged_max_grid(int device);
+
 // Seeded synthetic batch (kern_util.hip).
 template <typename T>
 hipError_t launch_generate(int kind, uint64_t seed, T* a, T* b, T* c, T* d, int64_t n,
                            int64_t N, hipStream_t stream);
+
+// Relative residual per system (kern_util.hip).
+hipError_t launch_residual(const double* a, const double* b, const double* c, const double* d,
+                           const double* x, const int64_t* offsets, int64_t N, double* out,
+                           hipStream_t stream);
 
 }  // namespace trid

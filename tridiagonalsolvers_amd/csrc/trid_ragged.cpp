@@ -1,0 +1,283 @@
+// trid_ragged.cpp -- ragged-batch plans: size buckets, the work queue of the
+// persistent kernel (kern_ragged.hip) and its hipGraph.
+//
+// No counterpart in the reference (one systemSize per call). The plan is the
+// format boundary between callers (the request server, users) and the
+// kernel: host offsets in, a device work queue out. It depends only on the
+// sizes and the bucket table, never on the data.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "trid.h"
+#include "trid_launch.hpp"
+
+struct trid_ragged_plan_s {
+    int64_t N = 0;
+    int64_t total = 0;
+    int device = 0;
+    int grid = 0;
+    int n_wg_items = 0;
+    int n_wave_items = 0;
+    int n_buckets = 0;
+    std::vector<trid::RaggedBucket> buckets;  // host copy (info)
+    std::vector<int64_t> bucket_systems;
+    char* dev = nullptr;  // one device block: offsets, order, items, buckets, counters
+    int64_t* d_offsets = nullptr;
+    int32_t* d_order = nullptr;
+    trid::RaggedItem* d_items = nullptr;
+    trid::RaggedBucket* d_buckets = nullptr;
+    int* d_counters = nullptr;
+    // captured graph and what it was captured for
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void* key[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    int key_iters = -1;
+};
+
+namespace {
+
+bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
+
+int64_t next_pow2(int64_t v)
+{
+    int64_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+int log2i(int64_t v)
+{
+    int k = 0;
+    while ((int64_t(1) << k) < v) ++k;
+    return k;
+}
+
+// Default configuration of a bucket of padded size n_pad.
+trid::RaggedBucket default_bucket(int n_pad)
+{
+    trid::RaggedBucket b{};
+    b.n_pad = n_pad;
+    if (n_pad <= 512) {
+        b.per_lane = n_pad / 2 < 8 ? n_pad / 2 : 8;
+        b.lanes = n_pad / b.per_lane;
+        b.rest = b.lanes < n_pad / 2 ? b.lanes : n_pad / 2;
+        if (b.rest < 2) b.rest = 2;
+    } else {
+        b.per_lane = 8;
+        b.lanes = n_pad / 8;
+        b.rest = b.lanes / 64;  // one PCR equation per wave
+        if (b.rest < 2) b.rest = 2;
+    }
+    return b;
+}
+
+int check_bucket(const trid::RaggedBucket& b)
+{
+    if (!is_pow2(b.n_pad) || b.n_pad < 2 || b.n_pad > 8192) return TRID_EINVAL;
+    if (!is_pow2(b.per_lane) || !is_pow2(b.lanes) || !is_pow2(b.rest)) return TRID_EINVAL;
+    if ((int64_t)b.per_lane * b.lanes != b.n_pad) return TRID_EINVAL;
+    if (b.rest < 2 || b.rest > b.n_pad / 2 + (b.n_pad == 2 ? 1 : 0)) return TRID_EINVAL;
+    if (b.n_pad <= 512) {
+        // wave items: L <= 64 lanes, M <= 8, PCR on at most one equation per lane
+        if (b.lanes < 2 || b.lanes > 64 || b.per_lane > 8 || b.rest > b.lanes)
+            return TRID_EINVAL;
+    } else {
+        // workgroup items: 8 unknowns per lane, W = lanes/64 waves
+        if (b.per_lane != 8 || b.lanes < 128 || b.lanes > 1024 || b.rest > b.lanes)
+            return TRID_EINVAL;
+    }
+    return TRID_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int trid_ragged_plan_create(trid_ragged_plan* out, const int64_t* offsets, int64_t numSystems,
+                            const trid_bucket_cfg* table, int tableSize)
+{
+    if (!out || !offsets || numSystems < 1 || numSystems > 0x7fffffff || tableSize < 0 ||
+        (tableSize > 0 && !table))
+        return TRID_EINVAL;
+    *out = nullptr;
+    // per-system buckets
+    std::vector<int> bucket_of(numSystems);
+    for (int64_t s = 0; s < numSystems; ++s) {
+        const int64_t n = offsets[s + 1] - offsets[s];
+        if (n < 2) return TRID_EINVAL;
+        if (n > 8192) return TRID_ENOTSUP;
+        bucket_of[s] = log2i(n);  // 1 .. 13
+    }
+    if (offsets[0] < 0) return TRID_EINVAL;
+    // bucket table: defaults, overridden by the caller's (autotuned) entries
+    trid::RaggedBucket cfg[14];
+    for (int k = 1; k <= 13; ++k) cfg[k] = default_bucket(1 << k);
+    for (int t = 0; t < tableSize; ++t) {
+        trid::RaggedBucket b{table[t].size, table[t].per_lane, table[t].lanes, table[t].rest};
+        if (check_bucket(b) != TRID_OK) return TRID_EINVAL;
+        cfg[log2i(b.n_pad)] = b;
+    }
+    // systems grouped by bucket, largest bucket first, stable by index
+    std::vector<int32_t> order(numSystems);
+    for (int64_t s = 0; s < numSystems; ++s) order[s] = (int32_t)s;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t x, int32_t y) { return bucket_of[x] > bucket_of[y]; });
+    trid_ragged_plan_s* p = new (std::nothrow) trid_ragged_plan_s();
+    if (!p) return TRID_ENOMEM;
+    p->N = numSystems;
+    p->total = offsets[numSystems];
+    std::vector<trid::RaggedItem> wg_items, wave_items;
+    std::vector<trid::RaggedBucket> used;
+    int64_t pos = 0;
+    while (pos < numSystems) {
+        const int k = bucket_of[order[pos]];
+        int64_t end = pos;
+        while (end < numSystems && bucket_of[order[end]] == k) ++end;
+        const trid::RaggedBucket& b = cfg[k];
+        const int bi = (int)used.size();
+        used.push_back(b);
+        p->bucket_systems.push_back(end - pos);
+        const int per_item = b.n_pad <= 512 ? 64 / b.lanes : 16 / (b.lanes / 64);
+        for (int64_t f = pos; f < end; f += per_item) {
+            trid::RaggedItem it{};
+            it.bucket = bi;
+            it.first = (int32_t)f;
+            it.count = (int32_t)std::min<int64_t>(per_item, end - f);
+            (b.n_pad <= 512 ? wave_items : wg_items).push_back(it);
+        }
+        pos = end;
+    }
+    p->buckets = used;
+    p->n_buckets = (int)used.size();
+    p->n_wg_items = (int)wg_items.size();
+    p->n_wave_items = (int)wave_items.size();
+    std::vector<trid::RaggedItem> items(wg_items);
+    items.insert(items.end(), wave_items.begin(), wave_items.end());
+
+    // device block
+    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t b_off = up(sizeof(int64_t) * (numSystems + 1));
+    const size_t b_ord = up(sizeof(int32_t) * numSystems);
+    const size_t b_itm = up(sizeof(trid::RaggedItem) * (items.size() + 1));
+    const size_t b_bkt = up(sizeof(trid::RaggedBucket) * (used.size() + 1));
+    const size_t b_ctr = 256;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->dev), b_off + b_ord + b_itm + b_bkt + b_ctr);
+    if (e != hipSuccess) {
+        delete p;
+        return TRID_ENOMEM;
+    }
+    p->d_offsets = reinterpret_cast<int64_t*>(p->dev);
+    p->d_order = reinterpret_cast<int32_t*>(p->dev + b_off);
+    p->d_items = reinterpret_cast<trid::RaggedItem*>(p->dev + b_off + b_ord);
+    p->d_buckets = reinterpret_cast<trid::RaggedBucket*>(p->dev + b_off + b_ord + b_itm);
+    p->d_counters = reinterpret_cast<int*>(p->dev + b_off + b_ord + b_itm + b_bkt);
+    if ((e = hipMemcpy(p->d_offsets, offsets, sizeof(int64_t) * (numSystems + 1),
+                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(p->d_order, order.data(), sizeof(int32_t) * numSystems,
+                       hipMemcpyHostToDevice)) != hipSuccess ||
+        (!items.empty() &&
+         (e = hipMemcpy(p->d_items, items.data(), sizeof(trid::RaggedItem) * items.size(),
+                        hipMemcpyHostToDevice)) != hipSuccess) ||
+        (e = hipMemcpy(p->d_buckets, used.data(), sizeof(trid::RaggedBucket) * used.size(),
+                       hipMemcpyHostToDevice)) != hipSuccess) {
+        (void)hipFree(p->dev);
+        delete p;
+        return TRID_EHIP;
+    }
+    (void)hipGetDevice(&p->device);
+    p->grid = trid::ragged_max_grid(p->device);
+    const int64_t work = (int64_t)p->n_wg_items + (p->n_wave_items + 15) / 16;
+    if (work < p->grid) p->grid = (int)std::max<int64_t>(1, work);
+    *out = p;
+    return TRID_OK;
+}
+
+int trid_ragged_plan_destroy(trid_ragged_plan plan)
+{
+    if (!plan) return TRID_EINVAL;
+    if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+    if (plan->graph) (void)hipGraphDestroy(plan->graph);
+    if (plan->dev) (void)hipFree(plan->dev);
+    delete plan;
+    return TRID_OK;
+}
+
+int trid_ragged_plan_info(trid_ragged_plan plan, trid_ragged_info* info, trid_bucket_cfg* buckets,
+                          int64_t* bucketSystems, int capacity)
+{
+    if (!plan || !info) return TRID_EINVAL;
+    info->num_systems = plan->N;
+    info->total_unknowns = plan->total;
+    info->num_buckets = plan->n_buckets;
+    info->workgroup_items = plan->n_wg_items;
+    info->wave_items = plan->n_wave_items;
+    info->grid = plan->grid;
+    for (int i = 0; i < plan->n_buckets && i < capacity; ++i) {
+        if (buckets) {
+            buckets[i].size = plan->buckets[i].n_pad;
+            buckets[i].per_lane = plan->buckets[i].per_lane;
+            buckets[i].lanes = plan->buckets[i].lanes;
+            buckets[i].rest = plan->buckets[i].rest;
+        }
+        if (bucketSystems) bucketSystems[i] = plan->bucket_systems[i];
+    }
+    return TRID_OK;
+}
+
+int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* b, const double* c,
+                          const double* d, double* x, int iters, int flags, void* stream)
+{
+    if (!plan || !a || !b || !c || !d || !x || iters < 0 || iters > 8) return TRID_EINVAL;
+    if (flags & ~TRID_RAGGED_NO_GRAPH) return TRID_EINVAL;
+    trid::RaggedArgs g{};
+    g.a = a;
+    g.b = b;
+    g.c = c;
+    g.d = d;
+    g.x = x;
+    g.offsets = plan->d_offsets;
+    g.order = plan->d_order;
+    g.items = plan->d_items;
+    g.buckets = plan->d_buckets;
+    g.counters = plan->d_counters;
+    g.n_wg_items = plan->n_wg_items;
+    g.n_wave_items = plan->n_wave_items;
+    g.iters = iters;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if ((flags & TRID_RAGGED_NO_GRAPH) || s == nullptr) {
+        hipError_t e = trid::launch_ragged(g, plan->grid, s);
+        return e == hipSuccess ? TRID_OK : TRID_EHIP;
+    }
+    const void* key[5] = {a, b, c, d, x};
+    const bool same = plan->exec && plan->key_iters == iters &&
+                      std::memcmp(key, plan->key, sizeof(key)) == 0;
+    if (!same) {
+        if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+        if (plan->graph) (void)hipGraphDestroy(plan->graph);
+        plan->exec = nullptr;
+        plan->graph = nullptr;
+        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return TRID_EHIP;
+        hipError_t le = trid::launch_ragged(g, plan->grid, s);
+        e = hipStreamEndCapture(s, &plan->graph);
+        if (le != hipSuccess || e != hipSuccess) {
+            if (plan->graph) (void)hipGraphDestroy(plan->graph);
+            plan->graph = nullptr;
+            return TRID_EHIP;
+        }
+        e = hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            plan->exec = nullptr;
+            return TRID_EHIP;
+        }
+        std::memcpy(plan->key, key, sizeof(key));
+        plan->key_iters = iters;
+    }
+    return hipGraphLaunch(plan->exec, s) == hipSuccess ? TRID_OK : TRID_EHIP;
+}
+
+}  // extern "C"
