@@ -55,13 +55,21 @@ def test_ragged_matches_oracle(oracle, trid, rg, kind):
     worst = _check_ragged(oracle, offs, host, x.cpu().numpy(),
                           1e-11 if kind == "dominant" else 1e-10)
     print(f"{kind}: max rel err {worst[0]:.2e}, max rel residual {worst[1]:.2e}")
-    # the device residual kernel agrees with the oracle's
+    # the device residual kernel: <= 1e-12 here, and equal to the oracle's
+    # where the residual is well above rounding level (perturbed unknowns)
     d_offs = torch.from_numpy(offs).cuda()
     rr = torch.empty(len(sizes), dtype=torch.float64, device="cuda")
     rg.residual(*dev, x, d_offs, len(sizes), rr)
     torch.cuda.synchronize()
-    ref = oracle.residual(*host, x.cpu().numpy(), offsets=offs)
-    assert np.allclose(rr.cpu().numpy(), ref, rtol=1e-3, atol=1e-18)
+    assert float(rr.max()) <= 1e-12
+    xp = x.cpu().numpy()
+    for s in range(0, len(sizes), 7):
+        xp[offs[s] + (offs[s + 1] - offs[s]) // 2] += 1e-5
+    rg.residual(*dev, torch.from_numpy(xp).cuda(), d_offs, len(sizes), rr)
+    torch.cuda.synchronize()
+    ref = oracle.residual(*host, xp, offsets=offs)
+    sel = np.arange(0, len(sizes), 7)
+    assert np.allclose(rr.cpu().numpy()[sel], ref[sel], rtol=1e-9, atol=0)
     plan.close()
 
 
