@@ -204,15 +204,15 @@ int trid_generate_f64(int kind, uint64_t seed, double* a, double* b, double* c, 
 /*
  * No counterpart in the reference (one systemSize per call). System s of a
  * ragged batch occupies [offsets[s], offsets[s+1]) of flat FP64 device
- * arrays a, b, c, d, x; sizes 2..8192. A plan buckets the systems by padded
+ * arrays a, b, c, d, x; sizes 2..16384. A plan buckets the systems by padded
  * size (power of two) and builds the work queue of one persistent kernel
  * (workgroup items for 1024..8192, wave items below); each bucket solves in
  * FP32 with its (per_lane, lanes, rest) configuration and refines in FP64
  * `iters` times. The plan depends only on the sizes and the table.
  */
 typedef struct trid_bucket_cfg {
-    int32_t size;      /* padded system size (power of two, 2..8192)            */
-    int32_t per_lane;  /* unknowns per lane: <= 8                               */
+    int32_t size;      /* padded system size (power of two, 2..16384)           */
+    int32_t per_lane;  /* unknowns per lane: <= 8 (16 for size 16384)          */
     int32_t lanes;     /* lanes per system: size/per_lane (<= 64 below 1024)    */
     int32_t rest;      /* CR -> PCR switch size, power of two <= lanes          */
 } trid_bucket_cfg;

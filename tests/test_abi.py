@@ -117,7 +117,7 @@ def test_ragged_plan_rejects_bad_sizes_without_a_device(trid):
         R.RaggedPlan(np.array([0, 5, 3], dtype=np.int64), table=[])       # negative size
     assert e.value.status == 1
     with pytest.raises(trid.TridError) as e:
-        R.RaggedPlan(np.array([0, 10000], dtype=np.int64), table=[])      # above 8192
+        R.RaggedPlan(np.array([0, 20000], dtype=np.int64), table=[])      # above 16384
     assert e.value.status == 4
     bad = [dict(size=64, per_lane=16, lanes=4, rest=4)]                    # per_lane > 8
     with pytest.raises(trid.TridError):
@@ -129,13 +129,14 @@ def test_ragged_plan_rejects_bad_sizes_without_a_device(trid):
 
 def test_autotune_candidates_are_valid_configs():
     from tridiagonalsolvers_amd.autotune import candidates
-    for k in range(1, 14):
+    for k in range(1, 15):
         size = 1 << k
         cs = candidates(size)
         assert cs, size
         for c in cs:
             assert c["per_lane"] * c["lanes"] == size
-            assert c["per_lane"] <= 8 and 2 <= c["rest"] <= c["lanes"]
+            assert c["per_lane"] <= (16 if size > 8192 else 8)
+            assert 2 <= c["rest"] <= c["lanes"] <= 1024
             if size <= 512:
                 assert c["lanes"] <= 64
 

@@ -10,7 +10,7 @@ from test_gpu_parity import check
 pytestmark = pytest.mark.gpu
 
 EDGE_SIZES = [2, 3, 5, 16, 17, 100, 511, 512, 513, 1000, 1024, 1025, 2048, 3000, 4096,
-              5000, 8191, 8192]
+              5000, 8191, 8192, 8193, 12000, 16384]
 
 
 def _batch(oracle, trid_ragged, sizes, seed=11, kind="dominant"):
@@ -121,7 +121,7 @@ def test_ragged_graph_replays_match_eager(oracle, trid, rg):
 def test_every_tuner_candidate_is_accurate(oracle, trid, rg):
     import torch
     from tridiagonalsolvers_amd.autotune import candidates
-    for size in (4, 64, 512, 1024, 8192):
+    for size in (4, 64, 512, 1024, 8192, 16384):
         n = size - (size // 5)   # not a power of two: padded
         n = max(n, 2)
         sizes = [n] * 12

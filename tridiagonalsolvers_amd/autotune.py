@@ -1,6 +1,6 @@
 """Per-bucket autotuner of the ragged solve.
 
-For every padded size 2..8192 it times the candidate configurations of the
+For every padded size 2..16384 it times the candidate configurations of the
 persistent ragged kernel -- unknowns per lane (threads per system), the
 CR -> PCR switch size (`rest`, with rest = 2 being plain CR) -- on a batch of
 that size alone, with the same FP32 solve + FP64 refinement the ragged solve
@@ -31,11 +31,12 @@ def candidates(size: int):
                     r *= 2
             m *= 2
     else:
-        lanes = size // 8
+        m = 16 if size > 8192 else 8
+        lanes = size // m
         w = lanes // 64
         for r in sorted({2, max(2, w // 2), w, 4 * w, 16 * w}):
             if 2 <= r <= lanes:
-                out.append(dict(size=size, per_lane=8, lanes=lanes, rest=r))
+                out.append(dict(size=size, per_lane=m, lanes=lanes, rest=r))
     return out
 
 
@@ -45,7 +46,7 @@ def tune(sizes=None, unknowns: int = 1 << 22, iters: int = 2, reps: int = 5, log
     from .ragged import RaggedPlan, offsets_of
     from .solvers import generate
 
-    sizes = sizes or [1 << k for k in range(1, 14)]
+    sizes = sizes or [1 << k for k in range(1, 15)]
     table = []
     for size in sizes:
         N = max(64, unknowns // size)

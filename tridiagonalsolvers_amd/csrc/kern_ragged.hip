@@ -12,10 +12,11 @@
 // per system, CR->PCR switch size) chosen by the autotune table.
 //   * wave items: buckets up to 512 unknowns; one item = up to 64/L systems
 //     of one bucket, solved by one wave, L lanes each (no barrier);
-//   * workgroup items: buckets of 1024..8192 unknowns; 8 unknowns per lane,
-//     W = n/512 waves per system, 16/W systems per item, one per group of W
-//     waves of the 1024-thread workgroup (barriers are workgroup-wide, and
-//     every group of an item runs the same bucket's code, so they match).
+//   * workgroup items: buckets of 1024..16384 unknowns; 8 unknowns per lane
+//     (16 for 16384), W = n/512 waves per system (16 for 16384), 16/W systems
+//     per item, one per group of W waves of the 1024-thread workgroup
+//     (barriers are workgroup-wide, and every group of an item runs the same
+//     bucket's code, so they match).
 // The grid is persistent (one workgroup per CU slot). Workgroups first drain
 // the workgroup items, largest bucket first, then each wave drains wave
 // items on its own; both queues are atomic counters that a small kernel
@@ -31,7 +32,11 @@ namespace {
 
 constexpr int kRaggedThreads = 1024;
 constexpr int kRaggedWaves = kRaggedThreads / kWave;
-constexpr int kMwM = 8;  // unknowns per lane of the workgroup items
+static_assert(kRaggedWaves == 16, "workgroup items assume 16 waves");
+// LDS of the workgroup items: 16/W groups of W waves, each with the
+// exchange area of wg_core and 2 W boundary unknowns. The largest case is
+// 16 waves at 16 unknowns per lane (16384-unknown systems).
+constexpr size_t kRaggedLds = 16 * (10 * 16 + 64 * 16 + 64 * 4 + 16);
 
 // FP32 operator + FP64 residual rows (as kern_refine.hip's build_rows) for a
 // ragged system starting at `base` with n unknowns; element-wise loads.
@@ -95,10 +100,10 @@ __device__ void wave_item(const RaggedArgs& g, const RaggedItem& it, const Ragge
 }
 
 // All 16 waves of the workgroup call this (wg_core has barriers).
+template <int M>
 __device__ void wg_item(const RaggedArgs& g, const RaggedItem& it, const RaggedBucket& bk,
                         int wave, int lane, char* smem)
 {
-    constexpr int M = kMwM;
     const int W = bk.lanes / kWave;
     const int grp = wave / W;
     const int w = wave % W;
@@ -149,9 +154,13 @@ __global__ void ragged_reset_kernel(int* counters)
     if (threadIdx.x < 2) counters[threadIdx.x] = 0;
 }
 
+// WIDE: the plan has 16384-unknown systems (16 unknowns per lane in the
+// workgroup items). That path needs more registers than the others, so it is
+// a separate instantiation and the common one keeps its register budget.
+template <bool WIDE>
 __global__ __launch_bounds__(kRaggedThreads) void ragged_kernel(RaggedArgs g)
 {
-    __shared__ __align__(16) char smem[kRaggedWaves * 1440 + 64];
+    __shared__ __align__(16) char smem[kRaggedLds];
     __shared__ int s_item;
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x & (kWave - 1);
@@ -164,7 +173,14 @@ __global__ __launch_bounds__(kRaggedThreads) void ragged_kernel(RaggedArgs g)
         __syncthreads();
         if (item >= g.n_wg_items) break;
         const RaggedItem it = g.items[item];
-        wg_item(g, it, g.buckets[it.bucket], wave, lane, smem);
+        const RaggedBucket bk = g.buckets[it.bucket];
+        if constexpr (WIDE) {
+            if (bk.per_lane == 16) {
+                wg_item<16>(g, it, bk, wave, lane, smem);
+                continue;
+            }
+        }
+        wg_item<8>(g, it, bk, wave, lane, smem);
     }
     // phase 2: wave items, each wave on its own
     for (;;) {
@@ -189,7 +205,12 @@ hipError_t launch_ragged(const RaggedArgs& g, int grid, hipStream_t stream)
     hipLaunchKernelGGL(ragged_reset_kernel, dim3(1), dim3(64), 0, stream, g.counters);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(ragged_kernel, dim3((unsigned)grid), dim3(kRaggedThreads), 0, stream, g);
+    if (g.wide)
+        hipLaunchKernelGGL(ragged_kernel<true>, dim3((unsigned)grid), dim3(kRaggedThreads), 0,
+                           stream, g);
+    else
+        hipLaunchKernelGGL(ragged_kernel<false>, dim3((unsigned)grid), dim3(kRaggedThreads), 0,
+                           stream, g);
     return hipGetLastError();
 }
 
@@ -197,7 +218,8 @@ hipError_t launch_ragged(const RaggedArgs& g, int grid, hipStream_t stream)
 int ragged_max_grid(int device)
 {
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ragged_kernel, kRaggedThreads, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ragged_kernel<false>,
+                                                     kRaggedThreads, 0) !=
         hipSuccess)
         per_cu = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)

@@ -4,6 +4,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
+#include "trid.h"
+
 namespace trid {
 
 // Register-resident CR / CR-PCR / PCR (kern_reg.hip).
@@ -118,10 +122,27 @@ struct RaggedArgs {
     int n_wg_items;
     int n_wave_items;
     int iters;
+    int wide;                 // 1 if any 16384-unknown bucket is present
 };
 
 hipError_t launch_ragged(const RaggedArgs& g, int grid, hipStream_t stream);
 int ragged_max_grid(int device);
+
+// Host side of a ragged plan (trid_ragged.cpp): systems grouped by padded
+// size, the bucket table and the work items of the persistent kernel.
+constexpr int64_t kRaggedMaxSize = 16384;
+struct RaggedHostPlan {
+    std::vector<int32_t> order;
+    std::vector<RaggedItem> items;  // workgroup items, then wave items
+    std::vector<RaggedBucket> buckets;
+    std::vector<int64_t> bucket_systems;
+    int n_wg_items = 0;
+    int n_wave_items = 0;
+};
+RaggedBucket ragged_default_bucket(int n_pad);
+int ragged_check_bucket(const RaggedBucket& b);
+int ragged_build_host(const int64_t* offsets, int64_t N, const trid_bucket_cfg* table,
+                      int tableSize, RaggedHostPlan* out);
 
 // Seeded synthetic batch (kern_util.hip).
 template <typename T>

@@ -23,6 +23,7 @@ struct trid_ragged_plan_s {
     int n_wg_items = 0;
     int n_wave_items = 0;
     int n_buckets = 0;
+    int wide = 0;
     std::vector<trid::RaggedBucket> buckets;  // host copy (info)
     std::vector<int64_t> bucket_systems;
     char* dev = nullptr;  // one device block: offsets, order, items, buckets, counters
@@ -42,13 +43,6 @@ namespace {
 
 bool is_pow2(int64_t v) { return v > 0 && (v & (v - 1)) == 0; }
 
-int64_t next_pow2(int64_t v)
-{
-    int64_t p = 1;
-    while (p < v) p <<= 1;
-    return p;
-}
-
 int log2i(int64_t v)
 {
     int k = 0;
@@ -56,12 +50,20 @@ int log2i(int64_t v)
     return k;
 }
 
+}  // namespace
+
+namespace trid {
+
 // Default configuration of a bucket of padded size n_pad.
-trid::RaggedBucket default_bucket(int n_pad)
+RaggedBucket ragged_default_bucket(int n_pad)
 {
-    trid::RaggedBucket b{};
+    RaggedBucket b{};
     b.n_pad = n_pad;
-    if (n_pad <= 512) {
+    if (n_pad > 8192) {
+        b.per_lane = 16;
+        b.lanes = n_pad / 16;
+        b.rest = b.lanes / 64;
+    } else if (n_pad <= 512) {
         b.per_lane = n_pad / 2 < 8 ? n_pad / 2 : 8;
         b.lanes = n_pad / b.per_lane;
         b.rest = b.lanes < n_pad / 2 ? b.lanes : n_pad / 2;
@@ -75,9 +77,9 @@ trid::RaggedBucket default_bucket(int n_pad)
     return b;
 }
 
-int check_bucket(const trid::RaggedBucket& b)
+int ragged_check_bucket(const RaggedBucket& b)
 {
-    if (!is_pow2(b.n_pad) || b.n_pad < 2 || b.n_pad > 8192) return TRID_EINVAL;
+    if (!is_pow2(b.n_pad) || b.n_pad < 2 || b.n_pad > kRaggedMaxSize) return TRID_EINVAL;
     if (!is_pow2(b.per_lane) || !is_pow2(b.lanes) || !is_pow2(b.rest)) return TRID_EINVAL;
     if ((int64_t)b.per_lane * b.lanes != b.n_pad) return TRID_EINVAL;
     if (b.rest < 2 || b.rest > b.n_pad / 2 + (b.n_pad == 2 ? 1 : 0)) return TRID_EINVAL;
@@ -86,64 +88,56 @@ int check_bucket(const trid::RaggedBucket& b)
         if (b.lanes < 2 || b.lanes > 64 || b.per_lane > 8 || b.rest > b.lanes)
             return TRID_EINVAL;
     } else {
-        // workgroup items: 8 unknowns per lane, W = lanes/64 waves
-        if (b.per_lane != 8 || b.lanes < 128 || b.lanes > 1024 || b.rest > b.lanes)
+        // workgroup items: 8 unknowns per lane (16 above 8192), W = lanes/64
+        // waves, W <= 16
+        if (b.per_lane != (b.n_pad > 8192 ? 16 : 8) || b.lanes < 128 || b.lanes > 1024 ||
+            b.rest > b.lanes)
             return TRID_EINVAL;
     }
     return TRID_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int trid_ragged_plan_create(trid_ragged_plan* out, const int64_t* offsets, int64_t numSystems,
-                            const trid_bucket_cfg* table, int tableSize)
+int ragged_build_host(const int64_t* offsets, int64_t N, const trid_bucket_cfg* table,
+                      int tableSize, RaggedHostPlan* out)
 {
-    if (!out || !offsets || numSystems < 1 || numSystems > 0x7fffffff || tableSize < 0 ||
-        (tableSize > 0 && !table))
+    if (!offsets || N < 1 || N > 0x7fffffff || tableSize < 0 || (tableSize > 0 && !table))
         return TRID_EINVAL;
-    *out = nullptr;
-    // per-system buckets
-    std::vector<int> bucket_of(numSystems);
-    for (int64_t s = 0; s < numSystems; ++s) {
+    if (offsets[0] < 0) return TRID_EINVAL;
+    std::vector<int> bucket_of(N);
+    for (int64_t s = 0; s < N; ++s) {
         const int64_t n = offsets[s + 1] - offsets[s];
         if (n < 2) return TRID_EINVAL;
-        if (n > 8192) return TRID_ENOTSUP;
-        bucket_of[s] = log2i(n);  // 1 .. 13
+        if (n > kRaggedMaxSize) return TRID_ENOTSUP;
+        bucket_of[s] = log2i(n);  // 1 .. 14
     }
-    if (offsets[0] < 0) return TRID_EINVAL;
     // bucket table: defaults, overridden by the caller's (autotuned) entries
-    trid::RaggedBucket cfg[14];
-    for (int k = 1; k <= 13; ++k) cfg[k] = default_bucket(1 << k);
+    RaggedBucket cfg[15];
+    for (int k = 1; k <= 14; ++k) cfg[k] = ragged_default_bucket(1 << k);
     for (int t = 0; t < tableSize; ++t) {
-        trid::RaggedBucket b{table[t].size, table[t].per_lane, table[t].lanes, table[t].rest};
-        if (check_bucket(b) != TRID_OK) return TRID_EINVAL;
+        RaggedBucket b{table[t].size, table[t].per_lane, table[t].lanes, table[t].rest};
+        if (ragged_check_bucket(b) != TRID_OK) return TRID_EINVAL;
         cfg[log2i(b.n_pad)] = b;
     }
     // systems grouped by bucket, largest bucket first, stable by index
-    std::vector<int32_t> order(numSystems);
-    for (int64_t s = 0; s < numSystems; ++s) order[s] = (int32_t)s;
-    std::stable_sort(order.begin(), order.end(),
+    out->order.resize(N);
+    for (int64_t s = 0; s < N; ++s) out->order[s] = (int32_t)s;
+    std::stable_sort(out->order.begin(), out->order.end(),
                      [&](int32_t x, int32_t y) { return bucket_of[x] > bucket_of[y]; });
-    trid_ragged_plan_s* p = new (std::nothrow) trid_ragged_plan_s();
-    if (!p) return TRID_ENOMEM;
-    p->N = numSystems;
-    p->total = offsets[numSystems];
-    std::vector<trid::RaggedItem> wg_items, wave_items;
-    std::vector<trid::RaggedBucket> used;
+    std::vector<RaggedItem> wg_items, wave_items;
+    out->buckets.clear();
+    out->bucket_systems.clear();
     int64_t pos = 0;
-    while (pos < numSystems) {
-        const int k = bucket_of[order[pos]];
+    while (pos < N) {
+        const int k = bucket_of[out->order[pos]];
         int64_t end = pos;
-        while (end < numSystems && bucket_of[order[end]] == k) ++end;
-        const trid::RaggedBucket& b = cfg[k];
-        const int bi = (int)used.size();
-        used.push_back(b);
-        p->bucket_systems.push_back(end - pos);
+        while (end < N && bucket_of[out->order[end]] == k) ++end;
+        const RaggedBucket& b = cfg[k];
+        const int bi = (int)out->buckets.size();
+        out->buckets.push_back(b);
+        out->bucket_systems.push_back(end - pos);
         const int per_item = b.n_pad <= 512 ? 64 / b.lanes : 16 / (b.lanes / 64);
         for (int64_t f = pos; f < end; f += per_item) {
-            trid::RaggedItem it{};
+            RaggedItem it{};
             it.bucket = bi;
             it.first = (int32_t)f;
             it.count = (int32_t)std::min<int64_t>(per_item, end - f);
@@ -151,12 +145,39 @@ int trid_ragged_plan_create(trid_ragged_plan* out, const int64_t* offsets, int64
         }
         pos = end;
     }
-    p->buckets = used;
-    p->n_buckets = (int)used.size();
-    p->n_wg_items = (int)wg_items.size();
-    p->n_wave_items = (int)wave_items.size();
-    std::vector<trid::RaggedItem> items(wg_items);
-    items.insert(items.end(), wave_items.begin(), wave_items.end());
+    out->n_wg_items = (int)wg_items.size();
+    out->n_wave_items = (int)wave_items.size();
+    out->items = wg_items;
+    out->items.insert(out->items.end(), wave_items.begin(), wave_items.end());
+    return TRID_OK;
+}
+
+}  // namespace trid
+
+extern "C" {
+
+int trid_ragged_plan_create(trid_ragged_plan* out, const int64_t* offsets, int64_t numSystems,
+                            const trid_bucket_cfg* table, int tableSize)
+{
+    if (!out) return TRID_EINVAL;
+    *out = nullptr;
+    trid::RaggedHostPlan hp;
+    int st = trid::ragged_build_host(offsets, numSystems, table, tableSize, &hp);
+    if (st != TRID_OK) return st;
+    trid_ragged_plan_s* p = new (std::nothrow) trid_ragged_plan_s();
+    if (!p) return TRID_ENOMEM;
+    p->N = numSystems;
+    p->total = offsets[numSystems];
+    p->buckets = hp.buckets;
+    p->bucket_systems = hp.bucket_systems;
+    p->n_buckets = (int)hp.buckets.size();
+    p->n_wg_items = hp.n_wg_items;
+    p->n_wave_items = hp.n_wave_items;
+    for (const auto& b : hp.buckets)
+        if (b.per_lane == 16) p->wide = 1;
+    const std::vector<trid::RaggedItem>& items = hp.items;
+    const std::vector<trid::RaggedBucket>& used = hp.buckets;
+    const std::vector<int32_t>& order = hp.order;
 
     // device block
     auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
@@ -247,6 +268,7 @@ int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* 
     g.n_wg_items = plan->n_wg_items;
     g.n_wave_items = plan->n_wave_items;
     g.iters = iters;
+    g.wide = plan->wide;
     hipStream_t s = static_cast<hipStream_t>(stream);
     if ((flags & TRID_RAGGED_NO_GRAPH) || s == nullptr) {
         hipError_t e = trid::launch_ragged(g, plan->grid, s);
