@@ -36,7 +36,8 @@ typedef enum trid_status {
     TRID_ENOMEM = 2,  /* device or host allocation failed */
     TRID_EHIP = 3,    /* a HIP runtime call or kernel launch failed */
     TRID_ENOTSUP = 4, /* valid request that this build cannot run */
-    TRID_ENODEV = 5   /* no usable gfx950 device */
+    TRID_ENODEV = 5,  /* no usable gfx950 device */
+    TRID_EAGAIN = 6   /* not finished / still in use: try again later */
 } trid_status;
 
 typedef enum trid_method {
@@ -246,6 +247,125 @@ int trid_ragged_plan_info(trid_ragged_plan plan, trid_ragged_info* info,
 int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* b,
                           const double* c, const double* d, double* x, int iters, int flags,
                           void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Resident ADI grids (2D heat equation, Peaceman-Rachford)                 */
+/* ---------------------------------------------------------------------- */
+/*
+ * No counterpart in the reference. An n x n FP64 grid u (row-major, zero
+ * Dirichlet boundary) stays on the device; each step with parameter r solves
+ * (I - r dxx) u* = (I + r dyy) u   (n row systems tridiag(-r, 1+2r, -r))
+ * (I - r dyy) u' = (I + r dxx) u*  (n column systems, stride n)
+ * with the constant-coefficient register CR-PCR kernel for the rows and a
+ * pre-factored, coalesced one-lane-per-column Thomas for the columns.
+ */
+typedef struct trid_adi_s* trid_adi;
+
+/* u_device: an existing n*n device grid to step in place, or NULL to
+ * allocate one (trid_adi_grid returns it). 2 <= n <= 16384. */
+int trid_adi_create(trid_adi* adi, int64_t n, double* u_device);
+int trid_adi_destroy(trid_adi adi);
+double* trid_adi_grid(trid_adi adi);
+/* Enqueue `steps` steps with parameter r > 0 on `stream`. */
+int trid_adi_step(trid_adi adi, double r, int steps, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Continuous-batching request server                                       */
+/* ---------------------------------------------------------------------- */
+/*
+ * No counterpart in the reference (a single-threaded test driver). Requests
+ * (one or more FP64 systems, sizes 2..16384, host arrays) are submitted from
+ * any thread; a scheduler thread admits them by device-memory footprint,
+ * batches what has arrived into ragged solves (FP32 + FP64 refinement) on
+ * `slots` streams, overlaps batches on the device, and copies results back.
+ * Registered operators keep coefficient arrays resident; requests on them
+ * send only d. Each request's result is independent of the batch it lands in.
+ */
+typedef struct trid_server_s* trid_server;
+
+/* Fake executor: no device work at all, nothing is computed; batches finish
+ * after a modelled delay. For testing admission and scheduling only. */
+#define TRID_SERVER_FAKE_EXECUTOR 1
+
+typedef struct trid_server_opts {
+    int64_t hbm_budget_bytes;    /* device memory reserved for batches and operators (0 = 4 GiB) */
+    int32_t slots;               /* batches in flight / streams (0 = 3)                  */
+    int32_t max_batch_systems;   /* (0 = 65536)                                           */
+    int64_t max_batch_unknowns;  /* (0 = 2^23)                                            */
+    int32_t batch_window_us;     /* while the device is busy, wait up to this long for a batch to grow (0 = 200) */
+    int32_t iters;               /* FP64 refinement steps (0..8; -1 = 2)                  */
+    int32_t flags;               /* TRID_SERVER_*                                         */
+    int32_t fake_fail_every;     /* fake executor: fail every k-th batch (0 = never)      */
+    int32_t fake_batch_us;       /* fake executor: time a batch takes (0 = size model)    */
+    const trid_bucket_cfg* bucket_table;  /* optional per-bucket configurations          */
+    int32_t bucket_table_size;
+} trid_server_opts;
+
+#define TRID_REQ_SOLVE 0
+#define TRID_REQ_OPERATOR 1
+#define TRID_REQ_ADI 2
+
+typedef struct trid_request {
+    int32_t kind;                /* TRID_REQ_SOLVE, TRID_REQ_OPERATOR or TRID_REQ_ADI     */
+    int64_t num_systems;         /* SOLVE: systems in the request                         */
+    const int64_t* offsets;      /* SOLVE: host, num_systems+1, into a,b,c,d,x            */
+    const double* a;             /* SOLVE: host coefficients                              */
+    const double* b;
+    const double* c;
+    const double* d;             /* SOLVE, OPERATOR: host right-hand sides                */
+    double* x;                   /* SOLVE, OPERATOR: host solution, written before the
+                                    request completes                                     */
+    int32_t op;                  /* OPERATOR: resident operator id                        */
+    int32_t adi;                 /* ADI: resident grid id, advanced by adi_steps
+                                    Peaceman-Rachford steps with parameter adi_r          */
+    int32_t adi_steps;
+    double adi_r;
+} trid_request;
+
+typedef struct trid_request_result {
+    int32_t status;              /* TRID_OK, or why the request failed                    */
+    int32_t state;               /* 2 done, 3 cancelled, 4 failed                         */
+    double latency_us;           /* submit -> results copied back                         */
+    double queue_us;             /* submit -> batch launched                              */
+    int64_t batch;               /* batch id it ran in                                    */
+    int64_t batch_systems;       /* systems in that batch                                 */
+} trid_request_result;
+
+typedef struct trid_server_stats {
+    int64_t submitted, completed, failed, cancelled, batches;
+    int64_t queued;
+    int64_t device_bytes_in_use;  /* batches in flight + resident operators              */
+    int64_t peak_device_bytes;
+    int64_t budget_bytes;
+} trid_server_stats;
+
+int trid_server_create(trid_server* server, const trid_server_opts* opts);
+/* Finishes every queued and running request, then frees everything. */
+int trid_server_destroy(trid_server server);
+/* sizes: numSystems system sizes; a, b, c: host arrays of their sum. */
+int trid_server_register_operator(trid_server server, const int64_t* sizes, int64_t numSystems,
+                                  const double* a, const double* b, const double* c,
+                                  int32_t* op);
+/* TRID_EAGAIN while requests on the operator are queued or running. */
+int trid_server_release_operator(trid_server server, int32_t op);
+/* A resident n x n FP64 ADI grid (initial values u0, host) inside the
+ * server's budget; requests with .adi = id step it. Steps on one grid run in
+ * submission order, never two at once. */
+int trid_server_create_adi(trid_server server, int64_t n, const double* u0, int32_t* adi);
+/* Copy the grid to the host; TRID_EAGAIN while requests on it are queued or
+ * running. */
+int trid_server_read_adi(trid_server server, int32_t adi, double* u);
+int trid_server_release_adi(trid_server server, int32_t adi);
+/* TRID_ENOMEM if the request can never fit the budget or the batch limits.
+ * The caller keeps every array alive until trid_server_wait reports it. */
+int trid_server_submit(trid_server server, const trid_request* request, uint64_t* ticket);
+/* timeout_us < 0 waits without limit; TRID_EAGAIN on timeout. After a final
+ * state is reported the ticket is retired. */
+int trid_server_wait(trid_server server, uint64_t ticket, int64_t timeout_us,
+                     trid_request_result* result);
+/* Cancels a request that is still queued; TRID_EINVAL once it is running. */
+int trid_server_cancel(trid_server server, uint64_t ticket);
+int trid_server_get_stats(trid_server server, trid_server_stats* stats);
 
 /* ---------------------------------------------------------------------- */
 /* Plan introspection (host only; no device needed)                         */

@@ -261,3 +261,42 @@ def residual(a, b, c, d, x, n: int | None = None, offsets=None) -> np.ndarray:
     out = np.empty(ns)
     lib().oracle_residual_ragged(*(_p(v) for v in arrs), _p(offsets), ns, _p(out))
     return out
+
+
+# --------------------------------------------------------------------------
+# ADI (2D heat equation, Peaceman-Rachford), FP64, for checking the device
+# ADI steps: each half step solves n tridiag(-r, 1+2r, -r) systems with the
+# oracle's Thomas.
+# --------------------------------------------------------------------------
+
+def _sweep_rows(rhs: np.ndarray, r: float) -> np.ndarray:
+    n = rhs.shape[1]
+    m = rhs.shape[0]
+    a = np.full(m * n, -r)
+    c = np.full(m * n, -r)
+    b = np.full(m * n, 1.0 + 2.0 * r)
+    return thomas(a, b, c, np.ascontiguousarray(rhs).reshape(-1), n=n).reshape(m, n)
+
+
+def adi_step_np(u: np.ndarray, r: float) -> np.ndarray:
+    """(I - r dxx) u* = (I + r dyy) u; (I - r dyy) u' = (I + r dxx) u*,
+    zero Dirichlet boundary; rows are x (contiguous), columns y."""
+    u = np.asarray(u, dtype=np.float64)
+    up = np.zeros_like(u)
+    dn = np.zeros_like(u)
+    up[1:] = u[:-1]
+    dn[:-1] = u[1:]
+    us = _sweep_rows(u + r * (up - 2 * u + dn), r)
+    lf = np.zeros_like(us)
+    rt = np.zeros_like(us)
+    lf[:, 1:] = us[:, :-1]
+    rt[:, :-1] = us[:, 1:]
+    rhs = us + r * (lf - 2 * us + rt)
+    return _sweep_rows(rhs.T, r).T
+
+
+def adi_mode_factor(n: int, p: int, q: int, r: float) -> float:
+    """Exact one-step amplification of the mode sin(p pi i/(n+1)) sin(q pi j/(n+1))."""
+    lp = 4 * np.sin(p * np.pi / (2 * (n + 1))) ** 2
+    lq = 4 * np.sin(q * np.pi / (2 * (n + 1))) ** 2
+    return (1 - r * lp) * (1 - r * lq) / ((1 + r * lp) * (1 + r * lq))

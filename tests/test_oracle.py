@@ -279,3 +279,14 @@ def test_residual(oracle):
     x2[5] += 1e-6
     r = oracle.residual(a, b, c, d, x2, n=n)
     assert r[0] > 1e-7 and r[1:].max() <= 1e-14
+
+
+@pytest.mark.parametrize("n", [7, 32])
+def test_adi_oracle_mode_closed_form(oracle, n):
+    # the numpy ADI step against the exact amplification of a sine mode
+    i = np.arange(1, n + 1)
+    for p, q, r in ((1, 1, 0.3), (3, 5, 2.0), (n, 2, 9.5)):
+        u0 = np.outer(np.sin(p * np.pi * i / (n + 1)), np.sin(q * np.pi * i / (n + 1)))
+        u1 = oracle.adi_step_np(u0, r)
+        g = oracle.adi_mode_factor(n, p, q, r)
+        assert np.max(np.abs(u1 - g * u0)) <= 1e-13 * max(1.0, abs(g))

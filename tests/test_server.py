@@ -1,0 +1,152 @@
+"""CPU tests of the request server's scheduling with the fake executor (no
+device work, nothing computed): admission never exceeds the HBM budget,
+requests that can never fit are rejected, queued requests can be cancelled,
+a failing batch fails only its own requests, operators are reference
+counted, and concurrent submitters are all served."""
+import threading
+
+import numpy as np
+import pytest
+
+
+@pytest.fixture
+def S(trid):
+    import tridiagonalsolvers_amd.server as S
+    return S
+
+
+def _req(rng, lo=32, hi=4000, most=20):
+    sizes = rng.integers(lo, hi, size=int(rng.integers(1, most)))
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    n = int(offs[-1])
+    return offs, [np.ones(n) for _ in range(4)], np.zeros(n)
+
+
+def test_budget_is_never_exceeded(S):
+    budget = 48 << 20
+    srv = S.Server(fake=True, hbm_budget_bytes=budget, slots=3, batch_window_us=100)
+    rng = np.random.default_rng(1)
+    peak_seen = 0
+    tickets = []
+
+    def client(seed):
+        r = np.random.default_rng(seed)
+        for _ in range(60):
+            offs, arrs, x = _req(r)
+            tickets.append(srv.submit(arrs[3], x, offs, *arrs[:3]))
+
+    ths = [threading.Thread(target=client, args=(s,)) for s in range(4)]
+    for t in ths:
+        t.start()
+    while any(t.is_alive() for t in ths):
+        st = srv.stats()
+        peak_seen = max(peak_seen, st["device_bytes_in_use"])
+    for t in ths:
+        t.join()
+    res = [srv.wait(t) for t in tickets]
+    st = srv.stats()
+    assert all(r["state"] == "done" and r["status"] == 0 for r in res)
+    assert st["completed"] == 240 and st["queued"] == 0
+    assert st["peak_device_bytes"] <= budget and peak_seen <= budget
+    assert st["device_bytes_in_use"] == 0
+    assert st["batches"] <= 240
+    srv.close()
+
+
+def test_request_that_never_fits_is_rejected(S, trid):
+    srv = S.Server(fake=True, hbm_budget_bytes=1 << 20)   # 1 MiB
+    n = 100000                                            # 5 arrays x 800 KB
+    offs = np.array([0, 5000] + [5000 * k for k in range(2, 21)], dtype=np.int64)
+    arrs = [np.ones(n) for _ in range(4)]
+    with pytest.raises(trid.TridError) as e:
+        srv.submit(arrs[3], np.zeros(n), offs, *arrs[:3])
+    assert e.value.status == 2      # TRID_ENOMEM
+    with pytest.raises(trid.TridError) as e:
+        srv.submit(np.ones(20000), np.zeros(20000), np.array([0, 20000]), *arrs[:3])
+    assert e.value.status == 4      # above 16384 unknowns: TRID_ENOTSUP
+    srv.close()
+
+
+def test_cancel_queued_requests(S):
+    # one slot busy for 200 ms per batch: later arrivals wait in the queue
+    srv = S.Server(fake=True, slots=1, batch_window_us=200000, max_batch_unknowns=1 << 16,
+                   fake_batch_us=200000)
+    rng = np.random.default_rng(3)
+    tickets = []
+    for _ in range(40):
+        offs, arrs, x = _req(rng, 2000, 4000, 8)
+        tickets.append(srv.submit(arrs[3], x, offs, *arrs[:3]))
+    cancelled = [t for t in tickets[20:] if srv.cancel(t)]
+    assert cancelled, "requests behind a busy slot should still be queued"
+    res = {t: srv.wait(t) for t in tickets}
+    for t in tickets:
+        want = "cancelled" if t in cancelled else "done"
+        assert res[t]["state"] == want
+    st = srv.stats()
+    assert st["cancelled"] == len(cancelled)
+    assert st["completed"] == len(tickets) - len(cancelled)
+    assert not srv.cancel(tickets[0])   # already finished and retired
+    srv.close()
+
+
+def test_failed_batch_fails_only_its_requests(S):
+    srv = S.Server(fake=True, slots=2, fake_fail_every=3)
+    rng = np.random.default_rng(5)
+    tickets = []
+    for _ in range(30):
+        offs, arrs, x = _req(rng)
+        tickets.append(srv.submit(arrs[3], x, offs, *arrs[:3]))
+    res = [srv.wait(t) for t in tickets]
+    failed = [r for r in res if r["state"] == "failed"]
+    done = [r for r in res if r["state"] == "done"]
+    assert failed and done and len(failed) + len(done) == 30
+    assert all(r["status"] == 3 for r in failed)          # TRID_EHIP
+    assert all(r["batch"] % 3 == 2 for r in failed)
+    assert all(r["batch"] % 3 != 2 for r in done)
+    srv.close()
+
+
+def test_operator_reference_counting(S):
+    srv = S.Server(fake=True, slots=1)
+    sizes = np.array([100, 200, 300], dtype=np.int64)
+    n = int(sizes.sum())
+    op = srv.register_operator(sizes, np.ones(n), 4 * np.ones(n), np.ones(n))
+    x = np.zeros(n)
+    t = srv.submit(np.ones(n), x, op=op)
+    assert srv.release_operator(op) == 6                 # TRID_EAGAIN while in use
+    assert srv.wait(t)["state"] == "done"
+    assert srv.release_operator(op) == 0
+    assert srv.stats()["device_bytes_in_use"] == 0
+    srv.close()
+
+
+def test_bad_arguments(S, trid):
+    with pytest.raises(trid.TridError):
+        S.Server(fake=True, slots=0 - 1)
+    srv = S.Server(fake=True)
+    with pytest.raises(trid.TridError):
+        srv.submit(np.ones(4), np.zeros(4), np.array([0, 1, 4]), *(np.ones(4),) * 3)  # size 1
+    with pytest.raises(trid.TridError):
+        srv.submit(np.ones(4), np.zeros(4), op=99)        # unknown operator
+    srv.close()
+
+
+def test_adi_requests_are_serialized_per_grid(S):
+    srv = S.Server(fake=True, slots=3, fake_batch_us=2000)
+    n = 64
+    g1 = srv.create_adi(np.zeros(n * n))
+    g2 = srv.create_adi(np.zeros(n * n))
+    t1 = [srv.submit_adi(g1, 0.5) for _ in range(4)]
+    t2 = [srv.submit_adi(g2, 2.0) for _ in range(4)]
+    assert srv.release_adi(g1) == 6          # TRID_EAGAIN while in use
+    r1 = [srv.wait(t) for t in t1]
+    r2 = [srv.wait(t) for t in t2]
+    assert all(r["state"] == "done" for r in r1 + r2)
+    # steps of one grid run in submission order, one at a time
+    for rs in (r1, r2):
+        ids = [r["batch"] for r in rs]
+        assert ids == sorted(ids)
+        for a, b in zip(rs, rs[1:]):
+            assert b["latency_us"] - b["queue_us"] >= 0
+    assert srv.release_adi(g1) == 0 and srv.release_adi(g2) == 0
+    srv.close()

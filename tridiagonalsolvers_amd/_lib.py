@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtrid.so")
 
 # trid_status
-OK, EINVAL, ENOMEM, EHIP, ENOTSUP, ENODEV = range(6)
+OK, EINVAL, ENOMEM, EHIP, ENOTSUP, ENODEV, EAGAIN = range(7)
 # trid_method
 CR, PCR, CRPCR, CRNBC, CRPCRNBC, CRNBC32, PCRGAUSS, AUTO = range(8)
 METHODS = {"cr": CR, "pcr": PCR, "crpcr": CRPCR, "crnbc": CRNBC, "crpcrnbc": CRPCRNBC,
@@ -30,6 +30,11 @@ EXPORTS = (
     "trid_solve_refined_f64", "trid_residual_f64",
     "trid_ragged_plan_create", "trid_ragged_plan_destroy", "trid_ragged_plan_info",
     "trid_ragged_solve_f64",
+    "trid_server_create", "trid_server_destroy", "trid_server_register_operator",
+    "trid_server_release_operator", "trid_server_submit", "trid_server_wait",
+    "trid_server_cancel", "trid_server_get_stats",
+    "trid_adi_create", "trid_adi_destroy", "trid_adi_grid", "trid_adi_step",
+    "trid_server_create_adi", "trid_server_read_adi", "trid_server_release_adi",
 )
 
 
@@ -56,6 +61,41 @@ class RaggedInfo(ctypes.Structure):
     _fields_ = [("num_systems", ctypes.c_int64), ("total_unknowns", ctypes.c_int64),
                 ("num_buckets", ctypes.c_int32), ("workgroup_items", ctypes.c_int32),
                 ("wave_items", ctypes.c_int32), ("grid", ctypes.c_int32)]
+
+
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("slots", ctypes.c_int32),
+                ("max_batch_systems", ctypes.c_int32), ("max_batch_unknowns", ctypes.c_int64),
+                ("batch_window_us", ctypes.c_int32), ("iters", ctypes.c_int32),
+                ("flags", ctypes.c_int32), ("fake_fail_every", ctypes.c_int32),
+                ("fake_batch_us", ctypes.c_int32),
+                ("bucket_table", ctypes.c_void_p), ("bucket_table_size", ctypes.c_int32)]
+
+
+REQ_SOLVE, REQ_OPERATOR, REQ_ADI = 0, 1, 2
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("num_systems", ctypes.c_int64),
+                ("offsets", ctypes.c_void_p),
+                ("a", ctypes.c_void_p), ("b", ctypes.c_void_p), ("c", ctypes.c_void_p),
+                ("d", ctypes.c_void_p), ("x", ctypes.c_void_p), ("op", ctypes.c_int32),
+                ("adi", ctypes.c_int32), ("adi_steps", ctypes.c_int32),
+                ("adi_r", ctypes.c_double)]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("state", ctypes.c_int32),
+                ("latency_us", ctypes.c_double), ("queue_us", ctypes.c_double),
+                ("batch", ctypes.c_int64), ("batch_systems", ctypes.c_int64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("completed", ctypes.c_int64),
+                ("failed", ctypes.c_int64), ("cancelled", ctypes.c_int64),
+                ("batches", ctypes.c_int64), ("queued", ctypes.c_int64),
+                ("device_bytes_in_use", ctypes.c_int64), ("peak_device_bytes", ctypes.c_int64),
+                ("budget_bytes", ctypes.c_int64)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -128,6 +168,31 @@ def lib() -> ctypes.CDLL:
     L.trid_ragged_plan_info.restype = i32
     L.trid_ragged_solve_f64.argtypes = [P, P, P, P, P, P, i32, i32, P]
     L.trid_ragged_solve_f64.restype = i32
+    L.trid_server_create.argtypes = [ctypes.POINTER(P), ctypes.POINTER(ServerOpts)]
+    L.trid_server_destroy.argtypes = [P]
+    L.trid_server_register_operator.argtypes = [P, P, i64, P, P, P, ctypes.POINTER(ctypes.c_int32)]
+    L.trid_server_release_operator.argtypes = [P, ctypes.c_int32]
+    L.trid_server_submit.argtypes = [P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]
+    L.trid_server_wait.argtypes = [P, ctypes.c_uint64, i64, ctypes.POINTER(RequestResult)]
+    L.trid_server_cancel.argtypes = [P, ctypes.c_uint64]
+    L.trid_server_get_stats.argtypes = [P, ctypes.POINTER(ServerStats)]
+    L.trid_server_create_adi.argtypes = [P, i64, P, ctypes.POINTER(ctypes.c_int32)]
+    L.trid_server_read_adi.argtypes = [P, ctypes.c_int32, P]
+    L.trid_server_release_adi.argtypes = [P, ctypes.c_int32]
+    for name in ("trid_server_create_adi", "trid_server_read_adi", "trid_server_release_adi"):
+        getattr(L, name).restype = i32
+    for name in ("trid_server_create", "trid_server_destroy", "trid_server_register_operator",
+                 "trid_server_release_operator", "trid_server_submit", "trid_server_wait",
+                 "trid_server_cancel", "trid_server_get_stats"):
+        getattr(L, name).restype = i32
+    L.trid_adi_create.argtypes = [ctypes.POINTER(P), i64, P]
+    L.trid_adi_create.restype = i32
+    L.trid_adi_destroy.argtypes = [P]
+    L.trid_adi_destroy.restype = i32
+    L.trid_adi_grid.argtypes = [P]
+    L.trid_adi_grid.restype = P
+    L.trid_adi_step.argtypes = [P, ctypes.c_double, i32, P]
+    L.trid_adi_step.restype = i32
     L.trid_plan_query.argtypes = [i32, i64, i64, i32, ctypes.POINTER(Opts),
                                   ctypes.POINTER(PlanInfo)]
     L.trid_plan_query.restype = i32

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256) void reg_crpcr_kernel(const T* __restrict__ A,
                                                         const T* __restrict__ D,
                                                         T* __restrict__ X, int64_t n,
                                                         int64_t N, int L, int logL, int rest,
-                                                        int vec)
+                                                        int vec, ConstCoef<T> cc)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int ll = lane & (L - 1);
@@ -60,12 +60,12 @@ __global__ __launch_bounds__(256) void reg_crpcr_kernel(const T* __restrict__ A,
     Eq<T> e[M];
     if constexpr (M % E == 0 && IO >= 2) {
         if (vec) {
-            load_staged<T, M, IO == 3>(A, B, C, D, wave_base, total, lane, lds, e);
+            load_staged<T, M, IO == 3>(A, B, C, D, wave_base, total, lane, lds, e, cc);
             if (!live) set_identity<T, M>(e);
         }
     } else if constexpr (M % E == 0) {
         if (vec) {
-            if (live) load_direct<T, M, IO == 0>(A, B, C, D, base + j0, e);
+            if (live) load_direct<T, M, IO == 0>(A, B, C, D, base + j0, e, cc);
             else set_identity<T, M>(e);
         }
     }
@@ -73,7 +73,9 @@ __global__ __launch_bounds__(256) void reg_crpcr_kernel(const T* __restrict__ A,
 #pragma unroll
         for (int s = 0; s < M; ++s) {
             const int64_t j = j0 + s;
-            if (live && j < n) e[s] = Eq<T>{A[base + j], B[base + j], C[base + j], D[base + j]};
+            if (live && j < n)
+                e[s] = cc.on ? Eq<T>{cc.a, cc.b, cc.c, D[base + j]}
+                             : Eq<T>{A[base + j], B[base + j], C[base + j], D[base + j]};
             else e[s] = identity_eq<T>();
         }
     }
@@ -116,7 +118,8 @@ static hipError_t launch_reg_io(const RegArgs<T>& g, int R, dim3 grid, dim3 bloc
     case RR:                                                                                 \
         if constexpr (RR <= M) {                                                             \
             hipLaunchKernelGGL((reg_crpcr_kernel<T, M, RR, IO>), grid, block, lds, g.stream, \
-                               g.a, g.b, g.c, g.d, g.x, g.n, g.N, g.L, logL, g.rest, g.vec); \
+                               g.a, g.b, g.c, g.d, g.x, g.n, g.N, g.L, logL, g.rest, g.vec,    \
+                               ConstCoef<T>{g.cc != 0, g.a0, g.b0, g.c0});                   \
             return hipGetLastError();                                                        \
         }                                                                                    \
         break;

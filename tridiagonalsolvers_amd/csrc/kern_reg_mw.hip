@@ -34,7 +34,7 @@ __global__ __launch_bounds__(1024) void reg_mw_kernel(const T* __restrict__ A,
                                                       const T* __restrict__ C,
                                                       const T* __restrict__ D,
                                                       T* __restrict__ X, int64_t n, int rest,
-                                                      int vec, int64_t N)
+                                                      int vec, int64_t N, ConstCoef<T> cc)
 {
     constexpr int E = 16 / (int)sizeof(T);
     const int W = blockDim.x / kWave;
@@ -55,12 +55,12 @@ __global__ __launch_bounds__(1024) void reg_mw_kernel(const T* __restrict__ A,
     if constexpr (M % E == 0 && IO >= 2) {
         if (vec) {
             load_staged<T, M, IO == 3>(A, B, C, D, base + (int64_t)w * kWave * M, n * N, lane,
-                                       lds_stage, e);
+                                       lds_stage, e, cc);
             loaded = true;
         }
     } else if constexpr (M % E == 0) {
         if (vec) {
-            load_direct<T, M, IO == 0>(A, B, C, D, base + j0, e);
+            load_direct<T, M, IO == 0>(A, B, C, D, base + j0, e, cc);
             loaded = true;
         }
     }
@@ -68,7 +68,9 @@ __global__ __launch_bounds__(1024) void reg_mw_kernel(const T* __restrict__ A,
 #pragma unroll
         for (int s = 0; s < M; ++s) {
             const int64_t j = j0 + s;
-            if (j < n) e[s] = Eq<T>{A[base + j], B[base + j], C[base + j], D[base + j]};
+            if (j < n)
+                e[s] = cc.on ? Eq<T>{cc.a, cc.b, cc.c, D[base + j]}
+                             : Eq<T>{A[base + j], B[base + j], C[base + j], D[base + j]};
             else e[s] = identity_eq<T>();
         }
     }
@@ -120,7 +122,8 @@ static hipError_t launch_mw_io(const RegArgs<T>& g)
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL((reg_mw_kernel<T, M, IO>), dim3((unsigned)g.N), dim3(kWave * W), lds,
-                       g.stream, g.a, g.b, g.c, g.d, g.x, g.n, g.rest, g.vec, g.N);
+                       g.stream, g.a, g.b, g.c, g.d, g.x, g.n, g.rest, g.vec, g.N,
+                       ConstCoef<T>{g.cc != 0, g.a0, g.b0, g.c0});
     return hipGetLastError();
 }
 

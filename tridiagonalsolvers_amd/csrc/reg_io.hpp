@@ -34,26 +34,37 @@ static __device__ __forceinline__ void set_identity(Eq<T> (&e)[M])
 }
 
 // Per-lane 16-byte loads of the lane's M consecutive unknowns.
+// Constant coefficients (cc): a, b, c are the same scalars for every row
+// and only d is read.
+template <typename T>
+struct ConstCoef {
+    bool on;
+    T a, b, c;
+};
+
 template <typename T, int M, bool NT>
 static __device__ __forceinline__ void load_direct(const T* __restrict__ A, const T* __restrict__ B,
                                             const T* __restrict__ C, const T* __restrict__ D,
-                                            int64_t off, Eq<T> (&e)[M])
+                                            int64_t off, Eq<T> (&e)[M], ConstCoef<T> cc)
 {
     constexpr int E = 16 / sizeof(T);
     T ta[M], tb[M], tc[M], td[M];
 #pragma unroll
     for (int q = 0; q < M / E; ++q) {
-        u32x4 va = gload16<NT>(A + off + q * E);
-        u32x4 vb = gload16<NT>(B + off + q * E);
-        u32x4 vc = gload16<NT>(C + off + q * E);
         u32x4 vd = gload16<NT>(D + off + q * E);
-        __builtin_memcpy(&ta[q * E], &va, 16);
-        __builtin_memcpy(&tb[q * E], &vb, 16);
-        __builtin_memcpy(&tc[q * E], &vc, 16);
         __builtin_memcpy(&td[q * E], &vd, 16);
+        if (!cc.on) {
+            u32x4 va = gload16<NT>(A + off + q * E);
+            u32x4 vb = gload16<NT>(B + off + q * E);
+            u32x4 vc = gload16<NT>(C + off + q * E);
+            __builtin_memcpy(&ta[q * E], &va, 16);
+            __builtin_memcpy(&tb[q * E], &vb, 16);
+            __builtin_memcpy(&tc[q * E], &vc, 16);
+        }
     }
 #pragma unroll
-    for (int s = 0; s < M; ++s) e[s] = Eq<T>{ta[s], tb[s], tc[s], td[s]};
+    for (int s = 0; s < M; ++s)
+        e[s] = cc.on ? Eq<T>{cc.a, cc.b, cc.c, td[s]} : Eq<T>{ta[s], tb[s], tc[s], td[s]};
 }
 
 template <typename T, int M, bool NT>
@@ -92,23 +103,33 @@ template <typename T, int M, bool NT>
 static __device__ __forceinline__ void load_staged(const T* __restrict__ A, const T* __restrict__ B,
                                             const T* __restrict__ C, const T* __restrict__ D,
                                             int64_t wave_base, int64_t total, int lane,
-                                            char* lds, Eq<T> (&e)[M])
+                                            char* lds, Eq<T> (&e)[M], ConstCoef<T> cc)
 {
     using S = Stage<T, M>;
     constexpr int E = S::E, Q = S::Q;
     u32x4 r[4][Q];
     const T* src[4] = {A, B, C, D};
+    const int first = cc.on ? 3 : 0;  // constant coefficients: d only
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int64_t el = wave_base + (int64_t)(kWave * q + lane) * E;
         const bool ok = el < total;
 #pragma unroll
         for (int arr = 0; arr < 4; ++arr)
-            r[arr][q] = ok ? gload16<NT>(src[arr] + el) : u32x4{0u, 0u, 0u, 0u};
+            r[arr][q] = (ok && arr >= first) ? gload16<NT>(src[arr] + el) : u32x4{0u, 0u, 0u, 0u};
     }
     T v[4][M];
+    if (cc.on) {
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+            v[0][s] = cc.a;
+            v[1][s] = cc.b;
+            v[2][s] = cc.c;
+        }
+    }
 #pragma unroll
     for (int arr = 0; arr < 4; ++arr) {
+        if (arr < first) continue;
 #pragma unroll
         for (int q = 0; q < Q; ++q)
             *reinterpret_cast<u32x4*>(lds + S::chunk_addr(kWave * q + lane)) = r[arr][q];

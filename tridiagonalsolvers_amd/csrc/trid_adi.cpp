@@ -1,0 +1,89 @@
+// trid_adi.cpp -- resident ADI grids: Peaceman-Rachford steps of the 2D heat
+// equation (kern_adi.hip) on a device grid that stays resident across steps.
+// No counterpart in the reference; this is the caller the batched solve was
+// built for (n row systems, n strided column systems per half step).
+#include <hip/hip_runtime.h>
+
+#include <new>
+
+#include "trid.h"
+#include "trid_launch.hpp"
+#include "trid_plan.hpp"
+
+struct trid_adi_s {
+    int64_t n = 0;
+    double* u = nullptr;
+    double* t = nullptr;
+    double* f = nullptr;  // cp | inv_m
+    bool owns_u = false;
+    trid::Plan plan{};
+};
+
+extern "C" {
+
+int trid_adi_create(trid_adi* out, int64_t n, double* u_device)
+{
+    if (!out || n < 2 || n > 16384) return TRID_EINVAL;
+    *out = nullptr;
+    trid_opts o{};
+    trid::Plan p;
+    int st = trid::make_plan(TRID_CRPCR, n, n, 8, o, &p);
+    if (st != TRID_OK) return st;
+    if (p.family != trid::Family::Reg) return TRID_ENOTSUP;
+    trid_adi_s* a = new (std::nothrow) trid_adi_s();
+    if (!a) return TRID_ENOMEM;
+    a->n = n;
+    a->plan = p;
+    const size_t grid = (size_t)n * (size_t)n * sizeof(double);
+    hipError_t e = hipSuccess;
+    if (u_device) {
+        a->u = u_device;
+    } else {
+        e = hipMalloc(reinterpret_cast<void**>(&a->u), grid);
+        a->owns_u = e == hipSuccess;
+    }
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&a->t), grid);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&a->f), 2 * (size_t)n * sizeof(double));
+    if (e != hipSuccess) {
+        trid_adi_destroy(a);
+        return TRID_ENOMEM;
+    }
+    *out = a;
+    return TRID_OK;
+}
+
+int trid_adi_destroy(trid_adi a)
+{
+    if (!a) return TRID_EINVAL;
+    if (a->owns_u && a->u) (void)hipFree(a->u);
+    if (a->t) (void)hipFree(a->t);
+    if (a->f) (void)hipFree(a->f);
+    delete a;
+    return TRID_OK;
+}
+
+double* trid_adi_grid(trid_adi a) { return a ? a->u : nullptr; }
+
+int trid_adi_step(trid_adi a, double r, int steps, void* stream)
+{
+    if (!a || !(r > 0.0) || r > 1.0e6 || steps < 0) return TRID_EINVAL;
+    trid::AdiArgs g{};
+    g.u = a->u;
+    g.t = a->t;
+    g.cp = a->f;
+    g.inv_m = a->f + a->n;
+    g.n = a->n;
+    g.r = r;
+    g.M = a->plan.M;
+    g.L = a->plan.L;
+    g.rest = a->plan.rest;
+    g.vec = a->plan.dense && ((reinterpret_cast<uintptr_t>(a->u) |
+                               reinterpret_cast<uintptr_t>(a->t)) & 15u) == 0;
+    g.stream = static_cast<hipStream_t>(stream);
+    for (int s = 0; s < steps; ++s) {
+        if (trid::launch_adi_step(g) != hipSuccess) return TRID_EHIP;
+    }
+    return TRID_OK;
+}
+
+}  // extern "C"

@@ -27,6 +27,8 @@ struct RegArgs {
     int io;          // dense-path I/O mode: 0 per-lane non-temporal, 1 per-lane,
                      // 2 LDS-staged coalesced, 3 LDS-staged non-temporal
     hipStream_t stream;
+    int cc = 0;      // constant coefficients a0, b0, c0 on every row (d only)
+    T a0 = 0, b0 = 1, c0 = 0;
 };
 
 template <typename T>
@@ -148,6 +150,20 @@ int ragged_build_host(const int64_t* offsets, int64_t N, const trid_bucket_cfg* 
 template <typename T>
 hipError_t launch_generate(int kind, uint64_t seed, T* a, T* b, T* c, T* d, int64_t n,
                            int64_t N, hipStream_t stream);
+
+// One Peaceman-Rachford ADI step on an n x n FP64 grid (kern_adi.hip).
+struct AdiArgs {
+    double* u;      // grid, row-major, n*n (input and output)
+    double* t;      // scratch grid, n*n
+    double* cp;     // n column-sweep factors
+    double* inv_m;  // n column-sweep inverse pivots
+    int64_t n;
+    double r;
+    int M, L, rest, vec;  // register-kernel plan of the row sweep (n systems of n)
+    hipStream_t stream;
+};
+
+hipError_t launch_adi_step(const AdiArgs& g);
 
 // Relative residual per system (kern_util.hip).
 hipError_t launch_residual(const double* a, const double* b, const double* c, const double* d,
