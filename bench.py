@@ -29,9 +29,11 @@ METRIC = ("unknowns solved/s + % HBM roofline; served p50/p99 ms vs plain, "
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", default="uniform", choices=["uniform", "ragged"],
+    p.add_argument("--workload", default="uniform", choices=["uniform", "ragged", "serve"],
                    help="uniform: the headline batch; ragged: 100,000 systems of "
-                        "log-uniform sizes in [16, 8192], FP32 solve + FP64 refinement")
+                        "log-uniform sizes in [16, 8192], FP32 solve + FP64 refinement; "
+                        "serve: request server with Poisson arrivals and resident ADI grids")
+    p.add_argument("--requests", type=int, default=64)
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--method", default="crpcr")
@@ -214,6 +216,167 @@ def bench_ragged(args):
     print(json.dumps(out), flush=True)
 
 
+def _percentile(v, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(v, dtype=np.float64), q)) if len(v) else None
+
+
+def bench_serve(args):
+    """C5: the request server under Poisson arrivals of requests of 1..4,096
+    systems with sizes log-uniform in [32, 16,384] (FP64 host data, FP32 solve
+    + 2 FP64 refinements), interleaved with ADI steps (r ~ U[0.1, 10]) on
+    resident 2048^2 and 4096^2 grids. Reports served throughput at saturation
+    (all requests submitted at once) against plain per-size FP64 launches of
+    the same requests, and p50/p99 latency at 70% of the saturated rate."""
+    import threading
+
+    import numpy as np
+    import torch
+
+    import tridiagonalsolvers_amd as T
+    import tridiagonalsolvers_amd.ragged as R
+    import tridiagonalsolvers_amd.server as S
+
+    rng = np.random.default_rng(1)
+    max_req = 1 << 24
+    pool_n = 1 << 25
+    dev = torch.device("cuda:0")
+    pool = [torch.empty(pool_n, dtype=torch.float64, device=dev) for _ in range(4)]
+    T.generate(0, 7, *pool, pool_n, 1)
+    host = [v.cpu().numpy() for v in pool]
+    reqs = []
+    while len(reqs) < args.requests:
+        k = int(rng.integers(1, 4097))
+        sizes = R.log_uniform_sizes(k, 32, 16384, seed=int(rng.integers(1 << 30)))
+        tot = int(sizes.sum())
+        if tot > max_req:
+            continue
+        start = int(rng.integers(0, pool_n - tot))
+        reqs.append((R.offsets_of(sizes), start, tot))
+    total_unknowns = sum(r[2] for r in reqs)
+
+    def req_arrays(i):
+        offs, start, tot = reqs[i]
+        return offs, [h[start:start + tot] for h in host]
+
+    srv = S.Server(hbm_budget_bytes=16 << 30, slots=3, max_batch_unknowns=max_req,
+                   batch_window_us=300)
+    xs = [np.empty(r[2]) for r in reqs]
+    # warm-up (kernel load, first-touch of staging)
+    offs0, arr0 = req_arrays(0)
+    srv.wait(srv.submit(arr0[3], xs[0], offs0, *arr0[:3]))
+
+    # --- saturation: everything submitted at once -----------------------------
+    t0 = time.perf_counter()
+    tickets = []
+    for i in range(len(reqs)):
+        offs, arr = req_arrays(i)
+        tickets.append(srv.submit(arr[3], xs[i], offs, *arr[:3]))
+    sat = [srv.wait(t) for t in tickets]
+    sat_s = time.perf_counter() - t0
+    assert all(r["state"] == "done" for r in sat)
+    served_rate = total_unknowns / sat_s
+
+    # accuracy of the served results: device residual of every 4th request
+    worst_res = 0.0
+    for i in range(0, len(reqs), 4):
+        offs, start, tot = reqs[i]
+        d_offs = torch.from_numpy(offs).to(dev)
+        x = torch.from_numpy(xs[i]).to(dev)
+        rr = torch.empty(len(offs) - 1, dtype=torch.float64, device=dev)
+        R.residual(*(v[start:start + tot] for v in pool), x, d_offs, len(offs) - 1, rr)
+        worst_res = max(worst_res, float(rr.max()))
+
+    # --- plain per-size FP64 launches of the same requests, back to back ----------
+    h = T.Handle()
+    x_dev = torch.empty(max_req, dtype=torch.float64, device=dev)
+    t0 = time.perf_counter()
+    for i in range(len(reqs)):
+        offs, arr = req_arrays(i)
+        sizes = np.diff(offs)
+        order = np.argsort(sizes, kind="stable")
+        idx = np.concatenate([np.arange(offs[s], offs[s + 1]) for s in order])
+        g = [torch.from_numpy(v[idx]).to(dev) for v in arr]
+        ssz = sizes[order]
+        uniq, first, cnt = np.unique(ssz, return_index=True, return_counts=True)
+        pos = np.concatenate([[0], np.cumsum(ssz)])
+        for n, f, c in zip(uniq.tolist(), first.tolist(), cnt.tolist()):
+            o = int(pos[f]) * 8
+            h.solve("crpcr", *(v.data_ptr() + o for v in g), x_dev.data_ptr() + o, n, c,
+                    dtype="f64")
+        out = np.empty(len(idx))
+        out[idx] = x_dev[:len(idx)].cpu().numpy()
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    plain_rate = total_unknowns / plain_s
+
+    # --- Poisson arrivals at 70% of the saturated request rate, with ADI ----------
+    lam = 0.7 * len(reqs) / sat_s
+    gaps = rng.exponential(1.0 / lam, size=len(reqs))
+    grids = {}
+    for n in (2048, 4096):
+        grids[n] = srv.create_adi(np.zeros(n * n) + 1.0)
+    adi_period = 4.0 / lam
+    lat, adi_lat = [], {2048: [], 4096: []}
+    stop = threading.Event()
+
+    def adi_client():
+        r_rng = np.random.default_rng(2)
+        tk = []
+        while not stop.is_set():
+            for n, gid in grids.items():
+                tk.append((n, srv.submit_adi(gid, float(r_rng.uniform(0.1, 10.0)))))
+            time.sleep(adi_period)
+        for n, t in tk:
+            adi_lat[n].append(srv.wait(t)["latency_us"])
+
+    th = threading.Thread(target=adi_client)
+    th.start()
+    tickets = []
+    t0 = time.perf_counter()
+    for i, gap in enumerate(gaps):
+        time.sleep(gap)
+        offs, arr = req_arrays(i)
+        tickets.append(srv.submit(arr[3], xs[i], offs, *arr[:3]))
+    res = [srv.wait(t) for t in tickets]
+    poisson_s = time.perf_counter() - t0
+    stop.set()
+    th.join()
+    lat = [r["latency_us"] / 1e3 for r in res]
+    st = srv.stats()
+    for gid in grids.values():
+        srv.release_adi(gid)
+    srv.close()
+    h.close()
+    out = {
+        "metric": METRIC, "value": served_rate, "unit": "unknowns/s", "n_gpus": 1,
+        "steps": len(reqs), "warmup": 1, "ms_per_step": sat_s * 1e3 / len(reqs),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded device generator, FP64 host copies; request sizes seeded)",
+        "config": {"workload": f"served: {len(reqs)} requests of 1..4096 systems, sizes "
+                               f"log-uniform [32, 16384], FP32 solve + 2 FP64 refinements; "
+                               f"ADI 2048^2 and 4096^2 resident",
+                   "unknowns": total_unknowns, "slots": 3, "budget_bytes": 16 << 30},
+        "served_saturated": {"unknowns_per_s": served_rate, "seconds": sat_s,
+                             "max_rel_residual_sampled": worst_res},
+        "plain_per_size": {"unknowns_per_s": plain_rate, "seconds": plain_s},
+        "speedup_vs_plain_per_size": plain_s / sat_s,
+        "poisson": {"rate_req_per_s": lam, "seconds": poisson_s,
+                    "latency_ms_p50": _percentile(lat, 50), "latency_ms_p99": _percentile(lat, 99),
+                    "adi_2048_step_ms_p50": _percentile(adi_lat[2048], 50) / 1e3
+                    if adi_lat[2048] else None,
+                    "adi_2048_step_ms_p99": _percentile(adi_lat[2048], 99) / 1e3
+                    if adi_lat[2048] else None,
+                    "adi_4096_step_ms_p50": _percentile(adi_lat[4096], 50) / 1e3
+                    if adi_lat[4096] else None,
+                    "adi_4096_step_ms_p99": _percentile(adi_lat[4096], 99) / 1e3
+                    if adi_lat[4096] else None,
+                    "adi_steps": sum(len(v) for v in adi_lat.values())},
+        "server_stats": st,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -223,6 +386,9 @@ def main():
         sys.exit(2)
     if args.workload == "ragged":
         bench_ragged(args)
+        return
+    if args.workload == "serve":
+        bench_serve(args)
         return
     import torch
 

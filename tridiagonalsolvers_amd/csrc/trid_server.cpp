@@ -135,6 +135,40 @@ struct Pool {
 
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+// Host copies of a batch (into / out of pinned staging), split over up to 8
+// threads: one thread moves ~10 GB/s, below what the PCIe link takes.
+struct CopyJob {
+    void* dst;
+    const void* src;
+    size_t bytes;
+};
+
+void parallel_copy(const std::vector<CopyJob>& jobs)
+{
+    size_t total = 0;
+    for (const CopyJob& j : jobs) total += j.bytes;
+    const size_t kChunk = size_t(4) << 20;
+    int nt = (int)std::min<size_t>(8, total / kChunk);
+    if (nt <= 1) {
+        for (const CopyJob& j : jobs) std::memcpy(j.dst, j.src, j.bytes);
+        return;
+    }
+    // cut every job into pieces, deal the pieces round-robin
+    std::vector<CopyJob> pieces;
+    for (const CopyJob& j : jobs)
+        for (size_t o = 0; o < j.bytes; o += kChunk)
+            pieces.push_back(CopyJob{static_cast<char*>(j.dst) + o,
+                                     static_cast<const char*>(j.src) + o,
+                                     std::min(kChunk, j.bytes - o)});
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&pieces, t, nt] {
+            for (size_t i = t; i < pieces.size(); i += nt)
+                std::memcpy(pieces[i].dst, pieces[i].src, pieces[i].bytes);
+        });
+    for (auto& x : th) x.join();
+}
+
 // Device (and staging) layout of one batch of N systems, `total` unknowns.
 struct BatchLayout {
     size_t arr;   // bytes of one FP64 array
@@ -296,15 +330,17 @@ int trid_server_s::launch(Slot& s)
     double* pc = reinterpret_cast<double*>(P + 2 * L.arr);
     double* pd = reinterpret_cast<double*>(P + 3 * L.arr);
     char* pdesc = P + 5 * L.arr;
+    std::vector<CopyJob> jobs;
     for (Req* q : s.reqs) {
         const size_t nb = (size_t)q->unknowns * sizeof(double);
         if (q->r.op < 0) {
-            std::memcpy(pa + q->pos, q->r.a, nb);
-            std::memcpy(pb + q->pos, q->r.b, nb);
-            std::memcpy(pc + q->pos, q->r.c, nb);
+            jobs.push_back(CopyJob{pa + q->pos, q->r.a, nb});
+            jobs.push_back(CopyJob{pb + q->pos, q->r.b, nb});
+            jobs.push_back(CopyJob{pc + q->pos, q->r.c, nb});
         }
-        std::memcpy(pd + q->pos, q->r.d, nb);
+        jobs.push_back(CopyJob{pd + q->pos, q->r.d, nb});
     }
+    parallel_copy(jobs);
     std::memcpy(pdesc, offs.data(), sizeof(int64_t) * (N + 1));
     std::memcpy(pdesc + L.off, hp.order.data(), sizeof(int32_t) * N);
     std::memcpy(pdesc + L.off + L.ord, hp.items.data(), sizeof(trid::RaggedItem) * hp.items.size());
@@ -361,8 +397,10 @@ void trid_server_s::retire(Slot& s, bool ok)
     if (ok && !fake && !adi_req) {
         BatchLayout L(s.total, s.N);
         const double* px = reinterpret_cast<const double*>(s.pinned + 4 * L.arr);
+        std::vector<CopyJob> jobs;
         for (Req* q : s.reqs)
-            std::memcpy(q->r.x, px + q->pos, (size_t)q->unknowns * sizeof(double));
+            jobs.push_back(CopyJob{q->r.x, px + q->pos, (size_t)q->unknowns * sizeof(double)});
+        parallel_copy(jobs);
     }
     for (Req* q : s.reqs) finish(q, ok ? kDone : kFailed, ok ? TRID_OK : TRID_EHIP);
     if (s.dev_bytes) pool.release(s.dev_off, s.dev_bytes);
