@@ -302,7 +302,10 @@ def bench_serve(args):
         pos = np.concatenate([[0], np.cumsum(ssz)])
         for n, f, c in zip(uniq.tolist(), first.tolist(), cnt.tolist()):
             o = int(pos[f]) * 8
-            h.solve("crpcr", *(v.data_ptr() + o for v in g), x_dev.data_ptr() + o, n, c,
+            # FP64 CR-PCR holds systems up to 8192 in one workgroup; longer
+            # ones use the two-pass PCR + Thomas form
+            method = "crpcr" if n <= 8192 else "pcrgauss"
+            h.solve(method, *(v.data_ptr() + o for v in g), x_dev.data_ptr() + o, n, c,
                     dtype="f64")
         out = np.empty(len(idx))
         out[idx] = x_dev[:len(idx)].cpu().numpy()
