@@ -207,9 +207,10 @@ int trid_generate_f64(int kind, uint64_t seed, double* a, double* b, double* c, 
  * ragged batch occupies [offsets[s], offsets[s+1]) of flat FP64 device
  * arrays a, b, c, d, x; sizes 2..16384. A plan buckets the systems by padded
  * size (power of two) and builds the work queue of one persistent kernel
- * (workgroup items for 1024..8192, wave items below); each bucket solves in
+ * (workgroup items for 1024..16384, wave items below); each bucket solves in
  * FP32 with its (per_lane, lanes, rest) configuration and refines in FP64
- * `iters` times. The plan depends only on the sizes and the table.
+ * `iters` times, or (TRID_RAGGED_FP64) solves once in FP64 with the same
+ * configuration. The plan depends only on the sizes and the table.
  */
 typedef struct trid_bucket_cfg {
     int32_t size;      /* padded system size (power of two, 2..16384)           */
@@ -232,6 +233,11 @@ typedef struct trid_ragged_plan_s* trid_ragged_plan;
 /* Do not capture/replay a hipGraph (a plain launch); also used when stream is
  * NULL, which cannot be captured. */
 #define TRID_RAGGED_NO_GRAPH 1
+/* Solve every system in one FP64 pass of the same kernels instead of FP32 +
+ * FP64 refinement (`iters` must be 0). One read of a, b, c, d and one write
+ * of x per unknown; systems up to 8192 unknowns (TRID_ENOTSUP if the plan
+ * holds larger ones). */
+#define TRID_RAGGED_FP64 2
 
 /* offsets: host array of numSystems+1 element offsets. table: optional
  * per-bucket configurations (e.g. from the autotuner); buckets not listed use
@@ -242,7 +248,7 @@ int trid_ragged_plan_destroy(trid_ragged_plan plan);
 int trid_ragged_plan_info(trid_ragged_plan plan, trid_ragged_info* info,
                           trid_bucket_cfg* buckets, int64_t* bucketSystems, int capacity);
 /* Enqueue the ragged solve on `stream`. The first call for a given set of
- * pointers and `iters` captures the reset + persistent launch into a
+ * pointers, `iters` and mode captures the reset + persistent launch into a
  * hipGraph; later calls replay it. */
 int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* b,
                           const double* c, const double* d, double* x, int iters, int flags,

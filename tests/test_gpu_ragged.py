@@ -1,7 +1,8 @@
-"""GPU tests of the ragged persistent solve (trid_ragged_*): every system of a
-mixed-size batch against the FP64 Thomas oracle, FP64-level residuals after
-refinement, bit-identical results alone and in the mix, graph replays equal
-to the eager launch, and every autotuner candidate within tolerance."""
+"""GPU tests of the ragged persistent solve (trid_ragged_*) in both modes
+(FP32 solve + FP64 refinement; one FP64 solve): every system of a
+mixed-size batch against the FP64 Thomas oracle, FP64-level residuals,
+bit-identical results alone and in the mix, graph replays equal to the
+eager launch, and every autotuner candidate within tolerance."""
 import numpy as np
 import pytest
 
@@ -11,6 +12,16 @@ pytestmark = pytest.mark.gpu
 
 EDGE_SIZES = [2, 3, 5, 16, 17, 100, 511, 512, 513, 1000, 1024, 1025, 2048, 3000, 4096,
               5000, 8191, 8192, 8193, 12000, 16384]
+# the FP64 mode takes systems up to 8192 unknowns
+EDGE_SIZES_FP64 = [n for n in EDGE_SIZES if n <= 8192]
+MODES = ["refine", "fp64"]
+
+
+def _solve(plan, dev, x, mode, **kw):
+    if mode == "fp64":
+        plan.solve(*dev, x, fp64=True, **kw)
+    else:
+        plan.solve(*dev, x, iters=2, **kw)
 
 
 def _batch(oracle, trid_ragged, sizes, seed=11, kind="dominant"):
@@ -38,10 +49,12 @@ def _check_ragged(oracle, offs, host, x, tol):
     return float(np.max(rel)), float(np.max(res))
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("kind", ["dominant", "weak"])
-def test_ragged_matches_oracle(oracle, trid, rg, kind):
+def test_ragged_matches_oracle(oracle, trid, rg, kind, mode):
     import torch
-    sizes = np.concatenate([EDGE_SIZES, rg.log_uniform_sizes(1500, 16, 8192, seed=5)])
+    edge = EDGE_SIZES if mode == "refine" else EDGE_SIZES_FP64
+    sizes = np.concatenate([edge, rg.log_uniform_sizes(1500, 16, 8192, seed=5)])
     offs, host, dev = _batch(oracle, rg, sizes, kind=kind)
     x = torch.full_like(dev[0], float("nan"))
     plan = rg.RaggedPlan(offs)
@@ -50,11 +63,11 @@ def test_ragged_matches_oracle(oracle, trid, rg, kind):
     assert info["num_systems"] == len(sizes)
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
-        plan.solve(*dev, x, iters=2, stream=s)
+        _solve(plan, dev, x, mode, stream=s)
     s.synchronize()
     worst = _check_ragged(oracle, offs, host, x.cpu().numpy(),
                           1e-11 if kind == "dominant" else 1e-10)
-    print(f"{kind}: max rel err {worst[0]:.2e}, max rel residual {worst[1]:.2e}")
+    print(f"{kind} {mode}: max rel err {worst[0]:.2e}, max rel residual {worst[1]:.2e}")
     # the device residual kernel: <= 1e-12 here, and equal to the oracle's
     # where the residual is well above rounding level (perturbed unknowns)
     d_offs = torch.from_numpy(offs).cuda()
@@ -73,22 +86,24 @@ def test_ragged_matches_oracle(oracle, trid, rg, kind):
     plan.close()
 
 
-def test_ragged_alone_equals_in_the_mix(oracle, trid, rg):
+@pytest.mark.parametrize("mode", MODES)
+def test_ragged_alone_equals_in_the_mix(oracle, trid, rg, mode):
     import torch
-    sizes = np.concatenate([EDGE_SIZES, rg.log_uniform_sizes(300, 2, 8192, seed=9)])
+    edge = EDGE_SIZES if mode == "refine" else EDGE_SIZES_FP64
+    sizes = np.concatenate([edge, rg.log_uniform_sizes(300, 2, 8192, seed=9)])
     offs, host, dev = _batch(oracle, rg, sizes, seed=4)
     x = torch.full_like(dev[0], float("nan"))
     plan = rg.RaggedPlan(offs)
-    plan.solve(*dev, x, iters=2, graph=False)
+    _solve(plan, dev, x, mode, graph=False)
     torch.cuda.synchronize()
     xm = x.cpu().numpy()
     plan.close()
-    for s in list(range(len(EDGE_SIZES))) + [40, 77, 150]:
+    for s in list(range(len(edge))) + [40, 77, 150]:
         o, e = int(offs[s]), int(offs[s + 1])
         one = [v[o:e].contiguous() for v in dev]
         xs = torch.full_like(one[0], float("nan"))
         p1 = rg.RaggedPlan(np.array([0, e - o], dtype=np.int64))
-        p1.solve(*one, xs, iters=2, graph=False)
+        _solve(p1, one, xs, mode, graph=False)
         torch.cuda.synchronize()
         p1.close()
         assert np.array_equal(xs.cpu().numpy(), xm[o:e]), (s, e - o)
@@ -99,29 +114,52 @@ def test_ragged_graph_replays_match_eager(oracle, trid, rg):
     sizes = rg.log_uniform_sizes(2000, 16, 8192, seed=21)
     offs, host, dev = _batch(oracle, rg, sizes, seed=8)
     plan = rg.RaggedPlan(offs)
-    x_eager = torch.full_like(dev[0], float("nan"))
-    plan.solve(*dev, x_eager, iters=2, graph=False)
-    torch.cuda.synchronize()
-    ref = x_eager.cpu().numpy()
+    refs = {}
+    for mode in MODES:
+        x_eager = torch.full_like(dev[0], float("nan"))
+        _solve(plan, dev, x_eager, mode, graph=False)
+        torch.cuda.synchronize()
+        refs[mode] = x_eager.cpu().numpy()
     x = torch.full_like(dev[0], float("nan"))
     s = torch.cuda.Stream()
-    for rep in range(4):
+    # same pointers, alternating modes: each change of mode captures anew
+    for rep, mode in enumerate(["refine", "refine", "fp64", "fp64", "refine", "fp64"]):
         x.fill_(float("nan"))
         junk = torch.randn(1 << 20, device="cuda").sum()  # eager work between replays
         torch.cuda.synchronize()
         with torch.cuda.stream(s):
-            plan.solve(*dev, x, iters=2, stream=s)
+            _solve(plan, dev, x, mode, stream=s)
         s.synchronize()
-        assert np.array_equal(x.cpu().numpy(), ref), rep
+        assert np.array_equal(x.cpu().numpy(), refs[mode]), (rep, mode)
         del junk
     plan.close()
-    _check_ragged(oracle, offs, host, ref, 1e-11)
+    for mode in MODES:
+        _check_ragged(oracle, offs, host, refs[mode], 1e-11)
 
 
-def test_every_tuner_candidate_is_accurate(oracle, trid, rg):
+def test_ragged_fp64_mode_limits(oracle, trid, rg):
+    import torch
+    from tridiagonalsolvers_amd._lib import TridError
+    offs, host, dev = _batch(oracle, rg, [100, 16384, 30], seed=3)
+    x = torch.zeros_like(dev[0])
+    plan = rg.RaggedPlan(offs)
+    with pytest.raises(TridError):       # 16384 unknowns: refine mode only
+        plan.solve(*dev, x, fp64=True, graph=False)
+    plan.close()
+    offs, host, dev = _batch(oracle, rg, [100, 8192, 30], seed=3)
+    plan = rg.RaggedPlan(offs)
+    with pytest.raises(TridError):       # no refinement passes in the FP64 mode
+        plan.solve(*dev, x[:8322], iters=2, fp64=True, graph=False)
+    plan.close()
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_every_tuner_candidate_is_accurate(oracle, trid, rg, mode):
     import torch
     from tridiagonalsolvers_amd.autotune import candidates
     for size in (4, 64, 512, 1024, 8192, 16384):
+        if mode == "fp64" and size > 8192:
+            continue
         n = size - (size // 5)   # not a power of two: padded
         n = max(n, 2)
         sizes = [n] * 12
@@ -129,7 +167,7 @@ def test_every_tuner_candidate_is_accurate(oracle, trid, rg):
         for cand in candidates(size):
             x = torch.full_like(dev[0], float("nan"))
             plan = rg.RaggedPlan(offs, table=[cand])
-            plan.solve(*dev, x, iters=2, graph=False)
+            _solve(plan, dev, x, mode, graph=False)
             torch.cuda.synchronize()
             plan.close()
             _check_ragged(oracle, offs, host, x.cpu().numpy(), 1e-11)

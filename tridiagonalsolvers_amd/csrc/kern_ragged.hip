@@ -26,6 +26,8 @@
 #include "trid_device.hpp"
 #include "trid_launch.hpp"
 
+#include <type_traits>
+
 namespace trid {
 
 namespace {
@@ -33,41 +35,198 @@ namespace {
 constexpr int kRaggedThreads = 1024;
 constexpr int kRaggedWaves = kRaggedThreads / kWave;
 static_assert(kRaggedWaves == 16, "workgroup items assume 16 waves");
-// LDS of the workgroup items: 16/W groups of W waves, each with the
-// exchange area of wg_core and 2 W boundary unknowns. The largest case is
-// 16 waves at 16 unknowns per lane (16384-unknown systems).
-constexpr size_t kRaggedLds = 16 * (10 * 16 + 64 * 16 + 64 * 4 + 16);
 
-// FP32 operator + FP64 residual rows (as kern_refine.hip's build_rows) for a
-// ragged system starting at `base` with n unknowns; element-wise loads.
+// Transposition area of one wave (coalesced loads and stores, below).
+// Row r holds the M consecutive unknowns of lane r, padded by 16 bytes:
+// a row stride of M*8+16 bytes puts the 16 lanes of one pass of a 16-byte
+// LDS access on disjoint banks. M = 16 is staged in two rounds of 32 rows.
 template <int M>
-__device__ __forceinline__ void ragged_rows(const double* __restrict__ A,
-                                            const double* __restrict__ B,
-                                            const double* __restrict__ C,
-                                            const double* __restrict__ D, const double (&x)[M],
-                                            double x_left, double x_right, int64_t base,
-                                            int64_t j0, int64_t n, bool live, Eq<float> (&e)[M])
+struct RStage {
+    static constexpr int kRowBytes = M * 8 + 16;
+    static constexpr int kRounds = M > 8 ? M / 8 : 1;
+    static constexpr int kQ = M / kRounds;         // loads per lane per round
+    static constexpr int kRows = kWave / kRounds;  // rows per round
+    static constexpr int kBytes = kRows * kRowBytes;
+};
+constexpr int kStageWaveBytes = RStage<8>::kBytes;  // 5120, the largest
+static_assert(RStage<16>::kBytes <= kStageWaveBytes, "stage area");
+
+// Kernel modes: FP32 solve + FP64 refinement (workgroup items at 8
+// unknowns per lane, or also 16 in the wide form for 16384-unknown
+// systems), or one FP64 solve (8 unknowns per lane, systems up to 8192).
+enum RaggedMode { kRefine = 0, kRefineWide = 1, kDirect = 2 };
+
+// LDS of the workgroup items: 16/W groups of W waves, each with the
+// exchange area of wg_core and 2 W boundary unknowns (wg_core_lds_bytes is
+// linear in W, so the 16-wave group is the largest). After it, one
+// transposition area per wave.
+template <int MODE>
+constexpr size_t ragged_xchg_bytes()
 {
+    return MODE == kDirect ? wg_core_lds_bytes<double, 8>(16) + 2 * 16 * sizeof(double)
+                           : wg_core_lds_bytes<float, 16>(16) + 2 * 16 * sizeof(double);
+}
+template <int MODE>
+constexpr size_t ragged_lds_bytes()
+{
+    return ragged_xchg_bytes<MODE>() + (size_t)kRaggedWaves * kStageWaveBytes;
+}
+static_assert(ragged_lds_bytes<kDirect>() <= 160 * 1024, "LDS of the FP64 mode");
+static_assert(ragged_lds_bytes<kRefineWide>() <= 160 * 1024, "LDS of the refine mode");
+
+// Lane-blocked gather of one coefficient array. The system (segment of L
+// lanes, first row of the segment at lane - ll) starts at `base` and has n
+// unknowns; lane ll must end with unknowns [ll*M, ll*M+M) (`fill` past n).
+// Load q of lane ll reads unknown q*L+ll, so every load instruction of the
+// wave reads L consecutive doubles of each system (coalesced whatever the
+// alignment of `base`), and the LDS area transposes them into rows.
+template <int M>
+__device__ __forceinline__ void gather_rows(const double* __restrict__ X, int64_t base, int64_t n,
+                                            int L, int ll, int lane, bool live, double fill,
+                                            char* stage, double (&v)[M])
+{
+    if constexpr (M == 1) {
+        v[0] = (live && ll < n) ? X[base + ll] : fill;
+    } else {
+        using S = RStage<M>;
+        const int seg_row = lane - ll;
+        // The index arithmetic below is cheap; without this the compiler
+        // hoists every address of every gather out of the refinement loop
+        // and spills them.
+        asm volatile("" : "+v"(ll), "+v"(n));
 #pragma unroll
-    for (int s = 0; s < M; ++s) {
-        const int64_t j = j0 + s;
-        const bool row = live && j < n;
-        const double va = row && j > 0 ? A[base + j] : 0.0;
-        const double vb = row ? B[base + j] : 1.0;
-        const double vc = row && j < n - 1 ? C[base + j] : 0.0;
-        const double vd = row ? D[base + j] : 0.0;
-        const double xl = s > 0 ? x[s > 0 ? s - 1 : 0] : x_left;
-        const double xr = s < M - 1 ? x[s < M - 1 ? s + 1 : 0] : x_right;
-        double r = fma(-vb, x[s], vd);
-        r = fma(-va, xl, r);
-        r = fma(-vc, xr, r);
-        e[s] = Eq<float>{(float)va, (float)vb, (float)vc, row ? (float)r : 0.0f};
+        for (int h = 0; h < S::kRounds; ++h) {
+            // in batches of 4 loads, to bound the registers in flight
+#pragma unroll
+            for (int q0 = 0; q0 < S::kQ; q0 += 4) {
+                constexpr int B = S::kQ < 4 ? S::kQ : 4;
+                double t[B];
+#pragma unroll
+                for (int q = 0; q < B; ++q) {
+                    const int64_t j = (int64_t)(h * S::kQ + q0 + q) * L + ll;
+                    t[q] = (live && j < n) ? X[base + j] : fill;
+                }
+#pragma unroll
+                for (int q = 0; q < B; ++q) {
+                    const int j = (h * S::kQ + q0 + q) * L + ll;
+                    const int row = seg_row + j / M - h * S::kRows;
+                    *reinterpret_cast<double*>(stage + row * S::kRowBytes + (j % M) * 8) = t[q];
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (lane / S::kRows == h) {
+                const char* src = stage + (lane - h * S::kRows) * S::kRowBytes;
+#pragma unroll
+                for (int k = 0; k < M / 2; ++k) {
+                    const double2 w = *reinterpret_cast<const double2*>(src + 16 * k);
+                    v[2 * k] = w.x;
+                    v[2 * k + 1] = w.y;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
     }
 }
 
+// Inverse of gather_rows: the lane's M unknowns go back to unknowns
+// [ll*M, ll*M+M) of the system through the LDS area, written with
+// coalesced stores (nothing past n).
 template <int M>
+__device__ __forceinline__ void scatter_rows(double* __restrict__ X, int64_t base, int64_t n,
+                                             int L, int ll, int lane, bool live, char* stage,
+                                             const double (&v)[M])
+{
+    if constexpr (M == 1) {
+        if (live && ll < n) X[base + ll] = v[0];
+    } else {
+        using S = RStage<M>;
+        const int seg_row = lane - ll;
+        asm volatile("" : "+v"(ll), "+v"(n));
+#pragma unroll
+        for (int h = 0; h < S::kRounds; ++h) {
+            if (lane / S::kRows == h) {
+                char* dst = stage + (lane - h * S::kRows) * S::kRowBytes;
+#pragma unroll
+                for (int k = 0; k < M / 2; ++k)
+                    *reinterpret_cast<double2*>(dst + 16 * k) = double2{v[2 * k], v[2 * k + 1]};
+            }
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int q = 0; q < S::kQ; ++q) {
+                const int j = (h * S::kQ + q) * L + ll;
+                const int row = seg_row + j / M - h * S::kRows;
+                const double t =
+                    *reinterpret_cast<const double*>(stage + row * S::kRowBytes + (j % M) * 8);
+                if (live && j < n) X[base + j] = t;
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+}
+
+// FP32 operator + FP64 residual rows (as kern_refine.hip's build_rows) of
+// the lane's unknowns [j0, j0+M) of a system with n unknowns. `base`, `nb`
+// and `j0b` locate the lane's segment for the gathers: the segment's first
+// unknown in memory, the unknowns it holds, and the lane's first unknown
+// relative to it. Identity rows past n; a[first] and c[last] ignored.
+template <int M>
+__device__ __forceinline__ void ragged_rows(const RaggedArgs& g, const double (&x)[M],
+                                            double x_left, double x_right, int64_t base,
+                                            int64_t nb, int64_t j0, int64_t n, int L, int ll,
+                                            int lane, bool live, char* stage, Eq<float> (&e)[M])
+{
+    double v[M], r[M];
+    gather_rows<M>(g.d, base, nb, L, ll, lane, live, 0.0, stage, r);
+    gather_rows<M>(g.b, base, nb, L, ll, lane, live, 1.0, stage, v);
+#pragma unroll
+    for (int s = 0; s < M; ++s) {
+        r[s] = fma(-v[s], x[s], r[s]);
+        e[s].b = (float)v[s];
+    }
+    gather_rows<M>(g.a, base, nb, L, ll, lane, live, 0.0, stage, v);
+#pragma unroll
+    for (int s = 0; s < M; ++s) {
+        const double al = j0 + s > 0 ? v[s] : 0.0;
+        const double xl = s > 0 ? x[s > 0 ? s - 1 : 0] : x_left;
+        r[s] = fma(-al, xl, r[s]);
+        e[s].a = (float)al;
+    }
+    gather_rows<M>(g.c, base, nb, L, ll, lane, live, 0.0, stage, v);
+#pragma unroll
+    for (int s = 0; s < M; ++s) {
+        const double cr = j0 + s < n - 1 ? v[s] : 0.0;
+        const double xr = s < M - 1 ? x[s < M - 1 ? s + 1 : 0] : x_right;
+        r[s] = fma(-cr, xr, r[s]);
+        e[s].c = (float)cr;
+        e[s].d = (live && j0 + s < n) ? (float)r[s] : 0.0f;
+    }
+}
+
+// FP64 operator of the lane's unknowns [j0, j0+M) (arguments as
+// ragged_rows). Identity rows past n; a[first] and c[last] ignored.
+template <int M>
+__device__ __forceinline__ void direct_rows(const RaggedArgs& g, int64_t base, int64_t nb,
+                                            int64_t j0, int64_t n, int L, int ll, int lane,
+                                            bool live, char* stage, Eq<double> (&e)[M])
+{
+    double v[M];
+    gather_rows<M>(g.d, base, nb, L, ll, lane, live, 0.0, stage, v);
+#pragma unroll
+    for (int s = 0; s < M; ++s) e[s].d = v[s];
+    gather_rows<M>(g.b, base, nb, L, ll, lane, live, 1.0, stage, v);
+#pragma unroll
+    for (int s = 0; s < M; ++s) e[s].b = v[s];
+    gather_rows<M>(g.a, base, nb, L, ll, lane, live, 0.0, stage, v);
+#pragma unroll
+    for (int s = 0; s < M; ++s) e[s].a = j0 + s > 0 ? v[s] : 0.0;
+    gather_rows<M>(g.c, base, nb, L, ll, lane, live, 0.0, stage, v);
+#pragma unroll
+    for (int s = 0; s < M; ++s) e[s].c = j0 + s < n - 1 ? v[s] : 0.0;
+}
+
+template <bool DIRECT, int M>
 __device__ void wave_item(const RaggedArgs& g, const RaggedItem& it, const RaggedBucket& bk,
-                          int lane)
+                          int lane, char* stage)
 {
     const int L = bk.lanes;
     const int ll = lane & (L - 1);
@@ -81,6 +240,13 @@ __device__ void wave_item(const RaggedArgs& g, const RaggedItem& it, const Ragge
     }
     const int64_t j0 = (int64_t)ll * M;
     double x[M];
+    if constexpr (DIRECT) {
+        Eq<double> e[M];
+        direct_rows<M>(g, base, n, j0, n, L, ll, lane, live, stage, e);
+        wave_core<double, M, 1>(e, x, lane, ll, L, bk.rest);
+        scatter_rows<M>(g.x, base, n, L, ll, lane, live, stage, x);
+        return;
+    }
 #pragma unroll
     for (int s = 0; s < M; ++s) x[s] = 0.0;
     Eq<float> e[M];
@@ -88,28 +254,26 @@ __device__ void wave_item(const RaggedArgs& g, const RaggedItem& it, const Ragge
     for (int step = 0; step <= g.iters; ++step) {
         const double xl = lane_read_or_zero(x[M - 1], lane - 1, ll >= 1);
         const double xr = lane_read_or_zero(x[0], lane + 1, ll + 1 < L);
-        ragged_rows<M>(g.a, g.b, g.c, g.d, x, xl, xr, base, j0, n, live, e);
+        ragged_rows<M>(g, x, xl, xr, base, n, j0, n, L, ll, lane, live, stage, e);
         wave_core<float, M, 1>(e, xf, lane, ll, L, bk.rest);
 #pragma unroll
         for (int s = 0; s < M; ++s) x[s] += (double)xf[s];
     }
-    if (!live) return;
-#pragma unroll
-    for (int s = 0; s < M; ++s)
-        if (j0 + s < n) g.x[base + j0 + s] = x[s];
+    scatter_rows<M>(g.x, base, n, L, ll, lane, live, stage, x);
 }
 
 // All 16 waves of the workgroup call this (wg_core has barriers).
-template <int M>
+template <bool DIRECT, int M>
 __device__ void wg_item(const RaggedArgs& g, const RaggedItem& it, const RaggedBucket& bk,
-                        int wave, int lane, char* smem)
+                        int wave, int lane, char* smem, char* stage)
 {
+    using TC = typename std::conditional<DIRECT, double, float>::type;  // core type
     const int W = bk.lanes / kWave;
     const int grp = wave / W;
     const int w = wave % W;
-    const size_t grp_bytes = wg_core_lds_bytes<float, M>(W) + (size_t)2 * W * sizeof(double);
+    const size_t grp_bytes = wg_core_lds_bytes<TC, M>(W) + (size_t)2 * W * sizeof(double);
     char* xchg = smem + grp * grp_bytes;
-    double* edge = reinterpret_cast<double*>(xchg + wg_core_lds_bytes<float, M>(W));
+    double* edge = reinterpret_cast<double*>(xchg + wg_core_lds_bytes<TC, M>(W));
     const bool live = grp < it.count;
     int64_t base = 0, n = 2;
     if (live) {
@@ -117,8 +281,18 @@ __device__ void wg_item(const RaggedArgs& g, const RaggedItem& it, const RaggedB
         base = g.offsets[s];
         n = g.offsets[s + 1] - base;
     }
-    const int64_t j0 = (int64_t)(w * kWave + lane) * M;
+    // this wave's block of the system: unknowns [w*64*M, (w+1)*64*M)
+    const int64_t wb = (int64_t)w * kWave * M;
+    const int64_t j0 = wb + (int64_t)lane * M;
     double x[M];
+    if constexpr (DIRECT) {
+        Eq<double> e[M];
+        direct_rows<M>(g, base + wb, n - wb, j0, n, kWave, lane, lane, live, stage, e);
+        wg_core<double, M>(e, x, w, W, lane, bk.rest, xchg);
+        scatter_rows<M>(g.x, base + wb, n - wb, kWave, lane, lane, live, stage, x);
+        __syncthreads();  // exchange areas are reused by the next item
+        return;
+    }
 #pragma unroll
     for (int s = 0; s < M; ++s) x[s] = 0.0;
     Eq<float> e[M];
@@ -134,16 +308,13 @@ __device__ void wg_item(const RaggedArgs& g, const RaggedItem& it, const RaggedB
             xl = lane > 0 ? xl0 : (w > 0 ? edge[2 * w - 1] : 0.0);
             xr = lane < kWave - 1 ? xr0 : (w + 1 < W ? edge[2 * w + 2] : 0.0);
         }
-        ragged_rows<M>(g.a, g.b, g.c, g.d, x, xl, xr, base, j0, n, live, e);
+        ragged_rows<M>(g, x, xl, xr, base + wb, n - wb, j0, n, kWave, lane, lane, live, stage,
+                       e);
         wg_core<float, M>(e, xf, w, W, lane, bk.rest, xchg);
 #pragma unroll
         for (int s = 0; s < M; ++s) x[s] += (double)xf[s];
     }
-    if (live) {
-#pragma unroll
-        for (int s = 0; s < M; ++s)
-            if (j0 + s < n) g.x[base + j0 + s] = x[s];
-    }
+    scatter_rows<M>(g.x, base + wb, n - wb, kWave, lane, lane, live, stage, x);
     __syncthreads();  // exchange areas are reused by the next item
 }
 
@@ -154,16 +325,19 @@ __global__ void ragged_reset_kernel(int* counters)
     if (threadIdx.x < 2) counters[threadIdx.x] = 0;
 }
 
-// WIDE: the plan has 16384-unknown systems (16 unknowns per lane in the
-// workgroup items). That path needs more registers than the others, so it is
-// a separate instantiation and the common one keeps its register budget.
-template <bool WIDE>
+// Modes (RaggedMode): the wide refine form handles 16384-unknown systems
+// (16 unknowns per lane in the workgroup items); it needs more registers
+// than the others, so it is a separate instantiation and the common one
+// keeps its register budget.
+template <int MODE>
 __global__ __launch_bounds__(kRaggedThreads) void ragged_kernel(RaggedArgs g)
 {
-    __shared__ __align__(16) char smem[kRaggedLds];
+    constexpr bool DIRECT = MODE == kDirect;
+    __shared__ __align__(16) char smem[ragged_lds_bytes<MODE>()];
     __shared__ int s_item;
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x & (kWave - 1);
+    char* stage = smem + ragged_xchg_bytes<MODE>() + (size_t)wave * kStageWaveBytes;
 
     // phase 1: workgroup items (long systems), largest bucket first
     for (;;) {
@@ -174,13 +348,13 @@ __global__ __launch_bounds__(kRaggedThreads) void ragged_kernel(RaggedArgs g)
         if (item >= g.n_wg_items) break;
         const RaggedItem it = g.items[item];
         const RaggedBucket bk = g.buckets[it.bucket];
-        if constexpr (WIDE) {
+        if constexpr (MODE == kRefineWide) {
             if (bk.per_lane == 16) {
-                wg_item<16>(g, it, bk, wave, lane, smem);
+                wg_item<false, 16>(g, it, bk, wave, lane, smem, stage);
                 continue;
             }
         }
-        wg_item<8>(g, it, bk, wave, lane, smem);
+        wg_item<DIRECT, 8>(g, it, bk, wave, lane, smem, stage);
     }
     // phase 2: wave items, each wave on its own
     for (;;) {
@@ -191,10 +365,10 @@ __global__ __launch_bounds__(kRaggedThreads) void ragged_kernel(RaggedArgs g)
         const RaggedItem it = g.items[g.n_wg_items + item];
         const RaggedBucket bk = g.buckets[it.bucket];
         switch (bk.per_lane) {
-            case 1: wave_item<1>(g, it, bk, lane); break;
-            case 2: wave_item<2>(g, it, bk, lane); break;
-            case 4: wave_item<4>(g, it, bk, lane); break;
-            case 8: wave_item<8>(g, it, bk, lane); break;
+            case 1: wave_item<DIRECT, 1>(g, it, bk, lane, stage); break;
+            case 2: wave_item<DIRECT, 2>(g, it, bk, lane, stage); break;
+            case 4: wave_item<DIRECT, 4>(g, it, bk, lane, stage); break;
+            case 8: wave_item<DIRECT, 8>(g, it, bk, lane, stage); break;
             default: break;
         }
     }
@@ -205,12 +379,17 @@ hipError_t launch_ragged(const RaggedArgs& g, int grid, hipStream_t stream)
     hipLaunchKernelGGL(ragged_reset_kernel, dim3(1), dim3(64), 0, stream, g.counters);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (g.wide)
-        hipLaunchKernelGGL(ragged_kernel<true>, dim3((unsigned)grid), dim3(kRaggedThreads), 0,
+    if (g.direct) {
+        if (g.wide) return hipErrorInvalidValue;  // no 16384-unknown FP64 form
+        hipLaunchKernelGGL(ragged_kernel<kDirect>, dim3((unsigned)grid), dim3(kRaggedThreads), 0,
                            stream, g);
-    else
-        hipLaunchKernelGGL(ragged_kernel<false>, dim3((unsigned)grid), dim3(kRaggedThreads), 0,
+    } else if (g.wide) {
+        hipLaunchKernelGGL(ragged_kernel<kRefineWide>, dim3((unsigned)grid),
+                           dim3(kRaggedThreads), 0, stream, g);
+    } else {
+        hipLaunchKernelGGL(ragged_kernel<kRefine>, dim3((unsigned)grid), dim3(kRaggedThreads), 0,
                            stream, g);
+    }
     return hipGetLastError();
 }
 
@@ -218,7 +397,7 @@ hipError_t launch_ragged(const RaggedArgs& g, int grid, hipStream_t stream)
 int ragged_max_grid(int device)
 {
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ragged_kernel<false>,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ragged_kernel<kRefine>,
                                                      kRaggedThreads, 0) !=
         hipSuccess)
         per_cu = 1;

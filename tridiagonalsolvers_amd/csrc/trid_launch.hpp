@@ -125,6 +125,7 @@ struct RaggedArgs {
     int n_wave_items;
     int iters;
     int wide;                 // 1 if any 16384-unknown bucket is present
+    int direct;               // 1: one FP64 solve per system (no FP32 stage)
 };
 
 hipError_t launch_ragged(const RaggedArgs& g, int grid, hipStream_t stream);

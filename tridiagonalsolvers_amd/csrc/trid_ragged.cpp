@@ -36,7 +36,7 @@ struct trid_ragged_plan_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     const void* key[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    int key_iters = -1;
+    int key_iters = -1;  // iters + 16 * (FP64 mode)
 };
 
 namespace {
@@ -253,7 +253,10 @@ int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* 
                           const double* d, double* x, int iters, int flags, void* stream)
 {
     if (!plan || !a || !b || !c || !d || !x || iters < 0 || iters > 8) return TRID_EINVAL;
-    if (flags & ~TRID_RAGGED_NO_GRAPH) return TRID_EINVAL;
+    if (flags & ~(TRID_RAGGED_NO_GRAPH | TRID_RAGGED_FP64)) return TRID_EINVAL;
+    const int direct = (flags & TRID_RAGGED_FP64) ? 1 : 0;
+    if (direct && iters != 0) return TRID_EINVAL;
+    if (direct && plan->wide) return TRID_ENOTSUP;
     trid::RaggedArgs g{};
     g.a = a;
     g.b = b;
@@ -269,13 +272,15 @@ int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* 
     g.n_wave_items = plan->n_wave_items;
     g.iters = iters;
     g.wide = plan->wide;
+    g.direct = direct;
     hipStream_t s = static_cast<hipStream_t>(stream);
     if ((flags & TRID_RAGGED_NO_GRAPH) || s == nullptr) {
         hipError_t e = trid::launch_ragged(g, plan->grid, s);
         return e == hipSuccess ? TRID_OK : TRID_EHIP;
     }
     const void* key[5] = {a, b, c, d, x};
-    const bool same = plan->exec && plan->key_iters == iters &&
+    const int key_mode = iters + 16 * direct;
+    const bool same = plan->exec && plan->key_iters == key_mode &&
                       std::memcmp(key, plan->key, sizeof(key)) == 0;
     if (!same) {
         if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
@@ -297,7 +302,7 @@ int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* 
             return TRID_EHIP;
         }
         std::memcpy(plan->key, key, sizeof(key));
-        plan->key_iters = iters;
+        plan->key_iters = key_mode;
     }
     return hipGraphLaunch(plan->exec, s) == hipSuccess ? TRID_OK : TRID_EHIP;
 }

@@ -18,6 +18,7 @@ from .solvers import _ptr, _stream_ptr
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_TABLE = os.path.join(HERE, "tune", "ragged_f64_refine.json")
 NO_GRAPH = 1
+FP64 = 2
 
 
 def log_uniform_sizes(num_systems: int, lo: int, hi: int, seed: int) -> np.ndarray:
@@ -73,10 +74,15 @@ class RaggedPlan:
                           for i in range(inf.num_buckets)]
         return out
 
-    def solve(self, a, b, c, d, x, iters: int = 2, stream=None, graph: bool = True) -> None:
+    def solve(self, a, b, c, d, x, iters: int | None = None, stream=None, graph: bool = True,
+              fp64: bool = False) -> None:
+        """FP32 solve + `iters` FP64 refinements (default 2), or with fp64=True
+        one FP64 solve (iters must then be 0 or None; systems up to 8192)."""
+        if iters is None:
+            iters = 0 if fp64 else 2
+        flags = (0 if graph else NO_GRAPH) | (FP64 if fp64 else 0)
         st = lib().trid_ragged_solve_f64(self._h, *(ctypes.c_void_p(_ptr(v)) for v in (a, b, c, d, x)),
-                                         int(iters), 0 if graph else NO_GRAPH,
-                                         ctypes.c_void_p(_stream_ptr(stream)))
+                                         int(iters), flags, ctypes.c_void_p(_stream_ptr(stream)))
         check(st, "trid_ragged_solve_f64")
 
     def close(self):
