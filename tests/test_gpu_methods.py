@@ -89,18 +89,21 @@ def test_pcrgauss_every_depth(oracle, trid, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_pcrgauss_two_pass_form(oracle, trid, dtype):
+@pytest.mark.parametrize("n", [4096, 5000, 1100])
+def test_pcrgauss_two_pass_form(oracle, trid, dtype, n):
+    # n = 5000, 1100: subsystems of unequal length (1100 with G = 512 gives
+    # subsystems of 3 and 2 rows; both halves of the two-ended elimination)
     import torch
-    n, N = 4096, 6
+    N = 6
     a, b, c, d = oracle.generate("dominant", 77, n, N, dtype)
     dev = [torch.from_numpy(v).cuda() for v in (a, b, c, d)]
     h = trid.Handle()
-    for G in (1, 2, 16, 64, 256):
+    for G in (1, 2, 16, 64, 256, 512):
         for flags in (0, 1):
             x = torch.full_like(dev[0], float("nan"))
             h.solve("pcrgauss", *dev, x, n, N, trid.make_opts(gaussian=G, flags=flags))
             torch.cuda.synchronize()
-            print(f"G={G} two_pass={flags}", flush=True)
+            print(f"n={n} G={G} two_pass={flags}", flush=True)
             check(oracle, x.cpu().numpy(), a, b, c, d, n, dtype)
     h.close()
 
@@ -113,7 +116,7 @@ def test_pcrgauss_million_unknowns(oracle, trid):
     trid.generate(0, 5, *dev, n, N)
     a, b, c, d = oracle.generate("dominant", 5, n, N, np.float32)
     h = trid.Handle()
-    for G in (4, 64, 256):
+    for G in (4, 64, 256, 1024):
         x = torch.full_like(dev[0], float("nan"))
         h.solve("pcrgauss", *dev, x, n, N, trid.make_opts(gaussian=G))
         torch.cuda.synchronize()

@@ -16,10 +16,11 @@
 // * two passes through HBM for long systems (the reference's large path,
 //   pcrGauss_app.cu:102-162, runs one global PCR level per launch and keeps
 //   one 1024-thread block per system): pass 1 runs all log2(G) levels in one
-//   launch on tiles with a halo of G-1 equations on each side, held in LDS;
-//   pass 2 runs Thomas with one lane per subsystem. Lanes r, r+1, ... read
-//   consecutive addresses at every step, so pass 2 is coalesced; its
-//   forward-sweep factors c', d' overwrite the pass-1 workspace in place.
+//   launch on tiles with a halo of G-1 equations on each side, held in
+//   registers and exchanged through LDS; pass 2 runs Thomas with one lane per
+//   subsystem. Lanes r, r+1, ... read consecutive addresses at every step,
+//   so pass 2 is coalesced; its forward-sweep factors c', d' overwrite the
+//   pass-1 workspace in place.
 #include "trid_device.hpp"
 #include "trid_launch.hpp"
 
@@ -130,10 +131,31 @@ __global__ __launch_bounds__(1024) void pcr_thomas_lds_kernel(const T* __restric
 // two-pass form
 // ---------------------------------------------------------------------------
 
+// Tiles of the global PCR pass: at least TRID_PCRG_TILE_MIN equations and
+// TRID_PCRG_TILE_PER_G * G (so the halo of 2(G-1) stays a small fraction),
+// halved while the window does not fit the registers (12 equations per
+// thread FP32, 10 FP64) or LDS; TRID_PCRG_THREADS threads per workgroup.
+// Measured on 64 x 2^20 FP32 (depth 2..10): 2048 / 8 G / 512 threads is
+// the best of the shapes tried (tiles of 2048..8192 equations, 256..1024
+// threads).
+#ifndef TRID_PCRG_TILE_MIN
+#define TRID_PCRG_TILE_MIN 2048
+#endif
+#ifndef TRID_PCRG_TILE_PER_G
+#define TRID_PCRG_TILE_PER_G 8
+#endif
+#ifndef TRID_PCRG_THREADS
+#define TRID_PCRG_THREADS 512
+#endif
+
 // Pass 1: log2(G) PCR levels on tiles of `tile` equations with a halo of
-// G-1 on each side. Output (a', b', c', d') goes to the workspace arrays.
+// G-1 on each side. Each thread keeps its equations (p = tid + j*nt of the
+// window) in registers; a level publishes them to LDS as one record per
+// equation, and reads the two neighbours back (48 bytes of LDS traffic per
+// equation and level for FP32). Output (a', b', c', d') goes to the
+// workspace arrays.
 template <typename T, int KPT>
-__global__ __launch_bounds__(512) void pcr_global_levels_kernel(
+__global__ __launch_bounds__(TRID_PCRG_THREADS) void pcr_global_levels_kernel(
     const T* __restrict__ A, const T* __restrict__ B, const T* __restrict__ C,
     const T* __restrict__ D, T* __restrict__ A2, T* __restrict__ B2, T* __restrict__ C2,
     T* __restrict__ D2, int64_t n, int G, int tile)
@@ -141,155 +163,189 @@ __global__ __launch_bounds__(512) void pcr_global_levels_kernel(
     extern __shared__ __align__(16) char trid_smem[];
     const int H = G - 1;
     const int len = tile + 2 * H;
-    T* sa = reinterpret_cast<T*>(trid_smem);
-    T* sb = sa + len;
-    T* sc = sb + len;
-    T* sd = sc + len;
+    Eq<T>* se = reinterpret_cast<Eq<T>*>(trid_smem);
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
     const int64_t sys = blockIdx.y;
     const int64_t base = sys * n;
     const int64_t t0 = (int64_t)blockIdx.x * tile;
 
-    for (int p = tid; p < len; p += nt) {
+    Eq<T> own[KPT];
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) {
+        const int p = tid + j * nt;
         const int64_t g = t0 - H + p;
-        Eq<T> e = identity_eq<T>();
-        if (g >= 0 && g < n) e = Eq<T>{A[base + g], B[base + g], C[base + g], D[base + g]};
-        if (g == 0) e.a = T(0);
-        if (g == n - 1) e.c = T(0);
-        sa[p] = e.a;
-        sb[p] = e.b;
-        sc[p] = e.c;
-        sd[p] = e.d;
+        own[j] = identity_eq<T>();
+        if (p < len && g >= 0 && g < n) {
+            own[j] = Eq<T>{A[base + g], B[base + g], C[base + g], D[base + g]};
+            if (g == 0) own[j].a = T(0);
+            if (g == n - 1) own[j].c = T(0);
+        }
     }
-    __syncthreads();
     for (int delta = 1; delta < G; delta <<= 1) {
-        Eq<T> nw[KPT];
 #pragma unroll
         for (int j = 0; j < KPT; ++j) {
             const int p = tid + j * nt;
-            if (p < len) {
-                const Eq<T> own{sa[p], sb[p], sc[p], sd[p]};
-                // outside the loaded window the value is never used by the
-                // tile (the window shrinks by delta per level, G-1 in total)
-                const Eq<T> lft = p - delta >= 0
-                                      ? Eq<T>{sa[p - delta], sb[p - delta], sc[p - delta], sd[p - delta]}
-                                      : identity_eq<T>();
-                const Eq<T> rgt = p + delta < len
-                                      ? Eq<T>{sa[p + delta], sb[p + delta], sc[p + delta], sd[p + delta]}
-                                      : identity_eq<T>();
-                nw[j] = reduce_eq(own, lft, rgt);
-            }
+            if (p < len) se[p] = own[j];
         }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < KPT; ++j) {
             const int p = tid + j * nt;
-            if (p < len) {
-                // rows outside the system stay identity rows
-                const int64_t g = t0 - H + p;
-                if (g >= 0 && g < n) {
-                    sa[p] = nw[j].a;
-                    sb[p] = nw[j].b;
-                    sc[p] = nw[j].c;
-                    sd[p] = nw[j].d;
-                }
+            const int64_t g = t0 - H + p;
+            // rows outside the system stay identity rows; outside the loaded
+            // window the value is never used by the tile (the window shrinks
+            // by delta per level, G-1 in total)
+            if (p < len && g >= 0 && g < n) {
+                const Eq<T> lft = p - delta >= 0 ? se[p - delta] : identity_eq<T>();
+                const Eq<T> rgt = p + delta < len ? se[p + delta] : identity_eq<T>();
+                own[j] = reduce_eq(own[j], lft, rgt);
             }
         }
         __syncthreads();
     }
-    for (int p = tid; p < tile; p += nt) {
-        const int64_t g = t0 + p;
-        if (g < n) {
-            A2[base + g] = sa[H + p];
-            B2[base + g] = sb[H + p];
-            C2[base + g] = sc[H + p];
-            D2[base + g] = sd[H + p];
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) {
+        const int p = tid + j * nt;
+        const int64_t g = t0 - H + p;
+        if (p >= H && p < H + tile && g < n) {
+            A2[base + g] = own[j].a;
+            B2[base + g] = own[j].b;
+            C2[base + g] = own[j].c;
+            D2[base + g] = own[j].d;
         }
     }
 }
 
-// Pass 2: Thomas on subsystem r of system s, one lane each. The forward
-// factors overwrite C2/D2; x goes to X.
-template <typename T>
-__global__ __launch_bounds__(256) void thomas_strided_kernel(const T* __restrict__ A2,
-                                                             const T* __restrict__ B2,
-                                                             T* __restrict__ C2,
-                                                             T* __restrict__ D2,
-                                                             T* __restrict__ X, int64_t n,
-                                                             int64_t N, int G)
+// Pass 2: Gaussian elimination on subsystem r of system s from both ends
+// at once (a twisted factorization): lane l < 32 of a wave eliminates rows
+// 0 .. h-1 downwards (the Thomas forward sweep), lane l + 32 eliminates rows
+// m-1 .. h upwards, the two meet in a 2x2 system for x_{h-1}, x_h, and
+// each lane back-substitutes its half. This halves the serial chain of
+// Thomas (n/G steps) without changing the elimination it performs on each
+// half. Half-waves r, r+1, ... read consecutive addresses at every step.
+// The chains are long and few, so the kernel runs one wave per workgroup
+// (spreading them over every CU) and keeps the next U steps' loads in
+// flight while it computes the current U (double buffering). Loads and
+// stores share one 6-bit outstanding-operation counter, so waiting for a
+// batch while the next batch's 4U loads and this batch's 2U stores are in
+// flight needs 6U <= 63: U = 8. The elimination factors overwrite C2/D2; x
+// goes to X.
+//
+// In elimination order i = 0, 1, ... a lane visits row k_i (k_i = i on the
+// top half, m-1-i on the bottom half) with p the coupling to the row
+// already eliminated (a on top, c below) and q the coupling to the next
+// (c on top, a below):
+//   den = b - p q'_{i-1},  q'_i = q / den,  d'_i = (d - p d'_{i-1}) / den
+// (p ignored for i = 0), so x_{k_i} = d'_i - q'_i x_{k_{i+1}}. At the meeting
+// point  x_{h-1} + c' x_h = d'_top  and  x_h + a' x_{h-1} = d'_bottom.
+template <typename T, int U>
+__global__ __launch_bounds__(64) void thomas_strided_kernel(const T* __restrict__ A2,
+                                                            const T* __restrict__ B2,
+                                                            T* __restrict__ C2,
+                                                            T* __restrict__ D2,
+                                                            T* __restrict__ X, int64_t n,
+                                                            int64_t N, int G)
 {
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= N * G) return;
-    const int64_t s = gid / G;
-    const int r = (int)(gid % G);
-    const int64_t base = s * n;
-    if (r >= n) return;
-    const int64_t m = (n - r + G - 1) / G;  // elements r, r+G, ... below n
-    constexpr int U = 8;                      // loads issued ahead
-    int64_t i = base + r;
-    T cp = tdiv(C2[i], B2[i]);
-    T dp = tdiv(D2[i], B2[i]);
-    C2[i] = cp;
-    D2[i] = dp;
-    int64_t j = 1;
-    for (; j + U <= m; j += U) {
-        T ta[U], tb[U], tc[U], td[U];
+    const int lane = threadIdx.x;
+    const bool top = lane < kWave / 2;
+    const int64_t chain = (int64_t)blockIdx.x * (kWave / 2) + (lane & (kWave / 2 - 1));
+    const int64_t s = chain / G;
+    const int r = (int)(chain % G);
+    const bool live = chain < N * G && r < n;
+    const int64_t m = live ? (n - r + G - 1) / G : 0;  // rows r, r+G, ... below n
+    const int64_t h = (m + 1) / 2;                     // rows of the top half
+    const int64_t len = top ? h : m - h;
+    // row k_i = first + i * step
+    const int64_t first = top ? 0 : m - 1;
+    const int64_t step = top ? (int64_t)G : -(int64_t)G;
+    const int64_t o0 = s * n + r + first * G;
+    const T* pp = (top ? A2 : C2) + o0;  // coupling to the eliminated side
+    const T* pb = B2 + o0;
+    const T* pq = (top ? C2 : A2) + o0;  // coupling to the next row
+    T* wq = C2 + o0;
+    T* wd = D2 + o0;
+    T* px = X + o0;
+
+    T qv = T(0), dv = T(0);
+    T np[U], nb[U], nq[U], nd[U];
+    auto fetch_elim = [&](int64_t i0) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int64_t q = base + r + (j + u) * G;
-            ta[u] = A2[q];
-            tb[u] = B2[q];
-            tc[u] = C2[q];
-            td[u] = D2[q];
+            const int64_t i = i0 + u;
+            if (i < len) {
+                np[u] = pp[i * step];
+                nb[u] = pb[i * step];
+                nq[u] = pq[i * step];
+                nd[u] = wd[i * step];
+            }
         }
+    };
+    fetch_elim(0);
+    for (int64_t i0 = 0; i0 < len; i0 += U) {
+        T tp[U], tb[U], tq[U], td[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const T mm = tb[u] - ta[u] * cp;
-            cp = tdiv(tc[u], mm);
-            dp = tdiv(td[u] - dp * ta[u], mm);
-            tc[u] = cp;
-            td[u] = dp;
+            tp[u] = np[u];
+            tb[u] = nb[u];
+            tq[u] = nq[u];
+            td[u] = nd[u];
         }
+        fetch_elim(i0 + U);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int64_t q = base + r + (j + u) * G;
-            C2[q] = tc[u];
-            D2[q] = td[u];
+            const int64_t i = i0 + u;
+            if (i < len) {
+                const T pk = i > 0 ? tp[u] : T(0);
+                const T den = tb[u] - pk * qv;
+                qv = tdiv(tq[u], den);
+                dv = tdiv(td[u] - dv * pk, den);
+                wq[i * step] = qv;
+                wd[i * step] = dv;
+            }
         }
     }
-    for (; j < m; ++j) {
-        const int64_t q = base + r + j * G;
-        const T ai = A2[q];
-        const T mm = B2[q] - ai * cp;
-        cp = tdiv(C2[q], mm);
-        dp = tdiv(D2[q] - dp * ai, mm);
-        C2[q] = cp;
-        D2[q] = dp;
+    // meeting point: lane l and l + 32 hold the two normalized rows
+    const int partner = top ? lane + kWave / 2 : lane - kWave / 2;
+    const T oq = lane_read(qv, partner);
+    const T od = lane_read(dv, partner);
+    T xv;
+    if (top) {
+        // x_{h-1} + qv x_h = dv;  x_h + oq x_{h-1} = od (absent when m == 1)
+        xv = m - h > 0 ? tdiv(dv - qv * od, T(1) - qv * oq) : dv;
+    } else {
+        const T xo = tdiv(od - oq * dv, T(1) - oq * qv);  // x_{h-1}
+        xv = dv - qv * xo;                                // x_h
     }
-    T xv = dp;
-    X[base + r + (m - 1) * G] = xv;
-    j = m - 2;
-    for (; j - U + 1 >= 0; j -= U) {
-        T tc[U], td[U];
+    if (len > 0) px[(len - 1) * step] = xv;
+    // back-substitution, i = len-2 .. 0
+    auto fetch_back = [&](int64_t i0) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int64_t q = base + r + (j - u) * G;
-            tc[u] = C2[q];
-            td[u] = D2[q];
+            const int64_t i = i0 - u;
+            if (i >= 0) {
+                nq[u] = wq[i * step];
+                nd[u] = wd[i * step];
+            }
         }
+    };
+    fetch_back(len - 2);
+    for (int64_t i0 = len - 2; i0 >= 0; i0 -= U) {
+        T tq[U], td[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            xv = td[u] - tc[u] * xv;
-            td[u] = xv;
+            tq[u] = nq[u];
+            td[u] = nd[u];
         }
+        fetch_back(i0 - U);
 #pragma unroll
-        for (int u = 0; u < U; ++u) X[base + r + (j - u) * G] = td[u];
-    }
-    for (; j >= 0; --j) {
-        const int64_t q = base + r + j * G;
-        xv = D2[q] - C2[q] * xv;
-        X[q] = xv;
+        for (int u = 0; u < U; ++u) {
+            const int64_t i = i0 - u;
+            if (i >= 0) {
+                xv = td[u] - tq[u] * xv;
+                px[i * step] = xv;
+            }
+        }
     }
 }
 
@@ -329,13 +385,17 @@ hipError_t launch_pcr_thomas(const PcrThomasArgs<T>& g)
     T* C2 = B2 + total;
     T* D2 = C2 + total;
     const int G = g.G;
-    int tile = 4096;
-    if (tile < 4 * G) tile = 4 * G;
+    const int nt = TRID_PCRG_THREADS;
+    const int max_kpt = sizeof(T) == 4 ? 12 : 10;
+    int tile = TRID_PCRG_TILE_MIN;
+    if (tile < TRID_PCRG_TILE_PER_G * G) tile = TRID_PCRG_TILE_PER_G * G;
+    while (tile > G && (tile + 2 * (G - 1) > max_kpt * nt ||
+                        (size_t)(tile + 2 * (G - 1)) * sizeof(Eq<T>) > 160 * 1024))
+        tile /= 2;
     const int len = tile + 2 * (G - 1);
-    const int nt = 512;
     const int kpt = (len + nt - 1) / nt;
-    const size_t lds = (size_t)4 * len * sizeof(T);
-    if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
+    const size_t lds = (size_t)len * sizeof(Eq<T>);
+    if (lds > 160 * 1024 || kpt > max_kpt) return hipErrorInvalidConfiguration;
     const int64_t tiles = (g.n + tile - 1) / tile;
     if (tiles > 0x7fffffffLL || g.N > 65535) return hipErrorInvalidConfiguration;
     {
@@ -351,14 +411,18 @@ hipError_t launch_pcr_thomas(const PcrThomasArgs<T>& g)
         };
         // G = 1 runs no PCR level: pass 1 only copies (and zeroes a[0], c[n-1])
         hipError_t e;
-        if (kpt <= 10) e = go(pcr_global_levels_kernel<T, 10>);
+        if (kpt <= 4) e = go(pcr_global_levels_kernel<T, 4>);
+        else if (kpt <= 6) e = go(pcr_global_levels_kernel<T, 6>);
+        else if (kpt <= 8) e = go(pcr_global_levels_kernel<T, 8>);
+        else if (kpt <= 10) e = go(pcr_global_levels_kernel<T, 10>);
         else if (kpt <= 12 && sizeof(T) == 4) e = go(pcr_global_levels_kernel<T, 12>);
         else return hipErrorInvalidValue;
         if (e != hipSuccess) return e;
     }
-    const int64_t lanes = g.N * (int64_t)G;
-    const int64_t blocks = (lanes + 255) / 256;
-    hipLaunchKernelGGL((thomas_strided_kernel<T>), dim3((unsigned)blocks), dim3(256), 0,
+    const int64_t chains = g.N * (int64_t)G;  // two lanes each
+    const int64_t blocks = (chains + kWave / 2 - 1) / (kWave / 2);
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL((thomas_strided_kernel<T, 8>), dim3((unsigned)blocks), dim3(kWave), 0,
                        g.stream, A2, B2, C2, D2, g.x, g.n, g.N, G);
     return hipGetLastError();
 }
