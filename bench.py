@@ -54,6 +54,12 @@ def parse():
     p.add_argument("--two-pass", action="store_true", help="pcrgauss: force the two-pass form")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--layout", default="system", choices=["system", "interleaved"],
+                   help="uniform workload: system-major arrays (the reference layout) or "
+                        "interleaved (row i of system j at i*N + j, rocSPARSE's "
+                        "gtsv_interleaved_batch layout)")
+    p.add_argument("--no-vendor", action="store_true",
+                   help="skip the rocSPARSE comparator and the device-copy ceiling")
     p.add_argument("--copy-probe", action="store_true",
                    help="after timing, copy the four input arrays once with a plain "
                         "device copy (known byte count) to calibrate HBM counters")
@@ -69,6 +75,192 @@ def traffic_from_profiles(workload_key: str):
             return json.load(f).get(workload_key, {}).get("bytes_per_launch")
     except (OSError, ValueError):
         return None
+
+
+def vendor_baseline(a, b, c, d, x_ours, n, N, dtype, stream, reps=20):
+    """rocSPARSE's gtsv_no_pivot_strided_batch (the vendor library's batched
+    no-pivot solver over the same system-major layout) on the same inputs,
+    timed with HIP events around the library call only (it overwrites its
+    right-hand side, so d is copied into the output before every call,
+    outside the timed interval); also the largest relative difference from
+    this library's result."""
+    import ctypes
+
+    import torch
+
+    try:
+        rs = ctypes.CDLL("librocsparse.so")
+    except OSError as e:
+        return {"error": f"librocsparse.so not loadable: {e}"}
+    pre = "s" if dtype == "f32" else "d"
+    vp = ctypes.c_void_p
+    hdl = vp()
+    if rs.rocsparse_create_handle(ctypes.byref(hdl)) != 0:
+        return {"error": "rocsparse_create_handle failed"}
+    rs.rocsparse_set_stream(hdl, vp(stream.cuda_stream))
+    xv = torch.empty_like(d)
+    args_ = (hdl, ctypes.c_int(n), vp(a.data_ptr()), vp(b.data_ptr()), vp(c.data_ptr()),
+             vp(xv.data_ptr()), ctypes.c_int(N), ctypes.c_int(n))
+    size = ctypes.c_size_t(0)
+    st = getattr(rs, f"rocsparse_{pre}gtsv_no_pivot_strided_batch_buffer_size")(
+        *args_, ctypes.byref(size))
+    if st != 0:
+        rs.rocsparse_destroy_handle(hdl)
+        return {"error": f"buffer_size status {st}"}
+    buf = torch.empty(max(1, size.value), dtype=torch.uint8, device=d.device)
+    solve = getattr(rs, f"rocsparse_{pre}gtsv_no_pivot_strided_batch")
+    times = []
+    for i in range(reps + 2):
+        xv.copy_(d)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        st = solve(*args_, vp(buf.data_ptr()))
+        e1.record(stream)
+        if st != 0:
+            rs.rocsparse_destroy_handle(hdl)
+            return {"error": f"solve status {st}"}
+        torch.cuda.synchronize()
+        if i >= 2:
+            times.append(e0.elapsed_time(e1))
+    rs.rocsparse_destroy_handle(hdl)
+    ms = sum(times) / len(times)
+    diff = ((xv - x_ours).abs().view(N, n).amax(dim=1) /
+            x_ours.abs().view(N, n).amax(dim=1)).max().item()
+    esize = 4 if dtype == "f32" else 8
+    return {"library": f"rocSPARSE rocsparse_{pre}gtsv_no_pivot_strided_batch",
+            "kernel_ms": ms, "unknowns_per_s": n * N / (ms * 1e-3),
+            "frac_of_peak": 5 * n * N * esize / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            "max_rel_diff_from_ours": diff}
+
+
+def bench_interleaved(args):
+    """Uniform batch in the interleaved layout: the seeded system-major batch
+    transposed on the device, solved by trid_solve_interleaved, and (unless
+    --no-vendor) by rocSPARSE's gtsv_interleaved_batch (Thomas and the
+    default QR algorithm) on copies of the same arrays."""
+    import ctypes
+
+    import torch
+
+    import tridiagonalsolvers_amd as T
+
+    n, N = args.n, args.systems
+    tdt = torch.float32 if args.dtype == "f32" else torch.float64
+    esize = 4 if args.dtype == "f32" else 8
+    stream = torch.cuda.current_stream()
+    sm = [torch.empty(n * N, dtype=tdt, device="cuda") for _ in range(4)]
+    T.generate(0, 1, *sm, n, N, stream=stream)
+    a, b, c, d = (v.view(N, n).t().contiguous().view(-1) for v in sm)
+    x = torch.empty_like(a)
+    h = T.Handle()
+
+    def step():
+        h.solve_interleaved(a, b, c, d, x, n, N, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for e0, e1 in evs:
+        e0.record(stream)
+        step()
+        e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = [e0.elapsed_time(e1) for e0, e1 in evs]
+    avg = sum(kms) / len(kms)
+    alg = 5 * n * N * esize
+    out = {"metric": METRIC, "value": n * N * args.steps / wall, "unit": "unknowns/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": args.dtype,
+           "data": "synthetic (seeded device generator, transposed to the interleaved layout)",
+           "config": {"workload": f"interleaved {args.dtype} {N} systems x {n} unknowns",
+                      "systems": N, "system_size": n},
+           "kernel_ms": {"avg": avg, "min": min(kms)},
+           "roofline": {"bound": "hbm", "achieved": alg / (avg * 1e-3) / 1e9,
+                        "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": alg / (avg * 1e-3) / 1e9 / PEAK_HBM_GBS, "traffic": None,
+                        "note": "the two-ended elimination moves 9 x T bytes per unknown "
+                                "(reads a, b, c, d; writes and re-reads c', d'; writes x)"}}
+    if not args.no_vendor:
+        rs = ctypes.CDLL("librocsparse.so")
+        vp = ctypes.c_void_p
+        pre = "s" if args.dtype == "f32" else "d"
+        hdl = vp()
+        rs.rocsparse_create_handle(ctypes.byref(hdl))
+        rs.rocsparse_set_stream(hdl, vp(stream.cuda_stream))
+        vend = {}
+        for alg_id, name in ((1, "thomas"), (0, "default (QR)")):
+            cp = [v.clone() for v in (a, b, c, d)]
+            size = ctypes.c_size_t(0)
+            st = getattr(rs, f"rocsparse_{pre}gtsv_interleaved_batch_buffer_size")(
+                hdl, ctypes.c_int(alg_id), ctypes.c_int(n), *(vp(v.data_ptr()) for v in cp),
+                ctypes.c_int(N), ctypes.c_int(N), ctypes.byref(size))
+            if st != 0:
+                vend[name] = {"error": f"buffer_size status {st}"}
+                continue
+            buf = torch.empty(max(1, size.value), dtype=torch.uint8, device="cuda")
+            times = []
+            for i in range(7):
+                for dst, src in zip(cp, (a, b, c, d)):
+                    dst.copy_(src)
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                st = getattr(rs, f"rocsparse_{pre}gtsv_interleaved_batch")(
+                    hdl, ctypes.c_int(alg_id), ctypes.c_int(n), *(vp(v.data_ptr()) for v in cp),
+                    ctypes.c_int(N), ctypes.c_int(N), vp(buf.data_ptr()))
+                e1.record(stream)
+                torch.cuda.synchronize()
+                if st != 0:
+                    break
+                if i >= 2:
+                    times.append(e0.elapsed_time(e1))
+            if st != 0:
+                vend[name] = {"error": f"solve status {st}"}
+                continue
+            ms = sum(times) / len(times)
+            xv = cp[3]
+            diff = ((xv - x).abs().view(n, N).amax(dim=0) /
+                    x.abs().view(n, N).amax(dim=0)).max().item()
+            vend[name] = {"kernel_ms": ms, "unknowns_per_s": n * N / (ms * 1e-3),
+                          "max_rel_diff_from_ours": diff}
+        rs.rocsparse_destroy_handle(hdl)
+        out["vendor_baseline"] = {"library": f"rocSPARSE rocsparse_{pre}gtsv_interleaved_batch",
+                                  **vend}
+    if not args.no_cpu:
+        xs = x.view(n, N).t().contiguous().view(-1)
+        out["cpu_baseline"] = cpu_baseline(args, xs.cpu().numpy(), n, N)
+    h.close()
+    print(json.dumps(out), flush=True)
+
+
+def copy_ceiling(a, b, c, d, stream, reps=10):
+    """Bandwidth of a plain device copy of the four input arrays (torch's copy
+    kernel): read + write bytes over HIP-event time, the achievable HBM rate
+    to compare roofline fractions against."""
+    import torch
+
+    dst = torch.empty_like(a)
+    for _ in range(2):
+        dst.copy_(a)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        for src in (a, b, c, d):
+            dst.copy_(src)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    moved = 2 * 4 * a.numel() * a.element_size()
+    return {"gbs": moved / (ms * 1e-3) / 1e9, "frac_of_peak": moved / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            "bytes": moved, "ms": ms}
 
 
 def cpu_baseline(args, x_gpu_np, n, N):
@@ -481,6 +673,9 @@ def main():
     if args.workload == "gauss-sweep":
         bench_gauss_sweep(args)
         return
+    if args.layout == "interleaved":
+        bench_interleaved(args)
+        return
     import torch
 
     import tridiagonalsolvers_amd as T
@@ -549,6 +744,9 @@ def main():
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": traffic},
     }
+    if not args.no_vendor:
+        out["vendor_baseline"] = vendor_baseline(a, b, c, d, x, n, N, args.dtype, stream)
+        out["copy_ceiling"] = copy_ceiling(a, b, c, d, stream)
     if args.copy_probe:
         # 4 * n * N * esize bytes read and written by a plain copy kernel
         dst = torch.empty_like(a)

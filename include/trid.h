@@ -176,6 +176,23 @@ int trid_solve_refined_f64(trid_handle handle, int method,
                            const trid_opts* opts, void* stream);
 
 /*
+ * Interleaved batch: row i of system j at i * batchStride + j (batchStride >=
+ * numSystems), the layout of rocSPARSE's gtsv_interleaved_batch. No
+ * counterpart in the reference (system-major only). Each system is solved by
+ * Gaussian elimination from both ends at once (two lanes per system, every
+ * access coalesced across systems); the a of row 0 and the c of the last
+ * row are ignored; x may be d (in place) but no other input. Uses the
+ * handle's workspace (systemSize * batchStride elements), so calls on one
+ * handle must not overlap on different streams.
+ */
+int trid_solve_interleaved_f32(trid_handle handle, const float* a, const float* b,
+                               const float* c, const float* d, float* x, int64_t systemSize,
+                               int64_t numSystems, int64_t batchStride, void* stream);
+int trid_solve_interleaved_f64(trid_handle handle, const double* a, const double* b,
+                               const double* c, const double* d, double* x, int64_t systemSize,
+                               int64_t numSystems, int64_t batchStride, void* stream);
+
+/*
  * Relative residual of every system, in FP64 on the device:
  * relres[s] = max_i |(d - T x)_i| / max_i |d_i| over system s =
  * [offsets[s], offsets[s+1]) (device arrays; a[first], c[last] ignored).

@@ -70,6 +70,15 @@ def test_device_api_rejects_bad_arguments_without_a_device(trid):
         h.solve("crpcr", 16, 16, 16, 16, 16, 64, 1, opts=trid.make_opts(flags=1))
     with pytest.raises(trid.TridError):
         h.solve(42, 16, 16, 16, 16, 16, 64, 1)
+    # interleaved batches: stride below the system count, one row, x on a/b/c
+    with pytest.raises(trid.TridError):
+        h.solve_interleaved(16, 32, 48, 64, 80, 8, 10, 9)
+    with pytest.raises(trid.TridError):
+        h.solve_interleaved(16, 32, 48, 64, 80, 1, 10, 10)
+    with pytest.raises(trid.TridError):
+        h.solve_interleaved(16, 32, 48, 64, 48, 8, 10, 10)
+    with pytest.raises(trid.TridError):
+        h.solve_interleaved(0, 32, 48, 64, 80, 8, 10, 10)
     h.close()
 
 

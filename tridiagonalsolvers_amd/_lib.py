@@ -28,6 +28,7 @@ EXPORTS = (
     "trid_create", "trid_destroy", "trid_solve_f32", "trid_solve_f64",
     "trid_generate_f32", "trid_generate_f64", "trid_plan_query",
     "trid_solve_refined_f64", "trid_residual_f64",
+    "trid_solve_interleaved_f32", "trid_solve_interleaved_f64",
     "trid_ragged_plan_create", "trid_ragged_plan_destroy", "trid_ragged_plan_info",
     "trid_ragged_solve_f64",
     "trid_server_create", "trid_server_destroy", "trid_server_register_operator",
@@ -155,6 +156,9 @@ def lib() -> ctypes.CDLL:
         g = getattr(L, f"trid_generate_{t}")
         g.argtypes = [i32, ctypes.c_uint64, P, P, P, P, i64, i64, P]
         g.restype = i32
+        v = getattr(L, f"trid_solve_interleaved_{t}")
+        v.argtypes = [P, P, P, P, P, P, i64, i64, i64, P]
+        v.restype = i32
     L.trid_solve_refined_f64.argtypes = [P, i32, P, P, P, P, P, i64, i64, i32,
                                          ctypes.POINTER(Opts), P]
     L.trid_solve_refined_f64.restype = i32

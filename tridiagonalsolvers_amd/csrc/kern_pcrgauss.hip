@@ -217,20 +217,25 @@ __global__ __launch_bounds__(TRID_PCRG_THREADS) void pcr_global_levels_kernel(
     }
 }
 
-// Pass 2: Gaussian elimination on subsystem r of system s from both ends
-// at once (a twisted factorization): lane l < 32 of a wave eliminates rows
-// 0 .. h-1 downwards (the Thomas forward sweep), lane l + 32 eliminates rows
-// m-1 .. h upwards, the two meet in a 2x2 system for x_{h-1}, x_h, and
-// each lane back-substitutes its half. This halves the serial chain of
-// Thomas (n/G steps) without changing the elimination it performs on each
-// half. Half-waves r, r+1, ... read consecutive addresses at every step.
-// The chains are long and few, so the kernel runs one wave per workgroup
-// (spreading them over every CU) and keeps the next U steps' loads in
-// flight while it computes the current U (double buffering). Loads and
-// stores share one 6-bit outstanding-operation counter, so waiting for a
-// batch while the next batch's 4U loads and this batch's 2U stores are in
-// flight needs 6U <= 63: U = 8. The elimination factors overwrite C2/D2; x
-// goes to X.
+// Strided elimination (pass 2 of the long form, and the interleaved-batch
+// solve): chain r of block s is the rows  base(s) + r + k * stride,
+// k = 0 .. m-1, base(s) = s * block_len, r < chains, m = the rows below
+// block_len. For pcrGauss the blocks are the systems and the chains their G
+// interleaved subsystems (stride G); an interleaved batch is one block whose
+// chains are the systems (row i of system j at i * stride + j).
+//
+// Each chain is eliminated from both ends at once (a twisted
+// factorization): lane l < 32 of a wave eliminates rows 0 .. h-1 downwards
+// (the Thomas forward sweep), lane l + 32 eliminates rows m-1 .. h upwards,
+// the two meet in a 2x2 system for x_{h-1}, x_h, and each lane
+// back-substitutes its half. This halves the serial chain of Thomas without
+// changing the elimination it performs on each half. Half-waves r, r+1, ...
+// read consecutive addresses at every step. The chains are long and few, so
+// the kernel runs one wave per workgroup (spreading them over every CU) and
+// keeps the next U steps' loads in flight while it computes the current U
+// (double buffering). Loads and stores share one 6-bit outstanding-operation
+// counter, so waiting for a batch while the next batch's 4U loads and this
+// batch's 2U stores are in flight needs 6U <= 63: U = 8.
 //
 // In elimination order i = 0, 1, ... a lane visits row k_i (k_i = i on the
 // top half, m-1-i on the bottom half) with p the coupling to the row
@@ -239,32 +244,33 @@ __global__ __launch_bounds__(TRID_PCRG_THREADS) void pcr_global_levels_kernel(
 //   den = b - p q'_{i-1},  q'_i = q / den,  d'_i = (d - p d'_{i-1}) / den
 // (p ignored for i = 0), so x_{k_i} = d'_i - q'_i x_{k_{i+1}}. At the meeting
 // point  x_{h-1} + c' x_h = d'_top  and  x_h + a' x_{h-1} = d'_bottom.
+// The factors q', d' go to FC, FD (which may be C, D themselves, or X for
+// FD: each location is read before the same lane overwrites it).
 template <typename T, int U>
-__global__ __launch_bounds__(64) void thomas_strided_kernel(const T* __restrict__ A2,
-                                                            const T* __restrict__ B2,
-                                                            T* __restrict__ C2,
-                                                            T* __restrict__ D2,
-                                                            T* __restrict__ X, int64_t n,
-                                                            int64_t N, int G)
+__global__ __launch_bounds__(64) void thomas_strided_kernel(const T* A, const T* B, const T* C,
+                                                            const T* D, T* FC, T* FD, T* X,
+                                                            int64_t block_len, int64_t blocks,
+                                                            int64_t chains, int64_t stride)
 {
     const int lane = threadIdx.x;
     const bool top = lane < kWave / 2;
     const int64_t chain = (int64_t)blockIdx.x * (kWave / 2) + (lane & (kWave / 2 - 1));
-    const int64_t s = chain / G;
-    const int r = (int)(chain % G);
-    const bool live = chain < N * G && r < n;
-    const int64_t m = live ? (n - r + G - 1) / G : 0;  // rows r, r+G, ... below n
-    const int64_t h = (m + 1) / 2;                     // rows of the top half
+    const int64_t s = chain / chains;
+    const int64_t r = chain % chains;
+    const bool live = chain < blocks * chains && r < block_len;
+    const int64_t m = live ? (block_len - r + stride - 1) / stride : 0;
+    const int64_t h = (m + 1) / 2;  // rows of the top half
     const int64_t len = top ? h : m - h;
     // row k_i = first + i * step
     const int64_t first = top ? 0 : m - 1;
-    const int64_t step = top ? (int64_t)G : -(int64_t)G;
-    const int64_t o0 = s * n + r + first * G;
-    const T* pp = (top ? A2 : C2) + o0;  // coupling to the eliminated side
-    const T* pb = B2 + o0;
-    const T* pq = (top ? C2 : A2) + o0;  // coupling to the next row
-    T* wq = C2 + o0;
-    T* wd = D2 + o0;
+    const int64_t step = top ? stride : -stride;
+    const int64_t o0 = s * block_len + r + first * stride;
+    const T* pp = (top ? A : C) + o0;  // coupling to the eliminated side
+    const T* pb = B + o0;
+    const T* pq = (top ? C : A) + o0;  // coupling to the next row
+    const T* pd = D + o0;
+    T* wq = FC + o0;
+    T* wd = FD + o0;
     T* px = X + o0;
 
     T qv = T(0), dv = T(0);
@@ -277,7 +283,7 @@ __global__ __launch_bounds__(64) void thomas_strided_kernel(const T* __restrict_
                 np[u] = pp[i * step];
                 nb[u] = pb[i * step];
                 nq[u] = pq[i * step];
-                nd[u] = wd[i * step];
+                nd[u] = pd[i * step];
             }
         }
     };
@@ -350,6 +356,30 @@ __global__ __launch_bounds__(64) void thomas_strided_kernel(const T* __restrict_
 }
 
 template <typename T>
+static hipError_t launch_strided(const T* A, const T* B, const T* C, const T* D, T* FC, T* FD,
+                                 T* X, int64_t block_len, int64_t blocks, int64_t chains,
+                                 int64_t stride, hipStream_t stream)
+{
+    const int64_t total = blocks * chains;  // two lanes each
+    const int64_t grid = (total + kWave / 2 - 1) / (kWave / 2);
+    if (total <= 0) return hipSuccess;
+    if (grid > 0x7fffffffLL) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL((thomas_strided_kernel<T, 8>), dim3((unsigned)grid), dim3(kWave), 0, stream,
+                       A, B, C, D, FC, FD, X, block_len, blocks, chains, stride);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_interleaved(const InterleavedArgs<T>& g)
+{
+    if (g.N <= 0) return hipSuccess;
+    // one block of n * stride elements whose N chains are the systems; the
+    // c' factors go to the workspace, the d' factors to x
+    return launch_strided<T>(g.a, g.b, g.c, g.d, g.ws, g.x, g.x, g.n * g.stride, 1, g.N,
+                             g.stride, g.stream);
+}
+
+template <typename T>
 hipError_t launch_pcr_thomas(const PcrThomasArgs<T>& g)
 {
     if (g.N <= 0) return hipSuccess;
@@ -419,15 +449,12 @@ hipError_t launch_pcr_thomas(const PcrThomasArgs<T>& g)
         else return hipErrorInvalidValue;
         if (e != hipSuccess) return e;
     }
-    const int64_t chains = g.N * (int64_t)G;  // two lanes each
-    const int64_t blocks = (chains + kWave / 2 - 1) / (kWave / 2);
-    if (blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL((thomas_strided_kernel<T, 8>), dim3((unsigned)blocks), dim3(kWave), 0,
-                       g.stream, A2, B2, C2, D2, g.x, g.n, g.N, G);
-    return hipGetLastError();
+    return launch_strided<T>(A2, B2, C2, D2, C2, D2, g.x, g.n, g.N, G, G, g.stream);
 }
 
 template hipError_t launch_pcr_thomas<float>(const PcrThomasArgs<float>&);
 template hipError_t launch_pcr_thomas<double>(const PcrThomasArgs<double>&);
+template hipError_t launch_interleaved<float>(const InterleavedArgs<float>&);
+template hipError_t launch_interleaved<double>(const InterleavedArgs<double>&);
 
 }  // namespace trid

@@ -20,7 +20,7 @@
 
 struct trid_context {
     int device;
-    void* ws;         // workspace of the two-pass pcrGauss form
+    void* ws;         // workspace of the two-pass pcrGauss form and interleaved solves
     size_t ws_bytes;
 };
 
@@ -213,9 +213,52 @@ int host_solve(int method, const T* a, const T* b, const T* c, const T* d, T* x,
     return rc;
 }
 
+template <typename T>
+int interleaved_impl(trid_context* h, const T* a, const T* b, const T* c, const T* d, T* x,
+                     int64_t n, int64_t N, int64_t stride, hipStream_t stream)
+{
+    if (!h || !a || !b || !c || !d || !x) return TRID_EINVAL;
+    if (n < 2 || N < 0 || stride < N || stride < 1) return TRID_EINVAL;
+    if (x == a || x == b || x == c) return TRID_EINVAL;
+    if (N == 0) return TRID_OK;
+    if (n > INT64_MAX / stride / (int64_t)sizeof(T)) return TRID_EINVAL;
+    int st = check_device();
+    if (st != TRID_OK) return st;
+    st = ensure_ws(h, (size_t)n * (size_t)stride * sizeof(T));
+    if (st != TRID_OK) return st;
+    trid::InterleavedArgs<T> g{};
+    g.a = a;
+    g.b = b;
+    g.c = c;
+    g.d = d;
+    g.x = x;
+    g.n = n;
+    g.N = N;
+    g.stride = stride;
+    g.ws = static_cast<T*>(h->ws);
+    g.stream = stream;
+    return from_hip(trid::launch_interleaved<T>(g));
+}
+
 }  // namespace
 
 extern "C" {
+
+int trid_solve_interleaved_f32(trid_handle handle, const float* a, const float* b,
+                               const float* c, const float* d, float* x, int64_t systemSize,
+                               int64_t numSystems, int64_t batchStride, void* stream)
+{
+    return set_status(interleaved_impl<float>(handle, a, b, c, d, x, systemSize, numSystems,
+                                              batchStride, static_cast<hipStream_t>(stream)));
+}
+
+int trid_solve_interleaved_f64(trid_handle handle, const double* a, const double* b,
+                               const double* c, const double* d, double* x, int64_t systemSize,
+                               int64_t numSystems, int64_t batchStride, void* stream)
+{
+    return set_status(interleaved_impl<double>(handle, a, b, c, d, x, systemSize, numSystems,
+                                               batchStride, static_cast<hipStream_t>(stream)));
+}
 
 const char* trid_status_string(int status)
 {

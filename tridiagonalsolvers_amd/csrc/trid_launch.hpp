@@ -78,6 +78,24 @@ struct PcrThomasArgs {
 template <typename T>
 hipError_t launch_pcr_thomas(const PcrThomasArgs<T>& g);
 
+// Interleaved batch (kern_pcrgauss.hip): row i of system j at i * stride + j.
+template <typename T>
+struct InterleavedArgs {
+    const T* a;
+    const T* b;
+    const T* c;
+    const T* d;
+    T* x;
+    int64_t n;       // rows per system
+    int64_t N;       // systems, N <= stride
+    int64_t stride;  // elements between consecutive rows of one system
+    T* ws;           // workspace of n * stride elements (c' factors)
+    hipStream_t stream;
+};
+
+template <typename T>
+hipError_t launch_interleaved(const InterleavedArgs<T>& g);
+
 // FP32 solve + FP64 residual refinement (kern_refine.hip).
 struct RefineArgs {
     const double* a;

@@ -26,7 +26,10 @@ constexpr int kDefaultIo = 3;  // wave-coalesced through LDS, non-temporal
 // L * M = n_pad. L <= 64: one wave holds a system (several systems share a
 // wave when L < 64). L = 64 * W, W = 2..16: a workgroup of W waves holds one
 // system (M <= 16 for FP32, M <= 8 for FP64, to fit 1024 threads' registers).
-int plan_reg(int64_t n_pad, int elem_size, const trid_opts& o, Plan* p)
+// Small batches trade unknowns per lane for lanes while a system still fits
+// one wave, until the launch has 1024 waves (4 per CU): 512 x 512 runs at
+// M = 8 (6.6 us, against 7.3 us at M = 16).
+int plan_reg(int64_t n_pad, int64_t N, int elem_size, const trid_opts& o, Plan* p)
 {
     int M, L;
     const int mmax_mw = elem_size == 4 ? 16 : 8;
@@ -42,6 +45,7 @@ int plan_reg(int64_t n_pad, int elem_size, const trid_opts& o, Plan* p)
         int64_t m = n_pad / 2;
         if (m > 16) m = 16;
         if (n_pad / m > 64) m = n_pad / 64;
+        while (m > 2 && n_pad / (m / 2) <= 64 && N * (n_pad / m) < 1024 * 64) m /= 2;
         M = (int)m;
         L = (int)(n_pad / m);
     } else {
@@ -98,7 +102,7 @@ int make_plan(int method, int64_t n, int64_t N, int elem_size, const trid_opts& 
                 if (!is_pow2(o.rest) || o.rest < 2 || o.rest > n_pad / 2) return TRID_EINVAL;
                 rest = o.rest;
             }
-            int st = plan_reg(n_pad, elem_size, o, p);
+            int st = plan_reg(n_pad, N, elem_size, o, p);
             if (st == TRID_EINVAL) return st;
             if (st == TRID_OK) {
                 if (rest < 0) {

@@ -157,6 +157,20 @@ class Handle:
                ctypes.c_void_p(_stream_ptr(stream)))
         check(st, "trid_solve")
 
+    def solve_interleaved(self, a, b, c, d, x, systemSize: int, numSystems: int,
+                          batchStride: int | None = None, stream=None,
+                          dtype: str | None = None) -> None:
+        """Interleaved batch: row i of system j at i * batchStride + j
+        (default batchStride = numSystems); x may be d."""
+        if dtype is None:
+            dtype = "f64" if str(getattr(a, "dtype", "")) == "torch.float64" else "f32"
+        f = getattr(lib(), f"trid_solve_interleaved_{dtype}")
+        stride = numSystems if batchStride is None else batchStride
+        st = f(self._h, *(ctypes.c_void_p(_ptr(v)) for v in (a, b, c, d, x)),
+               int(systemSize), int(numSystems), int(stride),
+               ctypes.c_void_p(_stream_ptr(stream)))
+        check(st, "trid_solve_interleaved")
+
     def solve_refined(self, method, a, b, c, d, x, systemSize: int, numSystems: int,
                       iters: int = 2, opts: Opts | None = None, stream=None) -> None:
         """FP64 arrays; FP32 solves plus `iters` FP64 residual corrections."""
