@@ -351,7 +351,9 @@ def bench_ragged(args):
     # every row strongly dominant: a,c ~ U[-1,1], b = |a|+|c|+U[1,2]; the
     # first/last rows' a/c are ignored by the solvers
     T.generate(0, 1, a, b, c, d, total, 1, stream=stream)
-    plan = R.RaggedPlan(offs)
+    plan_refine = R.RaggedPlan(offs)
+    plan_fp64 = R.RaggedPlan(offs, table="default-fp64")
+    plan = plan_fp64 if args.ragged_mode == "fp64" else plan_refine
     info = plan.info()
     h = T.Handle()
     uniq, first = np.unique(sizes, return_index=True)
@@ -361,11 +363,13 @@ def bench_ragged(args):
     x_other = torch.empty_like(a)
     fp64 = args.ragged_mode == "fp64"
 
+    other = plan_refine if fp64 else plan_fp64
+
     def ragged_step():
         plan.solve(a, b, c, d, x, stream=stream, fp64=fp64)
 
     def other_step():
-        plan.solve(a, b, c, d, x_other, stream=stream, fp64=not fp64)
+        other.solve(a, b, c, d, x_other, stream=stream, fp64=not fp64)
 
     def plain_step():
         for n, f, cnt in runs:
@@ -427,7 +431,8 @@ def bench_ragged(args):
     if not args.no_cpu:
         host = [v.cpu().numpy() for v in (a, b, c, d)]
         out["cpu_baseline"] = ragged_cpu_baseline(args, host, offs, x.cpu().numpy())
-    plan.close()
+    plan_refine.close()
+    plan_fp64.close()
     h.close()
     print(json.dumps(out), flush=True)
 

@@ -3,12 +3,12 @@
 For every padded size 2..16384 it times the candidate configurations of the
 persistent ragged kernel -- unknowns per lane (threads per system), the
 CR -> PCR switch size (`rest`, with rest = 2 being plain CR) -- on a batch of
-that size alone, with the same FP32 solve + FP64 refinement the ragged solve
-runs, and keeps the fastest. The result is a JSON table
-(tune/ragged_f64_refine.json) that RaggedPlan passes to
-trid_ragged_plan_create. Needs a gfx950 device.
+that size alone, in the mode the table is for (FP32 solve + FP64
+refinement, or one FP64 solve), and keeps the fastest. The result is a JSON
+table (tune/ragged_f64_refine.json, tune/ragged_f64_fp64.json) that
+RaggedPlan passes to trid_ragged_plan_create. Needs a gfx950 device.
 
-    python -m tridiagonalsolvers_amd.autotune [--out PATH] [--unknowns 4194304]
+    python -m tridiagonalsolvers_amd.autotune [--mode refine|fp64] [--out PATH]
 """
 from __future__ import annotations
 
@@ -34,19 +34,22 @@ def candidates(size: int):
         m = 16 if size > 8192 else 8
         lanes = size // m
         w = lanes // 64
-        for r in sorted({2, max(2, w // 2), w, 4 * w, 16 * w}):
+        for r in sorted({2, max(2, w // 2), w, 2 * w, 4 * w, 8 * w, 16 * w, 32 * w, 64 * w}):
             if 2 <= r <= lanes:
                 out.append(dict(size=size, per_lane=m, lanes=lanes, rest=r))
     return out
 
 
-def tune(sizes=None, unknowns: int = 1 << 22, iters: int = 2, reps: int = 5, log=print):
+def tune(sizes=None, unknowns: int = 1 << 22, iters: int = 2, reps: int = 5, log=print,
+         fp64: bool = False):
     import torch
 
     from .ragged import RaggedPlan, offsets_of
     from .solvers import generate
 
-    sizes = sizes or [1 << k for k in range(1, 15)]
+    sizes = sizes or [1 << k for k in range(1, 15 if not fp64 else 14)]
+    if fp64:
+        iters = 0
     table = []
     for size in sizes:
         N = max(64, unknowns // size)
@@ -56,14 +59,14 @@ def tune(sizes=None, unknowns: int = 1 << 22, iters: int = 2, reps: int = 5, log
         best = None
         for cand in candidates(size):
             plan = RaggedPlan(offs, table=[cand])
-            plan.solve(*arrs, iters=iters, graph=False)
+            plan.solve(*arrs, iters=iters, graph=False, fp64=fp64)
             torch.cuda.synchronize()
             times = []
             for _ in range(reps):
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
                 e0.record()
-                plan.solve(*arrs, iters=iters, graph=False)
+                plan.solve(*arrs, iters=iters, graph=False, fp64=fp64)
                 e1.record()
                 torch.cuda.synchronize()
                 times.append(e0.elapsed_time(e1))
@@ -78,17 +81,22 @@ def tune(sizes=None, unknowns: int = 1 << 22, iters: int = 2, reps: int = 5, log
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "tune",
-                                                  "ragged_f64_refine.json"))
+    ap.add_argument("--mode", default="refine", choices=["refine", "fp64"])
+    ap.add_argument("--out", default=None)
     ap.add_argument("--unknowns", type=int, default=1 << 22)
     ap.add_argument("--iters", type=int, default=2)
     args = ap.parse_args()
     import torch
+    fp64 = args.mode == "fp64"
+    out = args.out or os.path.join(os.path.dirname(__file__), "tune",
+                                   f"ragged_f64_{args.mode}.json")
     t0 = time.time()
-    table = tune(unknowns=args.unknowns, iters=args.iters)
-    os.makedirs(os.path.dirname(args.out), exist_ok=True)
+    table = tune(unknowns=args.unknowns, iters=args.iters, fp64=fp64)
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    args.out = out
     with open(args.out, "w") as f:
-        json.dump({"device": torch.cuda.get_device_name(0), "iters": args.iters,
+        json.dump({"device": torch.cuda.get_device_name(0), "mode": args.mode,
+                   "iters": 0 if fp64 else args.iters,
                    "unknowns_per_bucket": args.unknowns, "seconds": round(time.time() - t0, 1),
                    "buckets": table}, f, indent=1)
     print(args.out)

@@ -32,6 +32,14 @@ namespace trid {
 
 namespace {
 
+// Coefficient arrays whose loads are in flight at once in the refinement
+// passes (2 or 4). Measured on the 100,000-system batch: sequential 3.43 ms,
+// 4 in flight 3.68 ms (the registers spill), 2 in flight 3.34 ms. The FP64
+// mode always issues all four (1.95 ms, against 2.09 ms sequential).
+#ifndef TRID_RAGGED_AHEAD
+#define TRID_RAGGED_AHEAD 2
+#endif
+
 constexpr int kRaggedThreads = 1024;
 constexpr int kRaggedWaves = kRaggedThreads / kWave;
 static_assert(kRaggedWaves == 16, "workgroup items assume 16 waves");
@@ -74,44 +82,42 @@ constexpr size_t ragged_lds_bytes()
 static_assert(ragged_lds_bytes<kDirect>() <= 160 * 1024, "LDS of the FP64 mode");
 static_assert(ragged_lds_bytes<kRefineWide>() <= 160 * 1024, "LDS of the refine mode");
 
-// Lane-blocked gather of one coefficient array. The system (segment of L
-// lanes, first row of the segment at lane - ll) starts at `base` and has n
-// unknowns; lane ll must end with unknowns [ll*M, ll*M+M) (`fill` past n).
-// Load q of lane ll reads unknown q*L+ll, so every load instruction of the
-// wave reads L consecutive doubles of each system (coalesced whatever the
-// alignment of `base`), and the LDS area transposes them into rows.
+// Lane-blocked gathers. The system (segment of L lanes, first row of the
+// segment at lane - ll) starts at `base` and has n unknowns; lane ll must
+// end with unknowns [ll*M, ll*M+M) (`fill` past n). Load q of lane ll reads
+// unknown q*L+ll (fetch_cols), so every load instruction of the wave reads L
+// consecutive doubles of each system (coalesced whatever the alignment of
+// `base`); the LDS area then transposes them into rows (transpose_rows).
+// Fetching all four arrays before the first transpose keeps one memory
+// round trip per pass instead of four.
 template <int M>
-__device__ __forceinline__ void gather_rows(const double* __restrict__ X, int64_t base, int64_t n,
-                                            int L, int ll, int lane, bool live, double fill,
-                                            char* stage, double (&v)[M])
+__device__ __forceinline__ void fetch_cols(const double* __restrict__ X, int64_t base, int64_t n,
+                                           int L, int ll, bool live, double fill, double (&t)[M])
+{
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+        const int64_t j = (int64_t)q * L + ll;
+        t[q] = (live && j < n) ? X[base + j] : fill;
+    }
+}
+
+template <int M>
+__device__ __forceinline__ void transpose_rows(const double (&t)[M], int L, int ll, int lane,
+                                               char* stage, double (&v)[M])
 {
     if constexpr (M == 1) {
-        v[0] = (live && ll < n) ? X[base + ll] : fill;
+        v[0] = t[0];
     } else {
         using S = RStage<M>;
         const int seg_row = lane - ll;
-        // The index arithmetic below is cheap; without this the compiler
-        // hoists every address of every gather out of the refinement loop
-        // and spills them.
-        asm volatile("" : "+v"(ll), "+v"(n));
 #pragma unroll
         for (int h = 0; h < S::kRounds; ++h) {
-            // in batches of 4 loads, to bound the registers in flight
 #pragma unroll
-            for (int q0 = 0; q0 < S::kQ; q0 += 4) {
-                constexpr int B = S::kQ < 4 ? S::kQ : 4;
-                double t[B];
-#pragma unroll
-                for (int q = 0; q < B; ++q) {
-                    const int64_t j = (int64_t)(h * S::kQ + q0 + q) * L + ll;
-                    t[q] = (live && j < n) ? X[base + j] : fill;
-                }
-#pragma unroll
-                for (int q = 0; q < B; ++q) {
-                    const int j = (h * S::kQ + q0 + q) * L + ll;
-                    const int row = seg_row + j / M - h * S::kRows;
-                    *reinterpret_cast<double*>(stage + row * S::kRowBytes + (j % M) * 8) = t[q];
-                }
+            for (int q = 0; q < S::kQ; ++q) {
+                const int j = (h * S::kQ + q) * L + ll;
+                const int row = seg_row + j / M - h * S::kRows;
+                *reinterpret_cast<double*>(stage + row * S::kRowBytes + (j % M) * 8) =
+                    t[h * S::kQ + q];
             }
             __builtin_amdgcn_wave_barrier();
             if (lane / S::kRows == h) {
@@ -125,6 +131,42 @@ __device__ __forceinline__ void gather_rows(const double* __restrict__ X, int64_
             }
             __builtin_amdgcn_wave_barrier();
         }
+    }
+}
+
+// The four coefficient arrays of the lane's rows, lane-blocked. Up to 8
+// unknowns per lane all loads are issued before the first transpose; at 16
+// (the 16384-unknown systems) the arrays go one at a time to bound the
+// registers.
+template <int M>
+__device__ __forceinline__ void gather4(const RaggedArgs& g, int64_t base, int64_t n, int L,
+                                        int ll, int lane, bool live, char* stage,
+                                        double (&va)[M], double (&vb)[M], double (&vc)[M],
+                                        double (&vd)[M])
+{
+    // The index arithmetic is cheap; without this the compiler hoists every
+    // address of every gather out of the refinement loop and spills them.
+    asm volatile("" : "+v"(ll), "+v"(n));
+    if constexpr (M <= 8) {
+        double ta[M], tb[M], tc[M], td[M];
+        fetch_cols<M>(g.d, base, n, L, ll, live, 0.0, td);
+        fetch_cols<M>(g.b, base, n, L, ll, live, 1.0, tb);
+        fetch_cols<M>(g.a, base, n, L, ll, live, 0.0, ta);
+        fetch_cols<M>(g.c, base, n, L, ll, live, 0.0, tc);
+        transpose_rows<M>(td, L, ll, lane, stage, vd);
+        transpose_rows<M>(tb, L, ll, lane, stage, vb);
+        transpose_rows<M>(ta, L, ll, lane, stage, va);
+        transpose_rows<M>(tc, L, ll, lane, stage, vc);
+    } else {
+        double t[M];
+        fetch_cols<M>(g.d, base, n, L, ll, live, 0.0, t);
+        transpose_rows<M>(t, L, ll, lane, stage, vd);
+        fetch_cols<M>(g.b, base, n, L, ll, live, 1.0, t);
+        transpose_rows<M>(t, L, ll, lane, stage, vb);
+        fetch_cols<M>(g.a, base, n, L, ll, live, 0.0, t);
+        transpose_rows<M>(t, L, ll, lane, stage, va);
+        fetch_cols<M>(g.c, base, n, L, ll, live, 0.0, t);
+        transpose_rows<M>(t, L, ll, lane, stage, vc);
     }
 }
 
@@ -175,30 +217,78 @@ __device__ __forceinline__ void ragged_rows(const RaggedArgs& g, const double (&
                                             int64_t nb, int64_t j0, int64_t n, int L, int ll,
                                             int lane, bool live, char* stage, Eq<float> (&e)[M])
 {
-    double v[M], r[M];
-    gather_rows<M>(g.d, base, nb, L, ll, lane, live, 0.0, stage, r);
-    gather_rows<M>(g.b, base, nb, L, ll, lane, live, 1.0, stage, v);
+    double r[M], v[M];
+    auto use_b = [&]() {
 #pragma unroll
-    for (int s = 0; s < M; ++s) {
-        r[s] = fma(-v[s], x[s], r[s]);
-        e[s].b = (float)v[s];
-    }
-    gather_rows<M>(g.a, base, nb, L, ll, lane, live, 0.0, stage, v);
+        for (int s = 0; s < M; ++s) {
+            r[s] = fma(-v[s], x[s], r[s]);
+            e[s].b = (float)v[s];
+        }
+    };
+    auto use_a = [&]() {
 #pragma unroll
-    for (int s = 0; s < M; ++s) {
-        const double al = j0 + s > 0 ? v[s] : 0.0;
-        const double xl = s > 0 ? x[s > 0 ? s - 1 : 0] : x_left;
-        r[s] = fma(-al, xl, r[s]);
-        e[s].a = (float)al;
-    }
-    gather_rows<M>(g.c, base, nb, L, ll, lane, live, 0.0, stage, v);
+        for (int s = 0; s < M; ++s) {
+            const double al = j0 + s > 0 ? v[s] : 0.0;
+            const double xl = s > 0 ? x[s > 0 ? s - 1 : 0] : x_left;
+            r[s] = fma(-al, xl, r[s]);
+            e[s].a = (float)al;
+        }
+    };
+    auto use_c = [&]() {
 #pragma unroll
-    for (int s = 0; s < M; ++s) {
-        const double cr = j0 + s < n - 1 ? v[s] : 0.0;
-        const double xr = s < M - 1 ? x[s < M - 1 ? s + 1 : 0] : x_right;
-        r[s] = fma(-cr, xr, r[s]);
-        e[s].c = (float)cr;
-        e[s].d = (live && j0 + s < n) ? (float)r[s] : 0.0f;
+        for (int s = 0; s < M; ++s) {
+            const double cr = j0 + s < n - 1 ? v[s] : 0.0;
+            const double xr = s < M - 1 ? x[s < M - 1 ? s + 1 : 0] : x_right;
+            r[s] = fma(-cr, xr, r[s]);
+            e[s].c = (float)cr;
+            e[s].d = (live && j0 + s < n) ? (float)r[s] : 0.0f;
+        }
+    };
+    // see gather4 for why
+    asm volatile("" : "+v"(ll), "+v"(nb));
+    if constexpr (M <= 8 && TRID_RAGGED_AHEAD == 2) {
+        // two arrays in flight: the next array's loads are issued before
+        // the current one is transposed
+        double t0[M], t1[M];
+        fetch_cols<M>(g.d, base, nb, L, ll, live, 0.0, t0);
+        fetch_cols<M>(g.b, base, nb, L, ll, live, 1.0, t1);
+        transpose_rows<M>(t0, L, ll, lane, stage, r);
+        fetch_cols<M>(g.a, base, nb, L, ll, live, 0.0, t0);
+        transpose_rows<M>(t1, L, ll, lane, stage, v);
+        use_b();
+        fetch_cols<M>(g.c, base, nb, L, ll, live, 0.0, t1);
+        transpose_rows<M>(t0, L, ll, lane, stage, v);
+        use_a();
+        transpose_rows<M>(t1, L, ll, lane, stage, v);
+        use_c();
+    } else if constexpr (M <= 8) {
+        // all loads first (one memory round trip), then the transposes,
+        // each array consumed as soon as it is in rows
+        double ta[M], tb[M], tc[M], td[M];
+        fetch_cols<M>(g.d, base, nb, L, ll, live, 0.0, td);
+        fetch_cols<M>(g.b, base, nb, L, ll, live, 1.0, tb);
+        fetch_cols<M>(g.a, base, nb, L, ll, live, 0.0, ta);
+        fetch_cols<M>(g.c, base, nb, L, ll, live, 0.0, tc);
+        transpose_rows<M>(td, L, ll, lane, stage, r);
+        transpose_rows<M>(tb, L, ll, lane, stage, v);
+        use_b();
+        transpose_rows<M>(ta, L, ll, lane, stage, v);
+        use_a();
+        transpose_rows<M>(tc, L, ll, lane, stage, v);
+        use_c();
+    } else {
+        double t[M];
+        fetch_cols<M>(g.d, base, nb, L, ll, live, 0.0, t);
+        transpose_rows<M>(t, L, ll, lane, stage, r);
+        fetch_cols<M>(g.b, base, nb, L, ll, live, 1.0, t);
+        transpose_rows<M>(t, L, ll, lane, stage, v);
+        use_b();
+        fetch_cols<M>(g.a, base, nb, L, ll, live, 0.0, t);
+        transpose_rows<M>(t, L, ll, lane, stage, v);
+        use_a();
+        fetch_cols<M>(g.c, base, nb, L, ll, live, 0.0, t);
+        transpose_rows<M>(t, L, ll, lane, stage, v);
+        use_c();
     }
 }
 
@@ -209,19 +299,11 @@ __device__ __forceinline__ void direct_rows(const RaggedArgs& g, int64_t base, i
                                             int64_t j0, int64_t n, int L, int ll, int lane,
                                             bool live, char* stage, Eq<double> (&e)[M])
 {
-    double v[M];
-    gather_rows<M>(g.d, base, nb, L, ll, lane, live, 0.0, stage, v);
+    double va[M], vb[M], vc[M], vd[M];
+    gather4<M>(g, base, nb, L, ll, lane, live, stage, va, vb, vc, vd);
 #pragma unroll
-    for (int s = 0; s < M; ++s) e[s].d = v[s];
-    gather_rows<M>(g.b, base, nb, L, ll, lane, live, 1.0, stage, v);
-#pragma unroll
-    for (int s = 0; s < M; ++s) e[s].b = v[s];
-    gather_rows<M>(g.a, base, nb, L, ll, lane, live, 0.0, stage, v);
-#pragma unroll
-    for (int s = 0; s < M; ++s) e[s].a = j0 + s > 0 ? v[s] : 0.0;
-    gather_rows<M>(g.c, base, nb, L, ll, lane, live, 0.0, stage, v);
-#pragma unroll
-    for (int s = 0; s < M; ++s) e[s].c = j0 + s < n - 1 ? v[s] : 0.0;
+    for (int s = 0; s < M; ++s)
+        e[s] = Eq<double>{j0 + s > 0 ? va[s] : 0.0, vb[s], j0 + s < n - 1 ? vc[s] : 0.0, vd[s]};
 }
 
 template <bool DIRECT, int M>

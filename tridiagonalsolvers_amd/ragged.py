@@ -17,6 +17,8 @@ from .solvers import _ptr, _stream_ptr
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_TABLE = os.path.join(HERE, "tune", "ragged_f64_refine.json")
+# bucket table tuned for the one-pass FP64 mode (RaggedPlan(..., "default-fp64"))
+DEFAULT_TABLE_FP64 = os.path.join(HERE, "tune", "ragged_f64_fp64.json")
 NO_GRAPH = 1
 FP64 = 2
 
@@ -49,6 +51,8 @@ class RaggedPlan:
         self.num_systems = len(offsets) - 1
         if table == "default":
             table = load_table()
+        elif table == "default-fp64":
+            table = load_table(DEFAULT_TABLE_FP64)
         table = table or []
         arr = (BucketCfg * max(1, len(table)))()
         for i, t in enumerate(table):
