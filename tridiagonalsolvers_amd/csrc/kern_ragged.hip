@@ -316,9 +316,9 @@ __device__ void wave_item(const RaggedArgs& g, const RaggedItem& it, const Ragge
     const bool live = seg < it.count;
     int64_t base = 0, n = 2;
     if (live) {
-        const int s = g.order[it.first + seg];
-        base = g.offsets[s];
-        n = g.offsets[s + 1] - base;
+        const RaggedSys sy = g.sys[it.first + seg];
+        base = sy.base;
+        n = sy.n;
     }
     const int64_t j0 = (int64_t)ll * M;
     double x[M];
@@ -359,9 +359,9 @@ __device__ void wg_item(const RaggedArgs& g, const RaggedItem& it, const RaggedB
     const bool live = grp < it.count;
     int64_t base = 0, n = 2;
     if (live) {
-        const int s = g.order[it.first + grp];
-        base = g.offsets[s];
-        n = g.offsets[s + 1] - base;
+        const RaggedSys sy = g.sys[it.first + grp];
+        base = sy.base;
+        n = sy.n;
     }
     // this wave's block of the system: unknowns [w*64*M, (w+1)*64*M)
     const int64_t wb = (int64_t)w * kWave * M;
@@ -416,36 +416,47 @@ __global__ __launch_bounds__(kRaggedThreads) void ragged_kernel(RaggedArgs g)
 {
     constexpr bool DIRECT = MODE == kDirect;
     __shared__ __align__(16) char smem[ragged_lds_bytes<MODE>()];
-    __shared__ int s_item;
+    __shared__ int s_item[2];
+    __shared__ RaggedBucket s_bk[kRaggedMaxBuckets];
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     char* stage = smem + ragged_xchg_bytes<MODE>() + (size_t)wave * kStageWaveBytes;
+    if (threadIdx.x < g.n_buckets) s_bk[threadIdx.x] = g.buckets[threadIdx.x];
 
-    // phase 1: workgroup items (long systems), largest bucket first
-    for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(&g.counters[0], 1);
-        __syncthreads();
-        const int item = s_item;
-        __syncthreads();
-        if (item >= g.n_wg_items) break;
+    // phase 1: workgroup items (long systems), largest bucket first. The
+    // next item's queue slot is claimed when an item starts and published
+    // when it ends, so the atomic's latency hides behind the item.
+    if (threadIdx.x == 0) s_item[0] = atomicAdd(&g.counters[0], 1);
+    __syncthreads();
+    int item = s_item[0];
+    for (int k = 1; item < g.n_wg_items; ++k) {
+        int next = 0;
+        if (threadIdx.x == 0) next = atomicAdd(&g.counters[0], 1);
         const RaggedItem it = g.items[item];
-        const RaggedBucket bk = g.buckets[it.bucket];
+        const RaggedBucket bk = s_bk[it.bucket];
+        bool done = false;
         if constexpr (MODE == kRefineWide) {
             if (bk.per_lane == 16) {
                 wg_item<false, 16>(g, it, bk, wave, lane, smem, stage);
-                continue;
+                done = true;
             }
         }
-        wg_item<DIRECT, 8>(g, it, bk, wave, lane, smem, stage);
+        if (!done) wg_item<DIRECT, 8>(g, it, bk, wave, lane, smem, stage);
+        // wg_item ends with a barrier, after which every thread has read
+        // s_item[(k - 1) & 1]; slot k & 1 was last read before that barrier
+        if (threadIdx.x == 0) s_item[k & 1] = next;
+        __syncthreads();
+        item = s_item[k & 1];
     }
-    // phase 2: wave items, each wave on its own
-    for (;;) {
-        int item = 0;
-        if (lane == 0) item = atomicAdd(&g.counters[1], 1);
-        item = __builtin_amdgcn_readfirstlane(item);
-        if (item >= g.n_wave_items) break;
-        const RaggedItem it = g.items[g.n_wg_items + item];
-        const RaggedBucket bk = g.buckets[it.bucket];
+    // phase 2: wave items, each wave on its own (same claim-ahead)
+    int witem = 0;
+    if (lane == 0) witem = atomicAdd(&g.counters[1], 1);
+    witem = __builtin_amdgcn_readfirstlane(witem);
+    while (witem < g.n_wave_items) {
+        int next = 0;
+        if (lane == 0) next = atomicAdd(&g.counters[1], 1);
+        const RaggedItem it = g.items[g.n_wg_items + witem];
+        const RaggedBucket bk = s_bk[it.bucket];
         switch (bk.per_lane) {
             case 1: wave_item<DIRECT, 1>(g, it, bk, lane, stage); break;
             case 2: wave_item<DIRECT, 2>(g, it, bk, lane, stage); break;
@@ -453,6 +464,7 @@ __global__ __launch_bounds__(kRaggedThreads) void ragged_kernel(RaggedArgs g)
             case 8: wave_item<DIRECT, 8>(g, it, bk, lane, stage); break;
             default: break;
         }
+        witem = __builtin_amdgcn_readfirstlane(next);
     }
 }
 

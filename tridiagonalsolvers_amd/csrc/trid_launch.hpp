@@ -128,19 +128,24 @@ struct RaggedBucket {
     int32_t lanes;     // lanes per system (64*W for workgroup items)
     int32_t rest;      // CR -> PCR switch size
 };
+// One system of a ragged batch, in work-queue order: first unknown and size.
+struct RaggedSys {
+    int64_t base;
+    int64_t n;
+};
 struct RaggedArgs {
     const double* a;
     const double* b;
     const double* c;
     const double* d;
     double* x;
-    const int64_t* offsets;   // device, numSystems + 1
-    const int32_t* order;     // device, systems grouped by bucket
+    const RaggedSys* sys;     // device, numSystems, grouped by bucket
     const RaggedItem* items;  // device, workgroup items then wave items
     const RaggedBucket* buckets;
     int* counters;            // device, 2 queue counters
     int n_wg_items;
     int n_wave_items;
+    int n_buckets;            // <= kRaggedMaxBuckets
     int iters;
     int wide;                 // 1 if any 16384-unknown bucket is present
     int direct;               // 1: one FP64 solve per system (no FP32 stage)
@@ -152,8 +157,10 @@ int ragged_max_grid(int device);
 // Host side of a ragged plan (trid_ragged.cpp): systems grouped by padded
 // size, the bucket table and the work items of the persistent kernel.
 constexpr int64_t kRaggedMaxSize = 16384;
+constexpr int kRaggedMaxBuckets = 16;  // padded sizes 2 .. 16384
 struct RaggedHostPlan {
     std::vector<int32_t> order;
+    std::vector<RaggedSys> sys;     // systems in `order`
     std::vector<RaggedItem> items;  // workgroup items, then wave items
     std::vector<RaggedBucket> buckets;
     std::vector<int64_t> bucket_systems;

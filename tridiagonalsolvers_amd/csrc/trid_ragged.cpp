@@ -26,9 +26,8 @@ struct trid_ragged_plan_s {
     int wide = 0;
     std::vector<trid::RaggedBucket> buckets;  // host copy (info)
     std::vector<int64_t> bucket_systems;
-    char* dev = nullptr;  // one device block: offsets, order, items, buckets, counters
-    int64_t* d_offsets = nullptr;
-    int32_t* d_order = nullptr;
+    char* dev = nullptr;  // one device block: systems, items, buckets, counters
+    trid::RaggedSys* d_sys = nullptr;
     trid::RaggedItem* d_items = nullptr;
     trid::RaggedBucket* d_buckets = nullptr;
     int* d_counters = nullptr;
@@ -149,6 +148,12 @@ int ragged_build_host(const int64_t* offsets, int64_t N, const trid_bucket_cfg* 
     out->n_wave_items = (int)wave_items.size();
     out->items = wg_items;
     out->items.insert(out->items.end(), wave_items.begin(), wave_items.end());
+    if (out->buckets.size() > (size_t)kRaggedMaxBuckets) return TRID_EINVAL;
+    out->sys.resize(N);
+    for (int64_t k = 0; k < N; ++k) {
+        const int32_t s = out->order[k];
+        out->sys[k] = RaggedSys{offsets[s], offsets[s + 1] - offsets[s]};
+    }
     return TRID_OK;
 }
 
@@ -177,28 +182,22 @@ int trid_ragged_plan_create(trid_ragged_plan* out, const int64_t* offsets, int64
         if (b.per_lane == 16) p->wide = 1;
     const std::vector<trid::RaggedItem>& items = hp.items;
     const std::vector<trid::RaggedBucket>& used = hp.buckets;
-    const std::vector<int32_t>& order = hp.order;
-
     // device block
     auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
-    const size_t b_off = up(sizeof(int64_t) * (numSystems + 1));
-    const size_t b_ord = up(sizeof(int32_t) * numSystems);
+    const size_t b_sys = up(sizeof(trid::RaggedSys) * numSystems);
     const size_t b_itm = up(sizeof(trid::RaggedItem) * (items.size() + 1));
     const size_t b_bkt = up(sizeof(trid::RaggedBucket) * (used.size() + 1));
     const size_t b_ctr = 256;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->dev), b_off + b_ord + b_itm + b_bkt + b_ctr);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->dev), b_sys + b_itm + b_bkt + b_ctr);
     if (e != hipSuccess) {
         delete p;
         return TRID_ENOMEM;
     }
-    p->d_offsets = reinterpret_cast<int64_t*>(p->dev);
-    p->d_order = reinterpret_cast<int32_t*>(p->dev + b_off);
-    p->d_items = reinterpret_cast<trid::RaggedItem*>(p->dev + b_off + b_ord);
-    p->d_buckets = reinterpret_cast<trid::RaggedBucket*>(p->dev + b_off + b_ord + b_itm);
-    p->d_counters = reinterpret_cast<int*>(p->dev + b_off + b_ord + b_itm + b_bkt);
-    if ((e = hipMemcpy(p->d_offsets, offsets, sizeof(int64_t) * (numSystems + 1),
-                       hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemcpy(p->d_order, order.data(), sizeof(int32_t) * numSystems,
+    p->d_sys = reinterpret_cast<trid::RaggedSys*>(p->dev);
+    p->d_items = reinterpret_cast<trid::RaggedItem*>(p->dev + b_sys);
+    p->d_buckets = reinterpret_cast<trid::RaggedBucket*>(p->dev + b_sys + b_itm);
+    p->d_counters = reinterpret_cast<int*>(p->dev + b_sys + b_itm + b_bkt);
+    if ((e = hipMemcpy(p->d_sys, hp.sys.data(), sizeof(trid::RaggedSys) * numSystems,
                        hipMemcpyHostToDevice)) != hipSuccess ||
         (!items.empty() &&
          (e = hipMemcpy(p->d_items, items.data(), sizeof(trid::RaggedItem) * items.size(),
@@ -263,13 +262,13 @@ int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* 
     g.c = c;
     g.d = d;
     g.x = x;
-    g.offsets = plan->d_offsets;
-    g.order = plan->d_order;
+    g.sys = plan->d_sys;
     g.items = plan->d_items;
     g.buckets = plan->d_buckets;
     g.counters = plan->d_counters;
     g.n_wg_items = plan->n_wg_items;
     g.n_wave_items = plan->n_wave_items;
+    g.n_buckets = plan->n_buckets;
     g.iters = iters;
     g.wide = plan->wide;
     g.direct = direct;

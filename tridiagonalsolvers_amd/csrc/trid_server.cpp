@@ -172,8 +172,7 @@ void parallel_copy(const std::vector<CopyJob>& jobs)
 // Device (and staging) layout of one batch of N systems, `total` unknowns.
 struct BatchLayout {
     size_t arr;   // bytes of one FP64 array
-    size_t off;   // offsets
-    size_t ord;   // order
+    size_t sys;   // systems in queue order
     size_t itm;   // items
     size_t bkt;   // buckets
     size_t ctr;   // counters
@@ -181,12 +180,11 @@ struct BatchLayout {
     BatchLayout(int64_t total, int64_t N)
     {
         arr = align256((size_t)total * sizeof(double));
-        off = align256(sizeof(int64_t) * (N + 1));
-        ord = align256(sizeof(int32_t) * N);
+        sys = align256(sizeof(trid::RaggedSys) * N);
         itm = align256(sizeof(trid::RaggedItem) * (N + 1));  // at most one item per system
         bkt = align256(sizeof(trid::RaggedBucket) * 16);
         ctr = 256;
-        bytes = 5 * arr + off + ord + itm + bkt + ctr;
+        bytes = 5 * arr + sys + itm + bkt + ctr;
     }
 };
 
@@ -341,15 +339,14 @@ int trid_server_s::launch(Slot& s)
         jobs.push_back(CopyJob{pd + q->pos, q->r.d, nb});
     }
     parallel_copy(jobs);
-    std::memcpy(pdesc, offs.data(), sizeof(int64_t) * (N + 1));
-    std::memcpy(pdesc + L.off, hp.order.data(), sizeof(int32_t) * N);
-    std::memcpy(pdesc + L.off + L.ord, hp.items.data(), sizeof(trid::RaggedItem) * hp.items.size());
-    std::memcpy(pdesc + L.off + L.ord + L.itm, hp.buckets.data(),
+    std::memcpy(pdesc, hp.sys.data(), sizeof(trid::RaggedSys) * N);
+    std::memcpy(pdesc + L.sys, hp.items.data(), sizeof(trid::RaggedItem) * hp.items.size());
+    std::memcpy(pdesc + L.sys + L.itm, hp.buckets.data(),
                 sizeof(trid::RaggedBucket) * hp.buckets.size());
     char* D = d_pool + s.dev_off;
     hipError_t e = hipMemcpyAsync(D, P, 4 * L.arr, hipMemcpyHostToDevice, s.stream);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(D + 5 * L.arr, pdesc, L.off + L.ord + L.itm + L.bkt,
+        e = hipMemcpyAsync(D + 5 * L.arr, pdesc, L.sys + L.itm + L.bkt,
                            hipMemcpyHostToDevice, s.stream);
     for (Req* q : s.reqs) {
         if (e != hipSuccess) break;
@@ -369,13 +366,13 @@ int trid_server_s::launch(Slot& s)
     g.d = reinterpret_cast<const double*>(D + 3 * L.arr);
     g.x = reinterpret_cast<double*>(D + 4 * L.arr);
     char* desc = D + 5 * L.arr;
-    g.offsets = reinterpret_cast<const int64_t*>(desc);
-    g.order = reinterpret_cast<const int32_t*>(desc + L.off);
-    g.items = reinterpret_cast<const trid::RaggedItem*>(desc + L.off + L.ord);
-    g.buckets = reinterpret_cast<const trid::RaggedBucket*>(desc + L.off + L.ord + L.itm);
-    g.counters = reinterpret_cast<int*>(desc + L.off + L.ord + L.itm + L.bkt);
+    g.sys = reinterpret_cast<const trid::RaggedSys*>(desc);
+    g.items = reinterpret_cast<const trid::RaggedItem*>(desc + L.sys);
+    g.buckets = reinterpret_cast<const trid::RaggedBucket*>(desc + L.sys + L.itm);
+    g.counters = reinterpret_cast<int*>(desc + L.sys + L.itm + L.bkt);
     g.n_wg_items = hp.n_wg_items;
     g.n_wave_items = hp.n_wave_items;
+    g.n_buckets = (int)hp.buckets.size();
     g.iters = o.iters;
     g.wide = 0;
     for (const auto& b : hp.buckets)
