@@ -420,7 +420,10 @@ def bench_ragged(args):
         "kernel_ms": {"ragged_graph": gpu_r, "plain_per_size": gpu_p},
         "roofline": {"bound": "hbm", "achieved": alg / (gpu_r * 1e-3) / 1e9,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": alg / (gpu_r * 1e-3) / 1e9 / PEAK_HBM_GBS, "traffic": None,
+                     "frac": alg / (gpu_r * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                     "traffic": traffic_from_profiles(
+                         f"ragged f64 {N} systems, sizes log-uniform [16, 8192], "
+                         f"{names[fp64]}, one graph launch"),
                      "note": "algorithmic 5*8 B per unknown (FP64 in/out)"
                              + ("" if fp64 else "; refinement re-reads are not credited")},
         "plain_per_size": {"value": total / wall_p, "launches": len(runs),
@@ -649,7 +652,8 @@ def bench_gauss_sweep(args):
            "roofline": {"bound": "hbm", "achieved": 5 * 4 * n * N / (best["ms"] * 1e-3) / 1e9,
                         "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "frac": 5 * 4 * n * N / (best["ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS,
-                        "traffic": None,
+                        "traffic": traffic_from_profiles(
+                            f"pcrgauss f32 {N} systems x {n} unknowns, depth {best['depth']}"),
                         "note": "two passes move about 17 n x 4 B per system (pass 1 reads and "
                                 "writes 4 arrays, pass 2 reads 4, writes and re-reads c', d', "
                                 "writes x): at most 5/17 of peak in algorithmic bytes"}}
