@@ -285,9 +285,9 @@ def test_multiwave_shapes_and_io(oracle, trid, dtype):
     a, b, c, d = oracle.generate("weak", 8, n, N, dtype)
     dev = [torch.from_numpy(v).cuda() for v in (a, b, c, d)]
     h = trid.Handle()
-    mmax = 16 if dtype == np.float32 else 8
     for M in (2, 4, 8, 16):
-        if M > mmax or n // M > 1024:
+        # FP64 at 16 unknowns per lane runs 512-thread workgroups (<= 8 waves)
+        if n // M > (1024 if dtype == np.float32 or M <= 8 else 512):
             continue
         for io in (1, 2, 3, 4):
             x = torch.full_like(dev[0], float("nan"))
@@ -295,6 +295,16 @@ def test_multiwave_shapes_and_io(oracle, trid, dtype):
                     trid.make_opts(rest=32, lanes=n // M, per_lane=M, io_mode=io))
             torch.cuda.synchronize()
             check(oracle, x.cpu().numpy(), a, b, c, d, n, dtype, TOL[dtype] * 10)
+    # FP64 n = 8192 at 16 unknowns per lane (8 waves), every I/O mode
+    if dtype == np.float64:
+        a, b, c, d = oracle.generate("dominant", 10, 8192, 5, dtype)
+        dev8 = [torch.from_numpy(v).cuda() for v in (a, b, c, d)]
+        for io in (1, 2, 3, 4):
+            x = torch.full_like(dev8[0], float("nan"))
+            h.solve("crpcr", *dev8, x, 8192, 5,
+                    trid.make_opts(rest=64, lanes=512, per_lane=16, io_mode=io))
+            torch.cuda.synchronize()
+            check(oracle, x.cpu().numpy(), a, b, c, d, 8192, dtype)
     # FP64 n=1024 over two waves
     if dtype == np.float64:
         a, b, c, d = oracle.generate("dominant", 9, 1024, N, dtype)

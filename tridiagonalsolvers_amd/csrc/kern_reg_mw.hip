@@ -29,12 +29,10 @@
 namespace trid {
 
 template <typename T, int M, int IO>
-__global__ __launch_bounds__(1024) void reg_mw_kernel(const T* __restrict__ A,
-                                                      const T* __restrict__ B,
-                                                      const T* __restrict__ C,
-                                                      const T* __restrict__ D,
-                                                      T* __restrict__ X, int64_t n, int rest,
-                                                      int vec, int64_t N, ConstCoef<T> cc)
+__device__ __forceinline__ void reg_mw_body(const T* __restrict__ A, const T* __restrict__ B,
+                                            const T* __restrict__ C, const T* __restrict__ D,
+                                            T* __restrict__ X, int64_t n, int rest, int vec,
+                                            int64_t N, ConstCoef<T> cc)
 {
     constexpr int E = 16 / (int)sizeof(T);
     const int W = blockDim.x / kWave;
@@ -102,6 +100,31 @@ __global__ __launch_bounds__(1024) void reg_mw_kernel(const T* __restrict__ A,
     }
 }
 
+// Up to 16 waves per system (the register budget of 1024 threads: 128 per
+// lane) ...
+template <typename T, int M, int IO>
+__global__ __launch_bounds__(1024) void reg_mw_kernel(const T* __restrict__ A,
+                                                      const T* __restrict__ B,
+                                                      const T* __restrict__ C,
+                                                      const T* __restrict__ D,
+                                                      T* __restrict__ X, int64_t n, int rest,
+                                                      int vec, int64_t N, ConstCoef<T> cc)
+{
+    reg_mw_body<T, M, IO>(A, B, C, D, X, n, rest, vec, N, cc);
+}
+
+// ... or up to 8 waves with twice the registers: FP64 at 16 unknowns per lane.
+template <typename T, int M, int IO>
+__global__ __launch_bounds__(512) void reg_mw512_kernel(const T* __restrict__ A,
+                                                        const T* __restrict__ B,
+                                                        const T* __restrict__ C,
+                                                        const T* __restrict__ D,
+                                                        T* __restrict__ X, int64_t n, int rest,
+                                                        int vec, int64_t N, ConstCoef<T> cc)
+{
+    reg_mw_body<T, M, IO>(A, B, C, D, X, n, rest, vec, N, cc);
+}
+
 template <typename T, int M>
 size_t mw_lds_bytes(int W, int io, int vec)
 {
@@ -116,14 +139,24 @@ static hipError_t launch_mw_io(const RegArgs<T>& g)
     const int W = g.L / kWave;
     const size_t lds = mw_lds_bytes<T, M>(W, IO, g.vec);
     if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
+    // FP64 at 16 unknowns per lane needs the 512-thread form (W <= 8)
+    constexpr bool wide = sizeof(T) == 8 && M == 16;
+    if (wide && W > 8) return hipErrorInvalidValue;
+    const void* kern;
+    if constexpr (wide) kern = (const void*)reg_mw512_kernel<T, M, IO>;
+    else kern = (const void*)reg_mw_kernel<T, M, IO>;
     if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute((const void*)reg_mw_kernel<T, M, IO>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t e =
+            hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((reg_mw_kernel<T, M, IO>), dim3((unsigned)g.N), dim3(kWave * W), lds,
-                       g.stream, g.a, g.b, g.c, g.d, g.x, g.n, g.rest, g.vec, g.N,
-                       ConstCoef<T>{g.cc != 0, g.a0, g.b0, g.c0});
+    const ConstCoef<T> cc{g.cc != 0, g.a0, g.b0, g.c0};
+    if constexpr (wide)
+        hipLaunchKernelGGL((reg_mw512_kernel<T, M, IO>), dim3((unsigned)g.N), dim3(kWave * W),
+                           lds, g.stream, g.a, g.b, g.c, g.d, g.x, g.n, g.rest, g.vec, g.N, cc);
+    else
+        hipLaunchKernelGGL((reg_mw_kernel<T, M, IO>), dim3((unsigned)g.N), dim3(kWave * W), lds,
+                           g.stream, g.a, g.b, g.c, g.d, g.x, g.n, g.rest, g.vec, g.N, cc);
     return hipGetLastError();
 }
 
@@ -149,9 +182,7 @@ hipError_t launch_reg_mw(const RegArgs<T>& g)
         case 2: return launch_mw_m<T, 2>(g);
         case 4: return launch_mw_m<T, 4>(g);
         case 8: return launch_mw_m<T, 8>(g);
-        case 16:
-            if constexpr (sizeof(T) == 4) return launch_mw_m<T, 16>(g);
-            return hipErrorInvalidValue;
+        case 16: return launch_mw_m<T, 16>(g);
         default: return hipErrorInvalidValue;
     }
 }
