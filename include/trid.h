@@ -279,8 +279,8 @@ int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* 
  * Dirichlet boundary) stays on the device; each step with parameter r solves
  * (I - r dxx) u* = (I + r dyy) u   (n row systems tridiag(-r, 1+2r, -r))
  * (I - r dyy) u' = (I + r dxx) u*  (n column systems, stride n)
- * with the constant-coefficient register CR-PCR kernel for the rows and a
- * pre-factored, coalesced one-lane-per-column Thomas for the columns.
+ * with the constant-coefficient register CR-PCR kernel for the rows, and for
+ * the columns on the transposed grid (tiled transposes through LDS).
  */
 typedef struct trid_adi_s* trid_adi;
 

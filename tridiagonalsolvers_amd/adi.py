@@ -1,7 +1,7 @@
 """Resident ADI grid (trid_adi_* in include/trid.h): Peaceman-Rachford steps
-of the 2D heat equation on an n x n FP64 device grid, the row sweep on the
-constant-coefficient register CR-PCR kernel and the strided column sweep on a
-pre-factored coalesced Thomas."""
+of the 2D heat equation on an n x n FP64 device grid, both sweeps on the
+constant-coefficient register CR-PCR kernel (the column sweep on the
+transposed grid)."""
 from __future__ import annotations
 
 import ctypes

@@ -7,23 +7,24 @@
 //   (I - r dxx) u* = (I + r dyy) u        row sweep:    n systems, contiguous
 //   (I - r dyy) u' = (I + r dxx) u*       column sweep: n systems, stride n
 //
-// * the row sweep's right-hand side is one elementwise pass (adi_rhs_rows);
-//   the solve is the register CR-PCR kernel in its constant-coefficient mode,
-//   which reads only d and writes x (16 bytes per unknown in FP64);
-// * the column sweep runs one lane per column over the interleaved layout:
-//   at every row the 64 lanes of a wave touch 64 consecutive doubles, so both
-//   passes are coalesced. Its right-hand side is formed on the fly, and the
-//   Thomas factors (shared by every column) are precomputed once per r:
-//   m_i = lambda1 (1 - q^(i+2)) / (1 - q^(i+1)), q = lambda2 / lambda1, the
-//   closed form of the pivots of tridiag(-r, 1+2r, -r) (lambda1,2 the roots of
-//   t^2 - (1+2r) t + r^2), so all n pivots are computed independently.
-//   The forward pass writes d' to a scratch grid and the backward pass
-//   (a separate launch, so no wave reads a neighbour's column while another
-//   overwrites it) writes u'.
+// Both sweeps run on the register CR-PCR kernels in their constant-
+// coefficient mode (only d is read, x written: 16 bytes per unknown in FP64).
+// * row sweep: the right-hand side is one elementwise pass (adi_rhs_rows),
+//   then the row solve;
+// * column sweep: the right-hand side is formed and transposed in one tiled
+//   pass (adi_rhs_cols_t: 64 x 64 tiles through LDS, reads and writes
+//   coalesced), the columns are solved as the rows of the transposed grid,
+//   and a tiled transpose writes u'. A direct column solve (one lane per
+//   column, n/64 waves) leaves the GPU mostly idle; the transposes cost two
+//   streaming passes and keep every pass at full width.
 #include "trid_device.hpp"
 #include "trid_launch.hpp"
 
 namespace trid {
+
+namespace {
+constexpr int kTile = 64;
+}
 
 // D = u + r * (u[i-1][j] - 2 u[i][j] + u[i+1][j]), zero outside the grid.
 __global__ __launch_bounds__(256) void adi_rhs_rows_kernel(const double* __restrict__ U,
@@ -41,84 +42,54 @@ __global__ __launch_bounds__(256) void adi_rhs_rows_kernel(const double* __restr
     }
 }
 
-// Pivot factors of tridiag(-r, 1+2r, -r): inv_m[i] = 1/m_i, cp[i] = -r/m_i.
-__global__ __launch_bounds__(256) void adi_factors_kernel(double* __restrict__ cp,
-                                                          double* __restrict__ inv_m, int64_t n,
-                                                          double r)
+// Dt[j][i] = u[i][j] + r * (u[i][j-1] - 2 u[i][j] + u[i][j+1]) for one 64 x 64
+// tile: rows i0.., columns j0-1 .. j0+64 are read (coalesced along j) into
+// LDS, the tile is written transposed (coalesced along i). Row pitch 67
+// doubles: the 32 lanes of one LDS pass read tile[tx][c] on disjoint banks.
+__global__ __launch_bounds__(256) void adi_rhs_cols_t_kernel(const double* __restrict__ U,
+                                                             double* __restrict__ Dt, int64_t n,
+                                                             double r)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double b = 1.0 + 2.0 * r;
-    const double s = sqrt(1.0 + 4.0 * r);  // sqrt(b^2 - 4 r^2)
-    const double l1 = 0.5 * (b + s), l2 = 0.5 * (b - s);
-    const double q = l2 / l1;
-    const double qi1 = pow(q, (double)(i + 1));
-    const double m = l1 * (1.0 - qi1 * q) / (1.0 - qi1);
-    inv_m[i] = 1.0 / m;
-    cp[i] = (i < n - 1) ? -r / m : 0.0;
-}
-
-// Column sweep, forward: lane j walks column j of u*, forming
-// d_i = u*[i][j] + r (u*[i][j-1] - 2 u*[i][j] + u*[i][j+1]) and
-// dp_i = (d_i + r dp_{i-1}) / m_i (a = -r). dp goes to P.
-__global__ __launch_bounds__(256) void adi_cols_forward_kernel(const double* __restrict__ U,
-                                                               double* __restrict__ P,
-                                                               const double* __restrict__ inv_m,
-                                                               int64_t n, double r)
-{
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    constexpr int K = 8;  // rows read ahead
-    double dp = 0.0;
-    for (int64_t i0 = 0; i0 < n; i0 += K) {
-        double d[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int64_t i = i0 + k;
-            if (i < n) {
-                const int64_t g = i * n + j;
-                const double u = U[g];
-                const double ul = j > 0 ? U[g - 1] : 0.0;
-                const double ur = j < n - 1 ? U[g + 1] : 0.0;
-                d[k] = u + r * (ul - 2.0 * u + ur);
-            }
+    __shared__ double tile[kTile][kTile + 3];
+    const int tx = threadIdx.x & (kTile - 1);
+    const int ty = threadIdx.x / kTile;  // 0..3
+    const int64_t i0 = (int64_t)blockIdx.y * kTile;
+    const int64_t j0 = (int64_t)blockIdx.x * kTile;
+    for (int row = ty; row < kTile; row += 4) {
+        const int64_t i = i0 + row;
+        for (int c = tx; c < kTile + 2; c += kTile) {
+            const int64_t j = j0 - 1 + c;
+            tile[row][c] = (i < n && j >= 0 && j < n) ? U[i * n + j] : 0.0;
         }
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int64_t i = i0 + k;
-            if (i < n) {
-                dp = (d[k] + r * dp) * inv_m[i];
-                P[i * n + j] = dp;
-            }
+    }
+    __syncthreads();
+    const int64_t i = i0 + tx;
+    for (int jj = ty; jj < kTile; jj += 4) {
+        const int64_t j = j0 + jj;
+        if (i < n && j < n) {
+            const double u = tile[tx][jj + 1];
+            Dt[j * n + i] = u + r * (tile[tx][jj] - 2.0 * u + tile[tx][jj + 2]);
         }
     }
 }
 
-// Column sweep, backward: x_i = dp_i - cp_i x_{i+1}, written to X.
-__global__ __launch_bounds__(256) void adi_cols_backward_kernel(const double* __restrict__ P,
-                                                                double* __restrict__ X,
-                                                                const double* __restrict__ cp,
-                                                                int64_t n)
+// U[i][j] = Xt[j][i], 64 x 64 tiles through LDS (pitch 65 doubles).
+__global__ __launch_bounds__(256) void adi_transpose_kernel(const double* __restrict__ Xt,
+                                                            double* __restrict__ U, int64_t n)
 {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    constexpr int K = 8;
-    double x = 0.0;
-    for (int64_t i0 = n - 1; i0 >= 0; i0 -= K) {
-        double p[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int64_t i = i0 - k;
-            if (i >= 0) p[k] = P[i * n + j];
-        }
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int64_t i = i0 - k;
-            if (i >= 0) {
-                x = p[k] - cp[i] * x;
-                X[i * n + j] = x;
-            }
-        }
+    __shared__ double tile[kTile][kTile + 1];
+    const int tx = threadIdx.x & (kTile - 1);
+    const int ty = threadIdx.x / kTile;
+    const int64_t j0 = (int64_t)blockIdx.y * kTile;  // rows of Xt
+    const int64_t i0 = (int64_t)blockIdx.x * kTile;  // columns of Xt
+    for (int jj = ty; jj < kTile; jj += 4) {
+        const int64_t j = j0 + jj, i = i0 + tx;
+        if (j < n && i < n) tile[jj][tx] = Xt[j * n + i];
+    }
+    __syncthreads();
+    for (int ii = ty; ii < kTile; ii += 4) {
+        const int64_t i = i0 + ii, j = j0 + tx;
+        if (i < n && j < n) U[i * n + j] = tile[tx][ii];
     }
 }
 
@@ -128,20 +99,11 @@ hipError_t launch_adi_step(const AdiArgs& g)
     const int64_t total = n * n;
     int64_t blocks = (total + 255) / 256;
     if (blocks > 65536) blocks = 65536;
-    const unsigned colblocks = (unsigned)((n + 255) / 256);
+    const int64_t tiles = (n + kTile - 1) / kTile;
+    if (tiles > 65535) return hipErrorInvalidConfiguration;
     hipError_t e;
-    // factors of this r (shared by every column)
-    hipLaunchKernelGGL(adi_factors_kernel, dim3(colblocks), dim3(256), 0, g.stream, g.cp,
-                       g.inv_m, n, g.r);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    // row sweep: rhs into T, solve T -> U (u*)
-    hipLaunchKernelGGL(adi_rhs_rows_kernel, dim3((unsigned)blocks), dim3(256), 0, g.stream, g.u,
-                       g.t, n, g.r);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
     RegArgs<double> ra{};
     ra.a = ra.b = ra.c = nullptr;
-    ra.d = g.t;
-    ra.x = g.u;
     ra.n = n;
     ra.N = n;
     ra.M = g.M;
@@ -154,14 +116,23 @@ hipError_t launch_adi_step(const AdiArgs& g)
     ra.a0 = -g.r;
     ra.b0 = 1.0 + 2.0 * g.r;
     ra.c0 = -g.r;
-    e = ra.L <= kWave ? launch_reg_crpcr<double>(ra) : launch_reg_mw<double>(ra);
-    if (e != hipSuccess) return e;
-    // column sweep: forward (U -> T as d'), backward (T -> U)
-    hipLaunchKernelGGL(adi_cols_forward_kernel, dim3(colblocks), dim3(256), 0, g.stream, g.u,
-                       g.t, g.inv_m, n, g.r);
+    auto solve = [&](const double* d, double* x) -> hipError_t {
+        ra.d = d;
+        ra.x = x;
+        return ra.L <= kWave ? launch_reg_crpcr<double>(ra) : launch_reg_mw<double>(ra);
+    };
+    // row sweep: rhs into T, solve T -> U (u*)
+    hipLaunchKernelGGL(adi_rhs_rows_kernel, dim3((unsigned)blocks), dim3(256), 0, g.stream, g.u,
+                       g.t, n, g.r);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(adi_cols_backward_kernel, dim3(colblocks), dim3(256), 0, g.stream, g.t,
-                       g.u, g.cp, n);
+    if ((e = solve(g.t, g.u)) != hipSuccess) return e;
+    // column sweep: rhs transposed into T, solve T -> T2, transpose T2 -> U
+    hipLaunchKernelGGL(adi_rhs_cols_t_kernel, dim3((unsigned)tiles, (unsigned)tiles), dim3(256),
+                       0, g.stream, g.u, g.t, n, g.r);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = solve(g.t, g.t2)) != hipSuccess) return e;
+    hipLaunchKernelGGL(adi_transpose_kernel, dim3((unsigned)tiles, (unsigned)tiles), dim3(256), 0,
+                       g.stream, g.t2, g.u, n);
     return hipGetLastError();
 }
 

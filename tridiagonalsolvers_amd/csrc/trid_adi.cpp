@@ -1,7 +1,9 @@
 // trid_adi.cpp -- resident ADI grids: Peaceman-Rachford steps of the 2D heat
 // equation (kern_adi.hip) on a device grid that stays resident across steps.
 // No counterpart in the reference; this is the caller the batched solve was
-// built for (n row systems, n strided column systems per half step).
+// built for (n row systems, n strided column systems per half step). Two
+// scratch grids: the transposed right-hand side and the transposed solution
+// of the column sweep.
 #include <hip/hip_runtime.h>
 
 #include <new>
@@ -14,7 +16,7 @@ struct trid_adi_s {
     int64_t n = 0;
     double* u = nullptr;
     double* t = nullptr;
-    double* f = nullptr;  // cp | inv_m
+    double* t2 = nullptr;
     bool owns_u = false;
     trid::Plan plan{};
 };
@@ -43,7 +45,7 @@ int trid_adi_create(trid_adi* out, int64_t n, double* u_device)
         a->owns_u = e == hipSuccess;
     }
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&a->t), grid);
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&a->f), 2 * (size_t)n * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&a->t2), grid);
     if (e != hipSuccess) {
         trid_adi_destroy(a);
         return TRID_ENOMEM;
@@ -57,7 +59,7 @@ int trid_adi_destroy(trid_adi a)
     if (!a) return TRID_EINVAL;
     if (a->owns_u && a->u) (void)hipFree(a->u);
     if (a->t) (void)hipFree(a->t);
-    if (a->f) (void)hipFree(a->f);
+    if (a->t2) (void)hipFree(a->t2);
     delete a;
     return TRID_OK;
 }
@@ -70,15 +72,15 @@ int trid_adi_step(trid_adi a, double r, int steps, void* stream)
     trid::AdiArgs g{};
     g.u = a->u;
     g.t = a->t;
-    g.cp = a->f;
-    g.inv_m = a->f + a->n;
+    g.t2 = a->t2;
     g.n = a->n;
     g.r = r;
     g.M = a->plan.M;
     g.L = a->plan.L;
     g.rest = a->plan.rest;
     g.vec = a->plan.dense && ((reinterpret_cast<uintptr_t>(a->u) |
-                               reinterpret_cast<uintptr_t>(a->t)) & 15u) == 0;
+                               reinterpret_cast<uintptr_t>(a->t) |
+                               reinterpret_cast<uintptr_t>(a->t2)) & 15u) == 0;
     g.stream = static_cast<hipStream_t>(stream);
     for (int s = 0; s < steps; ++s) {
         if (trid::launch_adi_step(g) != hipSuccess) return TRID_EHIP;

@@ -181,8 +181,7 @@ hipError_t launch_generate(int kind, uint64_t seed, T* a, T* b, T* c, T* d, int6
 struct AdiArgs {
     double* u;      // grid, row-major, n*n (input and output)
     double* t;      // scratch grid, n*n
-    double* cp;     // n column-sweep factors
-    double* inv_m;  // n column-sweep inverse pivots
+    double* t2;     // second scratch grid, n*n
     int64_t n;
     double r;
     int M, L, rest, vec;  // register-kernel plan of the row sweep (n systems of n)

@@ -74,7 +74,7 @@ struct Req {
 
 struct Adi {
     int64_t n = 0;
-    char* dev = nullptr;  // u | t | cp | inv_m, inside the pool
+    char* dev = nullptr;  // u | t | t2 (grids), inside the pool
     size_t pool_off = 0;
     size_t bytes = 0;
     trid::Plan plan{};
@@ -284,8 +284,7 @@ int trid_server_s::launch(Slot& s)
         const size_t grid = (size_t)a.n * a.n * sizeof(double);
         g.u = reinterpret_cast<double*>(a.dev);
         g.t = reinterpret_cast<double*>(a.dev + grid);
-        g.cp = reinterpret_cast<double*>(a.dev + 2 * grid);
-        g.inv_m = g.cp + a.n;
+        g.t2 = reinterpret_cast<double*>(a.dev + 2 * grid);
         g.n = a.n;
         g.r = q->r.adi_r;
         g.M = a.plan.M;
@@ -679,7 +678,7 @@ int trid_server_create_adi(trid_server s, int64_t n, const double* u0, int32_t* 
     if (st != TRID_OK) return st;
     if (a.plan.family != trid::Family::Reg) return TRID_ENOTSUP;
     const size_t grid = (size_t)n * n * sizeof(double);
-    a.bytes = align256(2 * grid + 2 * (size_t)n * sizeof(double));
+    a.bytes = align256(3 * grid);
     std::lock_guard<std::mutex> lk(s->mu);
     if (a.bytes > s->budget_free()) return TRID_ENOMEM;
     size_t off = 0;
