@@ -1,0 +1,44 @@
+"""bench.py's output contract on the GPU: one JSON line per run with the
+metric, the roofline object (achieved / peak / frac over the launch-stream
+kernel time) and the CPU baseline, at a reduced size so it runs in seconds."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+KEYS = ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step",
+        "higher_is_better", "scaling", "vs_baseline", "dtype", "data", "config")
+
+
+def _run(args, timeout=240):
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), *args],
+                       capture_output=True, text=True, timeout=timeout, cwd=ROOT)
+    return r
+
+
+def _json(r):
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert len(lines) == 1, r.stdout[-2000:]
+    return json.loads(lines[0])
+
+
+def test_uniform_line_has_roofline_and_cpu_baseline():
+    j = _json(_run(["--n", "1024", "--systems", "2048", "--steps", "3", "--warmup", "1",
+                    "--cpu-seconds", "0.2"]))
+    for k in KEYS:
+        assert k in j, k
+    assert j["n_gpus"] == 1 and j["unit"] == "unknowns/s" and j["value"] > 0
+    rf = j["roofline"]
+    assert rf["bound"] == "hbm" and rf["unit"] == "GB/s" and rf["peak"] == 8000.0
+    assert abs(rf["frac"] - rf["achieved"] / rf["peak"]) < 1e-9
+    cb = j["cpu_baseline"]
+    assert cb["kind"] in ("port", "reference") and cb["cores"] >= 1 and cb["value"] > 0
+    assert cb["gpu_max_rel_err_vs_fp64_thomas"] <= 1e-4
+    assert "workload" in j["config"]
+
