@@ -218,9 +218,10 @@ int interleaved_impl(trid_context* h, const T* a, const T* b, const T* c, const 
                      int64_t n, int64_t N, int64_t stride, hipStream_t stream)
 {
     if (!h || !a || !b || !c || !d || !x) return TRID_EINVAL;
-    if (n < 2 || N < 0 || stride < N || stride < 1) return TRID_EINVAL;
+    if (n < 2 || N < 0) return TRID_EINVAL;
     if (x == a || x == b || x == c) return TRID_EINVAL;
     if (N == 0) return TRID_OK;
+    if (stride < N) return TRID_EINVAL;
     if (n > INT64_MAX / stride / (int64_t)sizeof(T)) return TRID_EINVAL;
     int st = check_device();
     if (st != TRID_OK) return st;
