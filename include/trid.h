@@ -20,6 +20,7 @@
 #ifndef TRID_H
 #define TRID_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -386,6 +387,12 @@ int trid_server_submit(trid_server server, const trid_request* request, uint64_t
  * state is reported the ticket is retired. */
 int trid_server_wait(trid_server server, uint64_t ticket, int64_t timeout_us,
                      trid_request_result* result);
+/* Page-locked host memory for request arrays (a, b, c, d, x): arrays that lie
+ * inside such a buffer are copied to and from the device by DMA directly,
+ * with no staging copy. Free a buffer only after every request that uses it
+ * has been waited for; trid_server_destroy frees the remaining ones. */
+int trid_server_host_alloc(trid_server server, size_t bytes, void** ptr);
+int trid_server_host_free(trid_server server, void* ptr);
 /* Cancels a request that is still queued; TRID_EINVAL once it is running. */
 int trid_server_cancel(trid_server server, uint64_t ticket);
 int trid_server_get_stats(trid_server server, trid_server_stats* stats);

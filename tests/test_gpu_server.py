@@ -112,3 +112,35 @@ def test_server_adi_grid(oracle, trid):
     assert np.max(np.abs(u - f * u0)) <= 1e-12
     assert srv.release_adi(g) == 0
     srv.close()
+
+
+def test_server_host_buffers_match_pageable(oracle, trid):
+    # requests whose arrays live in the server's page-locked memory (DMA,
+    # no staging copy) give the same bits as pageable ones, also mixed in
+    # one batch
+    import tridiagonalsolvers_amd.server as S
+    srv = S.Server(hbm_budget_bytes=2 << 30, slots=2, batch_window_us=2000)
+    rng = np.random.default_rng(11)
+    reqs = [_request(oracle, rng, 500 + i) for i in range(12)]
+    xs_page = [np.full(int(o[-1]), np.nan) for o, _ in reqs]
+    for i, (offs, (a, b, c, d)) in enumerate(reqs):
+        srv.wait(srv.submit(d, xs_page[i], offs, a, b, c))
+    pinned, tickets = [], []
+    for i, (offs, arrs) in enumerate(reqs):
+        n = int(offs[-1])
+        pa = [srv.host_array(n) for _ in range(5)]
+        for dst, src in zip(pa, arrs):
+            dst[:] = src
+        pa[4][:] = np.nan
+        pinned.append(pa)
+        if i % 3 == 2:   # some requests keep pageable inputs in the same batches
+            a, b, c, d = arrs
+        else:
+            a, b, c, d = pa[:4]
+        tickets.append(srv.submit(d, pa[4], offs, a, b, c))
+    for t in tickets:
+        assert srv.wait(t)["state"] == "done"
+    for i, (offs, arrs) in enumerate(reqs):
+        assert np.array_equal(pinned[i][4], xs_page[i]), i
+        _check(oracle, offs, arrs, pinned[i][4])
+    srv.close()

@@ -150,3 +150,16 @@ def test_adi_requests_are_serialized_per_grid(S):
             assert b["latency_us"] - b["queue_us"] >= 0
     assert srv.release_adi(g1) == 0 and srv.release_adi(g2) == 0
     srv.close()
+
+
+def test_host_buffers_fake_executor(S):
+    # page-locked request memory: requests inside it are served like any
+    # other; frees are checked
+    srv = S.Server(hbm_budget_bytes=64 << 20, fake=True)
+    a, b, c, d, x = (srv.host_array(1000) for _ in range(5))
+    t = srv.submit(d, x, np.array([0, 400, 1000]), a, b, c)
+    assert srv.wait(t)["state"] == "done"
+    srv.host_free(a)
+    with pytest.raises(ValueError):
+        srv.host_free(a)
+    srv.close()

@@ -33,7 +33,8 @@ EXPORTS = (
     "trid_ragged_solve_f64",
     "trid_server_create", "trid_server_destroy", "trid_server_register_operator",
     "trid_server_release_operator", "trid_server_submit", "trid_server_wait",
-    "trid_server_cancel", "trid_server_get_stats",
+    "trid_server_cancel", "trid_server_get_stats", "trid_server_host_alloc",
+    "trid_server_host_free",
     "trid_adi_create", "trid_adi_destroy", "trid_adi_grid", "trid_adi_step",
     "trid_server_create_adi", "trid_server_read_adi", "trid_server_release_adi",
 )
@@ -183,6 +184,10 @@ def lib() -> ctypes.CDLL:
     L.trid_server_create_adi.argtypes = [P, i64, P, ctypes.POINTER(ctypes.c_int32)]
     L.trid_server_read_adi.argtypes = [P, ctypes.c_int32, P]
     L.trid_server_release_adi.argtypes = [P, ctypes.c_int32]
+    L.trid_server_host_alloc.argtypes = [P, ctypes.c_size_t, ctypes.POINTER(P)]
+    L.trid_server_host_alloc.restype = i32
+    L.trid_server_host_free.argtypes = [P, P]
+    L.trid_server_host_free.restype = i32
     for name in ("trid_server_create_adi", "trid_server_read_adi", "trid_server_release_adi"):
         getattr(L, name).restype = i32
     for name in ("trid_server_create", "trid_server_destroy", "trid_server_register_operator",

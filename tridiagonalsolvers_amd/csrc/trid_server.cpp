@@ -35,6 +35,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -70,6 +71,7 @@ struct Req {
     int64_t batch = -1;
     int64_t batch_systems = 0;
     int64_t pos = 0;  // element offset inside its batch
+    bool x_direct = false;  // result copied straight into a client host buffer
 };
 
 struct Adi {
@@ -224,6 +226,17 @@ struct trid_server_s {
     std::vector<Slot> slots;
     Pool pool;
     char* d_pool = nullptr;
+    // page-locked client buffers handed out by trid_server_host_alloc:
+    // request arrays inside them are copied by DMA with no staging copy
+    std::map<const char*, size_t> host_bufs;
+    bool in_host_buf(const void* p, size_t bytes) const
+    {
+        const char* c = static_cast<const char*>(p);
+        auto it = host_bufs.upper_bound(c);
+        if (it == host_bufs.begin()) return false;
+        --it;
+        return c >= it->first && c + bytes <= it->first + it->second;
+    }
     size_t ops_bytes = 0;
     size_t in_use = 0;
     trid_server_stats st{};
@@ -327,23 +340,44 @@ int trid_server_s::launch(Slot& s)
     double* pc = reinterpret_cast<double*>(P + 2 * L.arr);
     double* pd = reinterpret_cast<double*>(P + 3 * L.arr);
     char* pdesc = P + 5 * L.arr;
+    // arrays in client page-locked buffers go to the device by DMA straight
+    // from there; the others are copied into the slot's staging block first
+    struct Dma {
+        int arr;
+        int64_t pos;
+        const void* src;
+        size_t bytes;
+    };
     std::vector<CopyJob> jobs;
+    std::vector<Dma> dmas;
+    bool staged_any = false;
     for (Req* q : s.reqs) {
         const size_t nb = (size_t)q->unknowns * sizeof(double);
-        if (q->r.op < 0) {
-            jobs.push_back(CopyJob{pa + q->pos, q->r.a, nb});
-            jobs.push_back(CopyJob{pb + q->pos, q->r.b, nb});
-            jobs.push_back(CopyJob{pc + q->pos, q->r.c, nb});
+        const double* src[4] = {q->r.a, q->r.b, q->r.c, q->r.d};
+        double* stg[4] = {pa, pb, pc, pd};
+        for (int a = q->r.op < 0 ? 0 : 3; a < 4; ++a) {
+            if (in_host_buf(src[a], nb)) {
+                dmas.push_back(Dma{a, q->pos, src[a], nb});
+            } else {
+                jobs.push_back(CopyJob{stg[a] + q->pos, src[a], nb});
+                dmas.push_back(Dma{a, q->pos, stg[a] + q->pos, nb});
+                staged_any = true;
+            }
         }
-        jobs.push_back(CopyJob{pd + q->pos, q->r.d, nb});
     }
     parallel_copy(jobs);
+    (void)staged_any;
     std::memcpy(pdesc, hp.sys.data(), sizeof(trid::RaggedSys) * N);
     std::memcpy(pdesc + L.sys, hp.items.data(), sizeof(trid::RaggedItem) * hp.items.size());
     std::memcpy(pdesc + L.sys + L.itm, hp.buckets.data(),
                 sizeof(trid::RaggedBucket) * hp.buckets.size());
     char* D = d_pool + s.dev_off;
-    hipError_t e = hipMemcpyAsync(D, P, 4 * L.arr, hipMemcpyHostToDevice, s.stream);
+    hipError_t e = hipSuccess;
+    for (const Dma& m : dmas) {
+        if (e != hipSuccess) break;
+        e = hipMemcpyAsync(D + m.arr * L.arr + m.pos * sizeof(double), m.src, m.bytes,
+                           hipMemcpyHostToDevice, s.stream);
+    }
     if (e == hipSuccess)
         e = hipMemcpyAsync(D + 5 * L.arr, pdesc, L.sys + L.itm + L.bkt,
                            hipMemcpyHostToDevice, s.stream);
@@ -380,9 +414,16 @@ int trid_server_s::launch(Slot& s)
     const int64_t work = (int64_t)hp.n_wg_items + (hp.n_wave_items + 15) / 16;
     if (work < grid_now) grid_now = (int)std::max<int64_t>(1, work);
     e = trid::launch_ragged(g, grid_now, s.stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(P + 4 * L.arr, D + 4 * L.arr, (size_t)total * sizeof(double),
+    // results: straight into client page-locked buffers, else via staging
+    for (Req* q : s.reqs) {
+        if (e != hipSuccess) break;
+        const size_t nb = (size_t)q->unknowns * sizeof(double);
+        q->x_direct = in_host_buf(q->r.x, nb);
+        void* dst = q->x_direct ? static_cast<void*>(q->r.x)
+                                : static_cast<void*>(P + 4 * L.arr + q->pos * sizeof(double));
+        e = hipMemcpyAsync(dst, D + 4 * L.arr + q->pos * sizeof(double), nb,
                            hipMemcpyDeviceToHost, s.stream);
+    }
     if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
     return e == hipSuccess ? TRID_OK : TRID_EHIP;
 }
@@ -395,7 +436,9 @@ void trid_server_s::retire(Slot& s, bool ok)
         const double* px = reinterpret_cast<const double*>(s.pinned + 4 * L.arr);
         std::vector<CopyJob> jobs;
         for (Req* q : s.reqs)
-            jobs.push_back(CopyJob{q->r.x, px + q->pos, (size_t)q->unknowns * sizeof(double)});
+            if (!q->x_direct)
+                jobs.push_back(
+                    CopyJob{q->r.x, px + q->pos, (size_t)q->unknowns * sizeof(double)});
         parallel_copy(jobs);
     }
     for (Req* q : s.reqs) finish(q, ok ? kDone : kFailed, ok ? TRID_OK : TRID_EHIP);
@@ -616,8 +659,43 @@ int trid_server_destroy(trid_server s)
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
     if (s->d_pool) (void)hipFree(s->d_pool);
+    for (auto& kv : s->host_bufs) {
+        if (s->fake) std::free(const_cast<char*>(kv.first));
+        else (void)hipHostFree(const_cast<char*>(kv.first));
+    }
     for (auto& kv : s->reqs) delete kv.second;
     delete s;
+    return TRID_OK;
+}
+
+int trid_server_host_alloc(trid_server s, size_t bytes, void** ptr)
+{
+    if (!s || !ptr || bytes == 0) return TRID_EINVAL;
+    *ptr = nullptr;
+    void* p = nullptr;
+    if (s->fake) {
+        p = std::aligned_alloc(256, (bytes + 255) & ~(size_t)255);
+        if (!p) return TRID_ENOMEM;
+    } else if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        return TRID_ENOMEM;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->host_bufs[static_cast<const char*>(p)] = bytes;
+    *ptr = p;
+    return TRID_OK;
+}
+
+int trid_server_host_free(trid_server s, void* ptr)
+{
+    if (!s || !ptr) return TRID_EINVAL;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->host_bufs.find(static_cast<const char*>(ptr));
+        if (it == s->host_bufs.end()) return TRID_EINVAL;
+        s->host_bufs.erase(it);
+    }
+    if (s->fake) std::free(ptr);
+    else if (hipHostFree(ptr) != hipSuccess) return TRID_EHIP;
     return TRID_OK;
 }
 

@@ -45,7 +45,26 @@ class Server:
         self._h = h
         self._keep = {}
         self._ops = {}
+        self._host = {}
         self._mu = threading.Lock()
+
+    # -- page-locked request memory ---------------------------------------------
+    def host_array(self, n: int) -> np.ndarray:
+        """n float64 values of page-locked memory owned by the server (valid
+        until host_free or close). Request arrays placed inside such memory
+        go to and from the device by DMA with no staging copy."""
+        p = ctypes.c_void_p()
+        check(lib().trid_server_host_alloc(self._h, int(n) * 8, ctypes.byref(p)),
+              "trid_server_host_alloc")
+        arr = np.ctypeslib.as_array((ctypes.c_double * int(n)).from_address(p.value))
+        self._host[arr.ctypes.data] = p.value
+        return arr
+
+    def host_free(self, arr: np.ndarray) -> None:
+        p = self._host.pop(arr.ctypes.data, None)
+        if p is None:
+            raise ValueError("not an array from host_array")
+        check(lib().trid_server_host_free(self._h, ctypes.c_void_p(p)), "trid_server_host_free")
 
     # -- operators ---------------------------------------------------------
     def register_operator(self, sizes, a, b, c) -> int:
@@ -141,9 +160,10 @@ class Server:
 
     def close(self):
         if self._h:
-            lib().trid_server_destroy(self._h)
+            lib().trid_server_destroy(self._h)   # frees the host arrays too
             self._h = None
             self._keep.clear()
+            self._host.clear()
 
     def __enter__(self):
         return self
