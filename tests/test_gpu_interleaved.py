@@ -28,8 +28,10 @@ def test_interleaved_matches_oracle(oracle, trid, dtype, kind):
     h = trid.Handle()
     # (rows, systems, stride): odd sizes, stride above the system count,
     # one system, two rows, chains whose halves differ by one row
+    # (the first, (4096, 64, 64), (301, 4096, 4128) and (2049, 96, 96) run the
+    # staged form: system counts in whole groups of 32, 256 rows and more)
     for n, N, stride in ((1024, 4096, 4096), (1000, 3000, 3100), (2, 5, 5), (3, 1, 7),
-                         (257, 700, 700), (4096, 64, 64)):
+                         (257, 700, 700), (4096, 64, 64), (301, 4096, 4128), (2049, 96, 96)):
         a, b, c, d = oracle.generate(kind, 17 + n, n, N, dtype)
         dev = [torch.from_numpy(_interleave(v, n, N, stride)).cuda() for v in (a, b, c, d)]
         x = torch.full_like(dev[0], float("nan"))

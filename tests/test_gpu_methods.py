@@ -108,6 +108,23 @@ def test_pcrgauss_two_pass_form(oracle, trid, dtype, n):
     h.close()
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_pcrgauss_staged_long_chains(oracle, trid, dtype):
+    # subsystems of ~1000 rows and more (the staged second pass), including
+    # n not a multiple of G (subsystems of unequal length) and several systems
+    import torch
+    h = trid.Handle()
+    for n, N, G in ((1 << 16, 3, 64), ((1 << 16) + 3, 2, 64), (40000, 2, 32), (1 << 17, 2, 128)):
+        a, b, c, d = oracle.generate("dominant", n + G, n, N, dtype)
+        dev = [torch.from_numpy(v).cuda() for v in (a, b, c, d)]
+        x = torch.full_like(dev[0], float("nan"))
+        h.solve("pcrgauss", *dev, x, n, N, trid.make_opts(gaussian=G, flags=1))
+        torch.cuda.synchronize()
+        print(f"n={n} N={N} G={G}", flush=True)
+        check(oracle, x.cpu().numpy(), a, b, c, d, n, dtype)
+    h.close()
+
+
 def test_pcrgauss_million_unknowns(oracle, trid):
     # the long-system form (global PCR pass + strided Thomas) at n = 2^20
     import torch

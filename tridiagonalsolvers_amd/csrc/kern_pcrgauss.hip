@@ -355,14 +355,250 @@ __global__ __launch_bounds__(64) void thomas_strided_kernel(const T* A, const T*
     }
 }
 
+// Staged form of the strided elimination for chains that are long and few
+// (the pcrGauss second pass at large depth). The direct kernel above keeps
+// at most ~63 memory operations in flight per wave, and with one lane per
+// half-chain the whole grid then keeps only a few MB in flight: the latency
+// of HBM, not its bandwidth, sets the pace. Here a workgroup of 5 waves
+// serves 32 consecutive chains (of one block): wave 0 computes the same
+// two-ended elimination as thomas_strided_kernel, reading its operands from
+// LDS, and waves 1-4 stream the rows into LDS, kRowsBlk rows of every
+// half-chain per step, ahead of the compute wave. In the forward sweep each
+// loader half-wave owns one (array, half) pair (4 arrays x 2 halves); in the
+// backward sweep two half-waves share each (factor, half) pair, 8 rows each.
+// One LDS-only barrier per step (the loads of later steps stay in flight
+// across it); two LDS slots alternate (a slot is rewritten two steps after
+// the compute wave read it, and the compute wave has passed the barrier in
+// between). The factors go to FC/FD as in the direct kernel; the
+// backward sweep streams them back after an agent-scope fence.
+// Chains of at least this many rows use the staged form (0: never), when
+// there are at most TRID_STAGED_MAX_WG_PER_CU workgroups of 32 chains per
+// CU. Measured on 64 x 2^20 FP32 (second pass, staged against direct):
+// 2,048 chains 2.62 / 3.81 ms, 8,192 chains 1.20 / 1.47 ms, 16,384 chains
+// 1.24 / 1.21 ms; interleaved 4,096 x 8,192 0.59 / 0.87 ms but 65,536 x 1,024
+// 0.94 / 0.53 ms. With more chains than CUs x 32 the direct kernel has
+// enough lanes; the staged form pays for its barriers.
+#ifndef TRID_STAGED_MIN_ROWS
+#define TRID_STAGED_MIN_ROWS 256
+#endif
+#ifndef TRID_STAGED_MAX_WG_PER_CU
+#define TRID_STAGED_MAX_WG_PER_CU 1
+#endif
+#ifndef TRID_STAGED_PF
+#define TRID_STAGED_PF 2
+#endif
+
+namespace {
+constexpr int kRowsBlk = 16;       // rows of every half-chain per step
+constexpr int kStagedChains = 32;  // chains per workgroup
+constexpr int kStagedWaves = 5;    // 1 compute + 4 loader waves
+constexpr int kSlotElems = 4 * 2 * kRowsBlk * kStagedChains;  // [array][half][row][chain]
+constexpr int kArrStride = 2 * kRowsBlk * kStagedChains;
+}  // namespace
+
+// Workgroup barrier that orders LDS only: global loads and stores stay in
+// flight across it (__syncthreads would first wait for every outstanding
+// global access of the wave, draining the loaders' prefetch at each step).
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <typename T>
+__global__ __launch_bounds__(kStagedWaves * 64) void strided_staged_kernel(
+    const T* A, const T* B, const T* C, const T* D, T* FC, T* FD, T* X, int64_t block_len,
+    int64_t blocks, int64_t chains, int64_t stride)
+{
+    // steps of loads in flight per loader lane (forward, backward): enough
+    // rows in flight to cover the memory latency at this workgroup's share
+    // of the bandwidth, within the register budget
+    constexpr int PF = sizeof(T) == 4 ? TRID_STAGED_PF : TRID_STAGED_PF / 2;
+    constexpr int PB = 2 * PF;
+    constexpr int BR = kRowsBlk / 2;  // backward rows per loader lane and step
+    __shared__ T slot[2][kSlotElems];
+
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int cl = lane & 31;  // chain within the workgroup
+    const int64_t chain0 = (int64_t)blockIdx.x * kStagedChains;
+    const int64_t chain = chain0 + cl;
+    const int64_t s = chain / chains;
+    const int64_t r = chain % chains;
+    const bool live = chain < blocks * chains && r < block_len;
+    const int64_t m = live ? (block_len - r + stride - 1) / stride : 0;
+    const int64_t h = (m + 1) / 2;
+    // the step count follows the longest half-chain of the workgroup (its
+    // first chain), the same for every wave
+    const int64_t r0 = chain0 % chains;
+    const int64_t m0 = chain0 < blocks * chains ? (block_len - r0 + stride - 1) / stride : 0;
+    const int nsteps = (int)(((m0 + 1) / 2 + kRowsBlk - 1) / kRowsBlk);
+
+    if (wave == 0) {
+        // ---- compute wave ---------------------------------------------------
+        const int half = lane / 32;
+        const bool top = half == 0;
+        const int64_t len = top ? h : m - h;
+        const int64_t o0 = s * block_len + r + (top ? 0 : m - 1) * stride;
+        const int64_t step = top ? stride : -stride;
+        T qv = T(0), dv = T(0);
+        for (int blk = 0; blk < nsteps; ++blk) {
+            lds_barrier();
+            const T* op = slot[blk & 1] + half * kRowsBlk * kStagedChains + cl;
+#pragma unroll
+            for (int row = 0; row < kRowsBlk; ++row) {
+                const int64_t i = (int64_t)blk * kRowsBlk + row;
+                if (i < len) {
+                    const T pk = i > 0 ? op[row * kStagedChains] : T(0);
+                    const T bk = op[kArrStride + row * kStagedChains];
+                    const T qk = op[2 * kArrStride + row * kStagedChains];
+                    const T dk = op[3 * kArrStride + row * kStagedChains];
+                    const T den = bk - pk * qv;
+                    qv = tdiv(qk, den);
+                    dv = tdiv(dk - dv * pk, den);
+                    FC[o0 + i * step] = qv;
+                    FD[o0 + i * step] = dv;
+                }
+            }
+        }
+        const int partner = top ? lane + kWave / 2 : lane - kWave / 2;
+        const T oq = lane_read(qv, partner);
+        const T od = lane_read(dv, partner);
+        T xv;
+        if (top) {
+            xv = m - h > 0 ? tdiv(dv - qv * od, T(1) - qv * oq) : dv;
+        } else {
+            const T xo = tdiv(od - oq * dv, T(1) - oq * qv);
+            xv = dv - qv * xo;
+        }
+        if (len > 0) X[o0 + (len - 1) * step] = xv;
+        __threadfence();
+        __syncthreads();
+        for (int blk = 0; blk < nsteps; ++blk) {
+            lds_barrier();
+            const T* op = slot[blk & 1] + half * kRowsBlk * kStagedChains + cl;
+#pragma unroll
+            for (int row = 0; row < kRowsBlk; ++row) {
+                const int64_t i = len - 2 - ((int64_t)blk * kRowsBlk + row);
+                if (i >= 0) {
+                    xv = op[kArrStride + row * kStagedChains] - op[row * kStagedChains] * xv;
+                    X[o0 + i * step] = xv;
+                }
+            }
+        }
+        return;
+    }
+
+    // ---- loader waves ---------------------------------------------------------
+    const int lh = (wave - 1) * 2 + lane / 32;  // loader half-wave 0..7
+    {
+        // forward: this half-wave streams array lh / 2 of half lh % 2
+        const int arr = lh / 2;
+        const bool top = (lh & 1) == 0;
+        const int64_t len = top ? h : m - h;
+        const int64_t step = top ? stride : -stride;
+        const T* base = arr == 0 ? (top ? A : C) : arr == 1 ? B : arr == 2 ? (top ? C : A) : D;
+        const T* ptr = base + s * block_len + r + (top ? 0 : m - 1) * stride;
+        T* dst0 = &slot[0][0] + arr * kArrStride + (lh & 1) * kRowsBlk * kStagedChains + cl;
+        T pre[PF][kRowsBlk];
+        auto fetch = [&](int blk, T (&v)[kRowsBlk]) {
+#pragma unroll
+            for (int row = 0; row < kRowsBlk; ++row) {
+                const int64_t i = (int64_t)blk * kRowsBlk + row;
+                v[row] = (live && i < len) ? ptr[i * step] : T(0);
+            }
+        };
+#pragma unroll
+        for (int k = 0; k < PF; ++k) fetch(k, pre[k]);
+        for (int blk = 0; blk < nsteps; ++blk) {
+            T* dst = dst0 + (blk & 1) * kSlotElems;
+#pragma unroll
+            for (int row = 0; row < kRowsBlk; ++row) dst[row * kStagedChains] = pre[0][row];
+#pragma unroll
+            for (int k = 0; k + 1 < PF; ++k)
+#pragma unroll
+                for (int row = 0; row < kRowsBlk; ++row) pre[k][row] = pre[k + 1][row];
+            fetch(blk + PF, pre[PF - 1]);
+            lds_barrier();
+        }
+    }
+    // the factors the compute wave wrote become visible: its fence, the
+    // barrier, then this acquire (which also drops lines of C, D cached in
+    // the forward sweep, which the factors may have overwritten)
+    __syncthreads();
+    __threadfence();
+    {
+        // backward: factor (lh / 4: q', d') of half (lh / 2) % 2, rows
+        // (lh % 2) * BR .. + BR of every step
+        const int arr = lh / 4;
+        const bool top = ((lh / 2) & 1) == 0;
+        const int rofs = (lh & 1) * BR;
+        const int64_t len = top ? h : m - h;
+        const int64_t step = top ? stride : -stride;
+        const T* ptr = (arr == 0 ? FC : FD) + s * block_len + r + (top ? 0 : m - 1) * stride;
+        T* dst0 = &slot[0][0] + arr * kArrStride + ((lh / 2) & 1) * kRowsBlk * kStagedChains +
+                  rofs * kStagedChains + cl;
+        T pre[PB][BR];
+        auto fetch = [&](int blk, T (&v)[BR]) {
+#pragma unroll
+            for (int q = 0; q < BR; ++q) {
+                const int64_t i = len - 2 - ((int64_t)blk * kRowsBlk + rofs + q);
+                v[q] = (live && i >= 0) ? ptr[i * step] : T(0);
+            }
+        };
+#pragma unroll
+        for (int k = 0; k < PB; ++k) fetch(k, pre[k]);
+        for (int blk = 0; blk < nsteps; ++blk) {
+            T* dst = dst0 + (blk & 1) * kSlotElems;
+#pragma unroll
+            for (int q = 0; q < BR; ++q) dst[q * kStagedChains] = pre[0][q];
+#pragma unroll
+            for (int k = 0; k + 1 < PB; ++k)
+#pragma unroll
+                for (int q = 0; q < BR; ++q) pre[k][q] = pre[k + 1][q];
+            fetch(blk + PB, pre[PB - 1]);
+            lds_barrier();
+        }
+    }
+}
+
+// Compute units of the current device (cached per device).
+static int device_cus()
+{
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cached[dev] == 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                hipSuccess || cus <= 0)
+            cus = 256;
+        cached[dev] = cus;
+    }
+    return cached[dev];
+}
+
 template <typename T>
 static hipError_t launch_strided(const T* A, const T* B, const T* C, const T* D, T* FC, T* FD,
                                  T* X, int64_t block_len, int64_t blocks, int64_t chains,
                                  int64_t stride, hipStream_t stream)
 {
     const int64_t total = blocks * chains;  // two lanes each
-    const int64_t grid = (total + kWave / 2 - 1) / (kWave / 2);
     if (total <= 0) return hipSuccess;
+    // long chains, whole groups of 32 per block: the staged form
+    const int64_t rows = block_len / stride;
+    if (TRID_STAGED_MIN_ROWS > 0 && chains % kStagedChains == 0 &&
+        rows >= TRID_STAGED_MIN_ROWS &&
+        total / kStagedChains <= (int64_t)TRID_STAGED_MAX_WG_PER_CU * device_cus()) {
+        const int64_t grid = total / kStagedChains;
+        if (grid > 0x7fffffffLL) return hipErrorInvalidConfiguration;
+        hipLaunchKernelGGL((strided_staged_kernel<T>), dim3((unsigned)grid),
+                           dim3(kStagedWaves * kWave), 0,
+                           stream, A, B, C, D, FC, FD, X, block_len, blocks, chains, stride);
+        return hipGetLastError();
+    }
+    const int64_t grid = (total + kWave / 2 - 1) / (kWave / 2);
     if (grid > 0x7fffffffLL) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL((thomas_strided_kernel<T, 8>), dim3((unsigned)grid), dim3(kWave), 0, stream,
                        A, B, C, D, FC, FD, X, block_len, blocks, chains, stride);

@@ -91,12 +91,12 @@ static_assert(ragged_lds_bytes<kRefineWide>() <= 160 * 1024, "LDS of the refine 
 // Fetching all four arrays before the first transpose keeps one memory
 // round trip per pass instead of four.
 template <int M>
-__device__ __forceinline__ void fetch_cols(const double* __restrict__ X, int64_t base, int64_t n,
+__device__ __forceinline__ void fetch_cols(const double* __restrict__ X, int64_t base, int n,
                                            int L, int ll, bool live, double fill, double (&t)[M])
 {
 #pragma unroll
     for (int q = 0; q < M; ++q) {
-        const int64_t j = (int64_t)q * L + ll;
+        const int j = q * L + ll;
         t[q] = (live && j < n) ? X[base + j] : fill;
     }
 }
@@ -139,7 +139,7 @@ __device__ __forceinline__ void transpose_rows(const double (&t)[M], int L, int 
 // (the 16384-unknown systems) the arrays go one at a time to bound the
 // registers.
 template <int M>
-__device__ __forceinline__ void gather4(const RaggedArgs& g, int64_t base, int64_t n, int L,
+__device__ __forceinline__ void gather4(const RaggedArgs& g, int64_t base, int n, int L,
                                         int ll, int lane, bool live, char* stage,
                                         double (&va)[M], double (&vb)[M], double (&vc)[M],
                                         double (&vd)[M])
@@ -174,7 +174,7 @@ __device__ __forceinline__ void gather4(const RaggedArgs& g, int64_t base, int64
 // [ll*M, ll*M+M) of the system through the LDS area, written with
 // coalesced stores (nothing past n).
 template <int M>
-__device__ __forceinline__ void scatter_rows(double* __restrict__ X, int64_t base, int64_t n,
+__device__ __forceinline__ void scatter_rows(double* __restrict__ X, int64_t base, int n,
                                              int L, int ll, int lane, bool live, char* stage,
                                              const double (&v)[M])
 {
@@ -214,7 +214,7 @@ __device__ __forceinline__ void scatter_rows(double* __restrict__ X, int64_t bas
 template <int M>
 __device__ __forceinline__ void ragged_rows(const RaggedArgs& g, const double (&x)[M],
                                             double x_left, double x_right, int64_t base,
-                                            int64_t nb, int64_t j0, int64_t n, int L, int ll,
+                                            int nb, int j0, int n, int L, int ll,
                                             int lane, bool live, char* stage, Eq<float> (&e)[M])
 {
     double r[M], v[M];
@@ -295,8 +295,8 @@ __device__ __forceinline__ void ragged_rows(const RaggedArgs& g, const double (&
 // FP64 operator of the lane's unknowns [j0, j0+M) (arguments as
 // ragged_rows). Identity rows past n; a[first] and c[last] ignored.
 template <int M>
-__device__ __forceinline__ void direct_rows(const RaggedArgs& g, int64_t base, int64_t nb,
-                                            int64_t j0, int64_t n, int L, int ll, int lane,
+__device__ __forceinline__ void direct_rows(const RaggedArgs& g, int64_t base, int nb,
+                                            int j0, int n, int L, int ll, int lane,
                                             bool live, char* stage, Eq<double> (&e)[M])
 {
     double va[M], vb[M], vc[M], vd[M];
@@ -314,13 +314,14 @@ __device__ void wave_item(const RaggedArgs& g, const RaggedItem& it, const Ragge
     const int ll = lane & (L - 1);
     const int seg = lane / L;
     const bool live = seg < it.count;
-    int64_t base = 0, n = 2;
+    int64_t base = 0;
+    int n = 2;
     if (live) {
         const RaggedSys sy = g.sys[it.first + seg];
         base = sy.base;
-        n = sy.n;
+        n = (int)sy.n;
     }
-    const int64_t j0 = (int64_t)ll * M;
+    const int j0 = ll * M;
     double x[M];
     if constexpr (DIRECT) {
         Eq<double> e[M];
@@ -357,15 +358,16 @@ __device__ void wg_item(const RaggedArgs& g, const RaggedItem& it, const RaggedB
     char* xchg = smem + grp * grp_bytes;
     double* edge = reinterpret_cast<double*>(xchg + wg_core_lds_bytes<TC, M>(W));
     const bool live = grp < it.count;
-    int64_t base = 0, n = 2;
+    int64_t base = 0;
+    int n = 2;
     if (live) {
         const RaggedSys sy = g.sys[it.first + grp];
         base = sy.base;
-        n = sy.n;
+        n = (int)sy.n;
     }
     // this wave's block of the system: unknowns [w*64*M, (w+1)*64*M)
-    const int64_t wb = (int64_t)w * kWave * M;
-    const int64_t j0 = wb + (int64_t)lane * M;
+    const int wb = w * kWave * M;
+    const int j0 = wb + lane * M;
     double x[M];
     if constexpr (DIRECT) {
         Eq<double> e[M];
