@@ -290,3 +290,43 @@ def test_adi_oracle_mode_closed_form(oracle, n):
         u1 = oracle.adi_step_np(u0, r)
         g = oracle.adi_mode_factor(n, p, q, r)
         assert np.max(np.abs(u1 - g * u0)) <= 1e-13 * max(1.0, abs(g))
+
+
+def _pcr_split_solve(oracle, a, b, c, d, n, G, levels):
+    """numpy restatement of `levels` PCR levels (pcr_kernel.cu:39-56, identity
+    rows outside the system), then Thomas on each subsystem {r, r+G, ...}."""
+    A, B, C, D = (v.astype(np.float64).copy() for v in (a, b, c, d))
+    A[0] = 0.0
+    C[-1] = 0.0
+    delta = 1
+    for _ in range(levels):
+        idx = np.arange(n)
+        lo, hi = idx - delta, idx + delta
+        okl, okh = lo >= 0, hi < n
+        li, hj = np.clip(lo, 0, n - 1), np.clip(hi, 0, n - 1)
+        aL, bL = np.where(okl, A[li], 0.0), np.where(okl, B[li], 1.0)
+        cL, dL = np.where(okl, C[li], 0.0), np.where(okl, D[li], 0.0)
+        aR, bR = np.where(okh, A[hj], 0.0), np.where(okh, B[hj], 1.0)
+        cR, dR = np.where(okh, C[hj], 0.0), np.where(okh, D[hj], 0.0)
+        k1, k2 = A / bL, C / bR
+        A, B, C, D = -aL * k1, B - cL * k1 - aR * k2, -cR * k2, D - dL * k1 - dR * k2
+        delta *= 2
+    x = np.empty(n)
+    for r in range(min(G, n)):
+        rows = np.arange(r, n, G)
+        x[rows] = oracle.thomas(A[rows], B[rows], C[rows], D[rows], n=len(rows))
+    return x
+
+
+@pytest.mark.parametrize("n,G", [(64, 4), (100, 8), (1024, 64), (37, 16)])
+def test_pcr_levels_split_into_independent_subsystems(oracle, n, G):
+    # the math pcrGauss rests on: after log2(G) PCR levels equation i couples
+    # only to i -+ G, so the subsystems {r, r+G, ...} solved on their own give
+    # the full solution; one level short, they do not
+    a, b, c, d = oracle.generate("weak", 9, n, 1, np.float64)
+    xref = oracle.thomas(a, b, c, d, n=n)
+    depth = G.bit_length() - 1
+    x = _pcr_split_solve(oracle, a, b, c, d, n, G, depth)
+    assert np.max(np.abs(x - xref)) <= 1e-10 * np.max(np.abs(xref))
+    x_short = _pcr_split_solve(oracle, a, b, c, d, n, G, depth - 1)
+    assert np.max(np.abs(x_short - xref)) > 1e-6 * np.max(np.abs(xref))
