@@ -30,10 +30,12 @@ def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--workload", default="uniform",
-                   choices=["uniform", "ragged", "serve", "gauss-sweep"],
+                   choices=["uniform", "ragged", "serve", "gauss-sweep", "pivoting"],
                    help="uniform: the headline batch; ragged: 100,000 systems of "
                         "log-uniform sizes in [16, 8192], FP32 solve + FP64 refinement; "
-                        "serve: request server with Poisson arrivals and resident ADI grids")
+                        "serve: request server with Poisson arrivals and resident ADI grids; "
+                        "pivoting: the partial-pivoting (LAPACK ?gtsv) solve of a batch "
+                        "that needs row interchanges")
     p.add_argument("--requests", type=int, default=64)
     p.add_argument("--ragged-mode", default="refine", choices=["refine", "fp64"],
                    help="ragged: FP32 solve + 2 FP64 refinements (the mixed-precision "
@@ -235,6 +237,151 @@ def bench_interleaved(args):
     if not args.no_cpu:
         xs = x.view(n, N).t().contiguous().view(-1)
         out["cpu_baseline"] = cpu_baseline(args, xs.cpu().numpy(), n, N)
+    h.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_pivoting(args):
+    """Partial-pivoting solve (trid_solve_pivoting_*, LAPACK ?gtsv
+    semantics) of the seeded "paired" batch, which is not diagonally dominant
+    and defeats the no-pivoting methods. System-major by default, or the
+    interleaved layout with --layout interleaved. Unless --no-vendor, the same
+    systems are also solved by rocSPARSE's gtsv_interleaved_batch with its
+    LU algorithm (partial pivoting) in the interleaved layout, the vendor's
+    batched pivoting solver. The CPU baseline is the oracle's ?gtsv
+    restatement (OpenMP over all host cores); the GPU result is compared with
+    it bit for bit."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    import tridiagonalsolvers_amd as T
+
+    n, N = args.n, args.systems
+    tdt = torch.float32 if args.dtype == "f32" else torch.float64
+    esize = 4 if args.dtype == "f32" else 8
+    stream = torch.cuda.current_stream()
+    sm = [torch.empty(n * N, dtype=tdt, device="cuda") for _ in range(4)]
+    T.generate(3, 1, *sm, n, N, stream=stream)
+    inter = args.layout == "interleaved"
+    if inter:
+        a, b, c, d = (v.view(N, n).t().contiguous().view(-1) for v in sm)
+    else:
+        a, b, c, d = sm
+    x = torch.empty_like(a)
+    info = torch.empty(N, dtype=torch.int32, device="cuda")
+    h = T.Handle()
+
+    def step():
+        h.solve_pivoting(a, b, c, d, x, n, N, info=info,
+                         batchStride=N if inter else None, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for e0, e1 in evs:
+        e0.record(stream)
+        step()
+        e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = [e0.elapsed_time(e1) for e0, e1 in evs]
+    avg = sum(kms) / len(kms)
+    alg = 5 * n * N * esize
+    layout = "interleaved" if inter else "system-major"
+    workload = f"pivoting {layout} {args.dtype} {N} systems x {n} unknowns"
+    out = {"metric": METRIC, "value": n * N * args.steps / wall, "unit": "unknowns/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": args.dtype,
+           "data": "synthetic (seeded device generator, kind 3: row pairs coupled by "
+                   "1+U, diagonals 0.001 U[-1,1]; needs row interchanges)",
+           "config": {"workload": workload, "systems": N, "system_size": n,
+                      "layout": layout},
+           "kernel_ms": {"avg": avg, "min": min(kms)},
+           "singular_systems": int((info != 0).sum().item()),
+           "roofline": {"bound": "hbm", "achieved": alg / (avg * 1e-3) / 1e9,
+                        "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": alg / (avg * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                        "traffic": traffic_from_profiles(workload),
+                        "note": "algorithmic bytes 5 x T per unknown; the checkpointed "
+                                "two-pass elimination moves 9 x T (a, b, c, d read twice, "
+                                "x written) plus 4 x T per chunk checkpoint"}}
+    if not args.no_vendor:
+        ia, ib, ic, idd = (v if inter else v.view(N, n).t().contiguous().view(-1)
+                           for v in (a, b, c, d))
+        xi = x if inter else x.view(N, n).t().contiguous().view(-1)
+        rs = ctypes.CDLL("librocsparse.so")
+        vp = ctypes.c_void_p
+        pre = "s" if args.dtype == "f32" else "d"
+        hdl = vp()
+        rs.rocsparse_create_handle(ctypes.byref(hdl))
+        rs.rocsparse_set_stream(hdl, vp(stream.cuda_stream))
+        lu = 2  # rocsparse_gtsv_interleaved_alg_lu: LU with partial pivoting
+        cp = [v.clone() for v in (ia, ib, ic, idd)]
+        size = ctypes.c_size_t(0)
+        st = getattr(rs, f"rocsparse_{pre}gtsv_interleaved_batch_buffer_size")(
+            hdl, ctypes.c_int(lu), ctypes.c_int(n), *(vp(v.data_ptr()) for v in cp),
+            ctypes.c_int(N), ctypes.c_int(N), ctypes.byref(size))
+        vend = {"library": f"rocSPARSE rocsparse_{pre}gtsv_interleaved_batch (LU, "
+                           f"partial pivoting, interleaved layout)"}
+        if st != 0:
+            vend["error"] = f"buffer_size status {st}"
+        else:
+            buf = torch.empty(max(1, size.value), dtype=torch.uint8, device="cuda")
+            times = []
+            for i in range(7):
+                for dst, src in zip(cp, (ia, ib, ic, idd)):
+                    dst.copy_(src)
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                st = getattr(rs, f"rocsparse_{pre}gtsv_interleaved_batch")(
+                    hdl, ctypes.c_int(lu), ctypes.c_int(n), *(vp(v.data_ptr()) for v in cp),
+                    ctypes.c_int(N), ctypes.c_int(N), vp(buf.data_ptr()))
+                e1.record(stream)
+                torch.cuda.synchronize()
+                if st != 0:
+                    break
+                if i >= 2:
+                    times.append(e0.elapsed_time(e1))
+            if st != 0:
+                vend["error"] = f"solve status {st}"
+            else:
+                ms = sum(times) / len(times)
+                diff = ((cp[3] - xi).abs().view(n, N).amax(dim=0) /
+                        xi.abs().view(n, N).amax(dim=0)).max().item()
+                vend.update({"kernel_ms": ms, "unknowns_per_s": n * N / (ms * 1e-3),
+                             "max_rel_diff_from_ours": diff})
+        rs.rocsparse_destroy_handle(hdl)
+        out["vendor_baseline"] = vend
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        dt = np.float32 if args.dtype == "f32" else np.float64
+        ha, hb, hc, hd = O.generate("paired", 1, n, N, dt)
+        threads = O.max_threads()
+        reps, t1 = 0, time.perf_counter()
+        while True:
+            xr, infor = O.gtsv(ha, hb, hc, hd, n, threads)
+            reps += 1
+            el = time.perf_counter() - t1
+            if el >= args.cpu_seconds:
+                break
+        xs = (x.view(n, N).t().contiguous().view(-1) if inter else x).cpu().numpy()
+        same = bool(np.all((xs == xr) | (np.isnan(xs) & np.isnan(xr))))
+        out["cpu_baseline"] = {
+            "value": reps * n * N / el, "unit": "unknowns/s", "cores": threads,
+            "kind": "port",
+            "sample": (f"oracle ?gtsv restatement ({args.dtype}), full {N}x{n} batch x "
+                       f"{reps} passes, {el:.1f} s, OpenMP {threads} threads"),
+            "gpu_bitwise_equal_to_oracle": same,
+            "gpu_info_equal_to_oracle": bool(np.array_equal(info.cpu().numpy(), infor)),
+        }
     h.close()
     print(json.dumps(out), flush=True)
 
@@ -706,6 +853,9 @@ def main():
         return
     if args.workload == "gauss-sweep":
         bench_gauss_sweep(args)
+        return
+    if args.workload == "pivoting":
+        bench_pivoting(args)
         return
     if args.layout == "interleaved":
         bench_interleaved(args)

@@ -6,8 +6,9 @@
  * stored system-major in five separate arrays: element j of system s is at
  * s*systemSize + j in a (sub-diagonal), b (diagonal), c (super-diagonal),
  * d (right-hand side) and x (solution). a[first] and c[last] of each system
- * are ignored (treated as 0). No pivoting is done: the methods are meant for
- * diagonally dominant systems, as in the reference.
+ * are ignored (treated as 0). The reference's methods do no pivoting and are
+ * meant for diagonally dominant systems; trid_gtsv_* and
+ * trid_solve_pivoting_* interchange rows (LAPACK ?gtsv) for the others.
  *
  * Every function returns an int status (trid_status); none of them exits
  * the process.
@@ -139,6 +140,15 @@ int trid_pcrgauss_f32(const float* a, const float* b, const float* c, const floa
 int trid_pcrgauss_f64(const double* a, const double* b, const double* c, const double* d,
                       double* x, int systemSize, int numSystems, int gaussian);
 
+/* replaces void tridiagonalMKL(T*,T*,T*,T*&,T*&,int,int) tridiagonalMKL.cpp:5
+ * (MKL ?gtsv per system, partial pivoting). Host pointers; a, b, c, d are not
+ * modified and x receives the solution (the reference overwrites d and swaps
+ * the d/x pointers). A singular system's x is NaN. systemSize >= 1.       */
+int trid_gtsv_f32(const float* a, const float* b, const float* c, const float* d,
+                  float* x, int systemSize, int numSystems);
+int trid_gtsv_f64(const double* a, const double* b, const double* c, const double* d,
+                  double* x, int systemSize, int numSystems);
+
 /* ---------------------------------------------------------------------- */
 /* Device-pointer, stream-ordered API                                       */
 /* ---------------------------------------------------------------------- */
@@ -194,6 +204,35 @@ int trid_solve_interleaved_f64(trid_handle handle, const double* a, const double
                                int64_t numSystems, int64_t batchStride, void* stream);
 
 /*
+ * Partial-pivoting solve: Gaussian elimination with row interchanges, the
+ * LAPACK ?gtsv algorithm for one right-hand side, for systems that are not
+ * diagonally dominant. The device-side counterpart of the reference's MKL
+ * baseline (tridiagonalMKL.cpp:5-26, ?gtsv per system). Any systemSize >= 1.
+ * `info` (device array of numSystems int32, may be NULL) receives 0, or the
+ * 1-based row of the first exactly zero pivot (LAPACK's INFO); such a system
+ * is singular and its x is set to NaN. a, b, c, d are not modified; x may be
+ * d but no other input. Uses the handle's workspace (a checkpoint of 4
+ * elements per 64 bytes of each system), so calls on one handle must not
+ * overlap on different streams. The interleaved form takes row i of system s
+ * at i * batchStride + s (batchStride >= numSystems), the layout of
+ * rocSPARSE's gtsv_interleaved_batch.
+ */
+int trid_solve_pivoting_f32(trid_handle handle, const float* a, const float* b, const float* c,
+                            const float* d, float* x, int64_t systemSize, int64_t numSystems,
+                            int32_t* info, void* stream);
+int trid_solve_pivoting_f64(trid_handle handle, const double* a, const double* b,
+                            const double* c, const double* d, double* x, int64_t systemSize,
+                            int64_t numSystems, int32_t* info, void* stream);
+int trid_solve_pivoting_interleaved_f32(trid_handle handle, const float* a, const float* b,
+                                        const float* c, const float* d, float* x,
+                                        int64_t systemSize, int64_t numSystems,
+                                        int64_t batchStride, int32_t* info, void* stream);
+int trid_solve_pivoting_interleaved_f64(trid_handle handle, const double* a, const double* b,
+                                        const double* c, const double* d, double* x,
+                                        int64_t systemSize, int64_t numSystems,
+                                        int64_t batchStride, int32_t* info, void* stream);
+
+/*
  * Relative residual of every system, in FP64 on the device:
  * relres[s] = max_i |(d - T x)_i| / max_i |d_i| over system s =
  * [offsets[s], offsets[s+1]) (device arrays; a[first], c[last] ignored).
@@ -210,7 +249,13 @@ int trid_residual_f64(const double* a, const double* b, const double* c, const d
  *   kind 0  a,c ~ U[-1,1], b = |a|+|c|+U[1,2],      d ~ U[-1,1]
  *   kind 1  a,c ~ U[-1,1], b = |a|+|c|+U[0.01,0.1], d ~ U[-1,1]
  *   kind 2  b = 8+U, a = 3+U, c = 2+U, d = 5+U (tridiagonal_gold.h:81-94)
- * with a[first] = c[last] = 0 in every system.
+ *   kind 3  not diagonally dominant but well conditioned once rows are
+ *           interchanged: c = 1+U on even rows and a = 1+U on odd rows
+ *           couple each row pair, b = 0.001 U[-1,1] (b = 1+U on the last row
+ *           of an odd-sized system), the other couplings 0.25 U[-1,1],
+ *           d ~ U[-1,1]; the no-pivoting methods meet tiny pivots on it
+ *   kind 4  a, b, c, d ~ U[-1,1] (general; may be ill-conditioned)
+ * with a[first] = c[last] = 0 in every system (U = U[0,1]).
  */
 int trid_generate_f32(int kind, uint64_t seed, float* a, float* b, float* c, float* d,
                       int64_t systemSize, int64_t numSystems, void* stream);

@@ -47,6 +47,27 @@ TRID_COMPAT_8(crpcrNBC, trid_crpcrnbc, restSystemSize)       /* crNBC.cu:344    
 TRID_COMPAT_7(crNBC32, trid_crnbc32)                         /* crNBC32banks.cu:123  */
 TRID_COMPAT_8(pcrGaussWrapper, trid_pcrgauss, gaussian)      /* pcrGauss_app.cu:164  */
 
+/* tridiagonalMKL.cpp:5: ?gtsv per system, solving in place in d and then
+ * swapping the d and x pointers, so that afterwards x points at the
+ * solution. Kept here: the solution is written into the d buffer and the
+ * pointers are swapped. a, b, c are left unmodified (?gtsv overwrites them). */
+inline void tridiagonalMKL(float* a, float* b, float* c, float*& d, float*& x, int system_size,
+                           int num_systems)
+{
+    if (trid_gtsv_f32(a, b, c, d, d, system_size, num_systems) != TRID_OK) return;
+    float* t = x;
+    x = d;
+    d = t;
+}
+inline void tridiagonalMKL(double* a, double* b, double* c, double*& d, double*& x,
+                           int system_size, int num_systems)
+{
+    if (trid_gtsv_f64(a, b, c, d, d, system_size, num_systems) != TRID_OK) return;
+    double* t = x;
+    x = d;
+    d = t;
+}
+
 #undef TRID_COMPAT_7
 #undef TRID_COMPAT_8
 

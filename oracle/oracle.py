@@ -29,15 +29,20 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-GEN_DOMINANT, GEN_WEAK, GEN_REFGEN = 0, 1, 2
-GEN_KINDS = {"dominant": GEN_DOMINANT, "weak": GEN_WEAK, "refgen": GEN_REFGEN}
+GEN_DOMINANT, GEN_WEAK, GEN_REFGEN, GEN_PAIRED, GEN_GENERAL = 0, 1, 2, 3, 4
+GEN_KINDS = {"dominant": GEN_DOMINANT, "weak": GEN_WEAK, "refgen": GEN_REFGEN,
+             "paired": GEN_PAIRED, "general": GEN_GENERAL}
 
 _lib = None
 
 
 def build(force: bool = False) -> str:
-    """Compile the C oracle with its Makefile (gcc + OpenMP)."""
-    if force or not os.path.exists(LIB_PATH):
+    """Compile the C oracle with its Makefile (gcc + OpenMP) when it is
+    missing or older than its source."""
+    src = os.path.join(HERE, "trid_oracle.c")
+    stale = (not os.path.exists(LIB_PATH) or
+             (os.path.exists(src) and os.path.getmtime(src) > os.path.getmtime(LIB_PATH)))
+    if force or stale:
         subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "all"],
                        check=True)
     return LIB_PATH
@@ -68,6 +73,10 @@ def lib() -> ctypes.CDLL:
         for name in ("oracle_generate_f32", "oracle_generate_f64"):
             f = getattr(L, name)
             f.argtypes = [ctypes.c_int, ctypes.c_uint64, P, P, P, P, P, i64]
+            f.restype = ctypes.c_int
+        for name in ("oracle_gtsv_f32", "oracle_gtsv_f64"):
+            f = getattr(L, name)
+            f.argtypes = [P, P, P, P, P, i64, i64, P, ctypes.c_int]
             f.restype = ctypes.c_int
         L.oracle_stream_key.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
         L.oracle_stream_key.restype = ctypes.c_uint64
@@ -153,6 +162,23 @@ def generate_np(kind, seed: int, n: int, num_systems: int, dtype=np.float32,
     first = offsets[:-1]
     last = offsets[1:] - 1
     dt = np.dtype(dtype)
+    if k == GEN_PAIRED:
+        j = g.astype(np.int64) - np.repeat(first, np.diff(offsets))
+        even = (j & 1) == 0
+        is_last = np.zeros(total, dtype=bool)
+        is_last[last] = True
+        a = np.where(even, 0.25 * (2.0 * ua - 1.0), 1.0 + ua).astype(dt)
+        b = np.where(even & is_last, 1.0 + ub, 0.001 * (2.0 * ub - 1.0)).astype(dt)
+        c = np.where(even, 1.0 + uc, 0.25 * (2.0 * uc - 1.0)).astype(dt)
+        d = (2.0 * ud - 1.0).astype(dt)
+        a[first] = 0
+        c[last] = 0
+        return a, b, c, d
+    if k == GEN_GENERAL:
+        a, b, c, d = ((2.0 * u - 1.0).astype(dt) for u in (ua, ub, uc, ud))
+        a[first] = 0
+        c[last] = 0
+        return a, b, c, d
     if k == GEN_REFGEN:
         a = (3.0 + ua).astype(dt)
         b = (8.0 + ub).astype(dt)
@@ -207,6 +233,26 @@ def thomas_f32(a, b, c, d, n: int, threads: int = 0) -> np.ndarray:
     if rc != 0:
         raise RuntimeError("oracle Thomas failed")
     return x
+
+
+def gtsv(a, b, c, d, n: int, threads: int = 0):
+    """Partial-pivoting solve (LAPACK ?gtsv restated, in the arrays' own
+    precision: float32 = sgtsv, float64 = dgtsv) of every system of a uniform
+    batch. The algorithm of the reference's MKL baseline,
+    tridiagonalMKL.cpp:5-26. Returns (x, info); a system with info != 0 has
+    x = NaN."""
+    a, b, c, d = (np.ascontiguousarray(v) for v in (a, b, c, d))
+    dt = a.dtype
+    assert dt in (np.float32, np.float64)
+    assert b.dtype == dt and c.dtype == dt and d.dtype == dt
+    assert a.shape[0] % n == 0
+    N = a.shape[0] // n
+    x = np.empty_like(a)
+    info = np.empty(N, dtype=np.int32)
+    f = lib().oracle_gtsv_f32 if dt == np.float32 else lib().oracle_gtsv_f64
+    if f(_p(a), _p(b), _p(c), _p(d), _p(x), n, N, _p(info), threads) != 0:
+        raise RuntimeError("oracle gtsv failed")
+    return x, info
 
 
 def thomas_np(a, b, c, d, n: int) -> np.ndarray:

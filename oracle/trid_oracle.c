@@ -27,6 +27,9 @@
  *    metric (max over systems of the relative infinity-norm error) and the
  *    relative residual ||d - T x||_inf / ||d||_inf in long double.
  *
+ *  - oracle_gtsv_*: partial pivoting (LAPACK ?gtsv), the algorithm of the
+ *    reference's MKL baseline (tridiagonalMKL.cpp:5-26), in FP32 and FP64.
+ *
  *  - oracle_generate_*: a seeded, counter-based input generator. The
  *    reference generator (tridiagonal_gold.h:75-94) uses unseeded rand(), so
  *    it cannot be reproduced; this one is defined here and restated
@@ -150,6 +153,114 @@ RAGGED_DRIVER(oracle_thomas_ragged_f64, double, double, double, thomas1_f64_f64i
 int oracle_max_threads(void) { return omp_get_max_threads(); }
 
 /* ------------------------------------------------------------------------ */
+/* Partial pivoting (LAPACK ?gtsv, one right-hand side)                      */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * The reference's CPU baseline calls MKL ?gtsv on each system
+ * (tridiagonalMKL.cpp:5-26; &a[s*n+1] is passed as the sub-diagonal, so
+ * a[first] is skipped, and c[last] is never read). MKL is not in this image.
+ * This is the published LAPACK algorithm (?gtsv, the NRHS = 1 branch)
+ * restated in the element type itself (float = sgtsv, double = dgtsv) and
+ * made non-destructive: D, U, W are caller scratch (diagonal, first and
+ * second super-diagonal of U after the row interchanges), the right-hand side
+ * is carried in x.
+ *   for i = 0 .. n-2, with l = a[i+1] the entry under the pivot:
+ *     |D_i| >= |l|: D_i == 0 -> info = i+1 (stop); else f = l/D_i,
+ *                   D_{i+1} -= f U_i, x_{i+1} -= f x_i, W_i = 0
+ *     else:         f = D_i/l, D_i = l, t = D_{i+1}, D_{i+1} = U_i - f t,
+ *                   W_i = U_{i+1}, U_{i+1} = -f W_i (both skipped at i = n-2),
+ *                   U_i = t, swap x_i, x_{i+1} with x_{i+1} = x_i - f x_{i+1}
+ *   D_{n-1} == 0 -> info = n
+ *   x_{n-1} /= D_{n-1}; x_{n-2} = (x_{n-2} - U x_{n-1}) / D;
+ *   x_i = (x_i - U_i x_{i+1} - W_i x_{i+2}) / D_i
+ * Returns info (0 = solved); with info != 0 x is set to NaN, the product's
+ * rule for a singular system (LAPACK leaves it uncomputed).
+ */
+#define GTSV_ONE(NAME, TT, ABS, QNAN)                                         \
+    static int64_t NAME(const TT* a, const TT* b, const TT* c, const TT* d,   \
+                        TT* x, TT* D, TT* U, TT* W, int64_t n)                \
+    {                                                                         \
+        int64_t info = 0;                                                     \
+        for (int64_t i = 0; i < n; i++) {                                     \
+            D[i] = b[i];                                                      \
+            U[i] = (i < n - 1) ? c[i] : (TT)0;                                \
+            W[i] = (i < n - 1) ? a[i + 1] : (TT)0;                            \
+            x[i] = d[i];                                                      \
+        }                                                                     \
+        for (int64_t i = 0; i < n - 1 && info == 0; i++) {                    \
+            if (ABS(D[i]) >= ABS(W[i])) {                                     \
+                if (D[i] != (TT)0) {                                          \
+                    TT f = W[i] / D[i];                                       \
+                    D[i + 1] = D[i + 1] - f * U[i];                           \
+                    x[i + 1] = x[i + 1] - f * x[i];                           \
+                } else {                                                      \
+                    info = i + 1;                                             \
+                }                                                             \
+                W[i] = (TT)0;                                                 \
+            } else {                                                          \
+                TT f = D[i] / W[i];                                           \
+                D[i] = W[i];                                                  \
+                TT t = D[i + 1];                                              \
+                D[i + 1] = U[i] - f * t;                                      \
+                if (i < n - 2) {                                              \
+                    W[i] = U[i + 1];                                          \
+                    U[i + 1] = -f * W[i];                                     \
+                } else {                                                      \
+                    W[i] = (TT)0;                                             \
+                }                                                             \
+                U[i] = t;                                                     \
+                t = x[i];                                                     \
+                x[i] = x[i + 1];                                              \
+                x[i + 1] = t - f * x[i + 1];                                  \
+            }                                                                 \
+        }                                                                     \
+        if (info == 0 && D[n - 1] == (TT)0) info = n;                         \
+        if (info != 0) {                                                      \
+            for (int64_t i = 0; i < n; i++) x[i] = QNAN;                      \
+            return info;                                                      \
+        }                                                                     \
+        x[n - 1] = x[n - 1] / D[n - 1];                                       \
+        if (n > 1) x[n - 2] = (x[n - 2] - U[n - 2] * x[n - 1]) / D[n - 2];    \
+        for (int64_t i = n - 3; i >= 0; i--)                                  \
+            x[i] = (x[i] - U[i] * x[i + 1] - W[i] * x[i + 2]) / D[i];         \
+        return 0;                                                             \
+    }
+
+GTSV_ONE(gtsv1_f32, float, fabsf, nanf(""))
+GTSV_ONE(gtsv1_f64, double, fabs, nan(""))
+
+/* Uniform batch, system s at [s*n, (s+1)*n); info may be NULL. */
+#define GTSV_DRIVER(NAME, TT, ONE)                                            \
+    int NAME(const TT* a, const TT* b, const TT* c, const TT* d, TT* x,      \
+             int64_t n, int64_t num_systems, int32_t* info, int threads)      \
+    {                                                                         \
+        if (n <= 0 || num_systems < 0) return 1;                              \
+        int nt = nthreads_or_max(threads);                                    \
+        int fail = 0;                                                         \
+        _Pragma("omp parallel num_threads(nt)")                               \
+        {                                                                     \
+            TT* D = (TT*)malloc(sizeof(TT) * (size_t)n * 3);                  \
+            if (!D) {                                                         \
+                _Pragma("omp atomic write") fail = 1;                         \
+            } else {                                                          \
+                _Pragma("omp for schedule(static)")                           \
+                for (int64_t s = 0; s < num_systems; s++) {                   \
+                    int64_t o = s * n;                                        \
+                    int64_t r = ONE(a + o, b + o, c + o, d + o, x + o, D,     \
+                                    D + n, D + 2 * n, n);                     \
+                    if (info) info[s] = (int32_t)r;                           \
+                }                                                             \
+            }                                                                 \
+            free(D);                                                          \
+        }                                                                     \
+        return fail;                                                          \
+    }
+
+GTSV_DRIVER(oracle_gtsv_f32, float, gtsv1_f32)
+GTSV_DRIVER(oracle_gtsv_f64, double, gtsv1_f64)
+
+/* ------------------------------------------------------------------------ */
 /* Comparison and residual                                                   */
 /* ------------------------------------------------------------------------ */
 
@@ -246,7 +357,7 @@ double oracle_u01(uint64_t seed, uint64_t stream, uint64_t g)
     return u01(oracle_stream_key(seed, stream), g);
 }
 
-enum { GEN_DOMINANT = 0, GEN_WEAK = 1, GEN_REFGEN = 2 };
+enum { GEN_DOMINANT = 0, GEN_WEAK = 1, GEN_REFGEN = 2, GEN_PAIRED = 3, GEN_GENERAL = 4 };
 
 /*
  * Fill one flat batch. Element g of the flat arrays belongs to the system
@@ -255,6 +366,11 @@ enum { GEN_DOMINANT = 0, GEN_WEAK = 1, GEN_REFGEN = 2 };
  *   DOMINANT: a,c ~ U[-1,1], b = |a|+|c| + U[1,2],      d ~ U[-1,1]
  *   WEAK:     a,c ~ U[-1,1], b = |a|+|c| + U[0.01,0.1], d ~ U[-1,1]
  *   REFGEN:   b = 8+U, a = 3+U, c = 2+U, d = 5+U (reference value ranges)
+ *   PAIRED:   not diagonally dominant, well conditioned after row
+ *             interchanges: within system index j, even rows a = 0.25(2U-1),
+ *             c = 1+U; odd rows a = 1+U, c = 0.25(2U-1); b = 0.001(2U-1)
+ *             (b = 1+U on the last row when it is even); d = 2U-1
+ *   GENERAL:  a, b, c, d = 2U-1 (not dominant; may be ill-conditioned)
  * All arithmetic is double; each value is rounded once to the output type.
  * b uses the already-rounded a and c.
  */
@@ -262,7 +378,7 @@ enum { GEN_DOMINANT = 0, GEN_WEAK = 1, GEN_REFGEN = 2 };
     int NAME(int kind, uint64_t seed, TT* a, TT* b, TT* c, TT* d,             \
              const int64_t* offsets, int64_t num_systems)                     \
     {                                                                         \
-        if (kind < 0 || kind > 2) return 1;                                   \
+        if (kind < 0 || kind > 4) return 1;                                   \
         uint64_t ka = oracle_stream_key(seed, 0), kb = oracle_stream_key(seed, 1); \
         uint64_t kc = oracle_stream_key(seed, 2), kd = oracle_stream_key(seed, 3); \
         _Pragma("omp parallel for schedule(dynamic, 16)")                     \
@@ -275,6 +391,20 @@ enum { GEN_DOMINANT = 0, GEN_WEAK = 1, GEN_REFGEN = 2 };
                 if (kind == GEN_REFGEN) {                                     \
                     av = (TT)(3.0 + ua); bv = (TT)(8.0 + ub);                 \
                     cv = (TT)(2.0 + uc); dv = (TT)(5.0 + ud);                 \
+                    if (g == o) av = 0;                                       \
+                    if (g == e - 1) cv = 0;                                   \
+                } else if (kind == GEN_PAIRED) {                              \
+                    int even = ((g - o) & 1) == 0;                            \
+                    av = (TT)(even ? 0.25 * (2.0 * ua - 1.0) : 1.0 + ua);     \
+                    bv = (TT)((even && g == e - 1) ? 1.0 + ub                 \
+                                                   : 0.001 * (2.0 * ub - 1.0)); \
+                    cv = (TT)(even ? 1.0 + uc : 0.25 * (2.0 * uc - 1.0));     \
+                    dv = (TT)(2.0 * ud - 1.0);                                \
+                    if (g == o) av = 0;                                       \
+                    if (g == e - 1) cv = 0;                                   \
+                } else if (kind == GEN_GENERAL) {                             \
+                    av = (TT)(2.0 * ua - 1.0); bv = (TT)(2.0 * ub - 1.0);     \
+                    cv = (TT)(2.0 * uc - 1.0); dv = (TT)(2.0 * ud - 1.0);     \
                     if (g == o) av = 0;                                       \
                     if (g == e - 1) cv = 0;                                   \
                 } else {                                                      \

@@ -6,8 +6,9 @@
 // with NaN counted as failure (tridiagonal_gold.h:111-148).
 //
 // Unlike the reference driver (which always returns EXIT_SUCCESS and runs
-// only pcrGaussWrapper(..., 64)), every method runs and the process exits
-// with the number of failed (method, system) pairs, capped at 255.
+// only pcrGaussWrapper(..., 64)), every method runs -- including the
+// partial-pivoting MKL baseline's name, tridiagonalMKL -- and the process
+// exits with the number of failed (method, system) pairs, capped at 255.
 //
 // Test infrastructure: links the CPU oracle (oracle/liboracle.so) as the
 // checker. Usage: test_tridiagonal [numSystems] [systemSize] [f32|f64]
@@ -77,9 +78,12 @@ static int run(int numSystems, int systemSize)
         cases.push_back({"crpcrNBC", rest});
     }
     for (int g = 1; g <= n_pad / 2 && g <= 512; g *= 2) cases.push_back({"pcrGaussWrapper", g});
+    cases.push_back({"tridiagonalMKL", 0});
+    std::vector<T> dwork(total);
 
     for (const Case& k : cases) {
         std::fill(x.begin(), x.end(), T(NAN));
+        const T* xs = x.data();
         if (k.name == "cr") cr(a, b, c, d, x.data(), systemSize, numSystems);
         else if (k.name == "pcr") pcr(a, b, c, d, x.data(), systemSize, numSystems);
         else if (k.name == "crNBC") crNBC(a, b, c, d, x.data(), systemSize, numSystems);
@@ -87,7 +91,21 @@ static int run(int numSystems, int systemSize)
         else if (k.name == "crpcr") crpcr(a, b, c, d, x.data(), systemSize, numSystems, k.param);
         else if (k.name == "crpcrNBC")
             crpcrNBC(a, b, c, d, x.data(), systemSize, numSystems, k.param);
-        else pcrGaussWrapper(a, b, c, d, x.data(), systemSize, numSystems, k.param);
+        else if (k.name == "pcrGaussWrapper")
+            pcrGaussWrapper(a, b, c, d, x.data(), systemSize, numSystems, k.param);
+        else {
+            // solves in the d buffer and swaps the pointers (tridiagonalMKL.cpp:22-25)
+            dwork = arr[3];
+            T* dp = dwork.data();
+            T* xp = x.data();
+            tridiagonalMKL(a, b, c, dp, xp, systemSize, numSystems);
+            if (trid_last_status() == TRID_OK && (xp != dwork.data() || dp != x.data())) {
+                std::printf("tridiagonalMKL did not swap d and x\n");
+                failures += numSystems;
+                continue;
+            }
+            xs = xp;
+        }
         const int st = trid_last_status();
         if (st == TRID_ENOTSUP) {
             std::printf("%-16s %5d  not supported at n=%d\n", k.name.c_str(), k.param,
@@ -100,7 +118,7 @@ static int run(int numSystems, int systemSize)
             failures += numSystems;
             continue;
         }
-        for (size_t i = 0; i < total; ++i) xd[i] = (double)x[i];
+        for (size_t i = 0; i < total; ++i) xd[i] = (double)xs[i];
         const int64_t f = oracle_compare_ragged(xd.data(), xref.data(), off.data(), numSystems,
                                                 1e-3, rel.data(), rms.data());
         double worst = 0.0;

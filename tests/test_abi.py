@@ -57,6 +57,10 @@ def test_host_dropins_reject_bad_arguments_without_a_device(trid):
     assert lib.trid_pcrgauss_f32(p, p, p, p, p, 16, 1, 3) == EINVAL
     assert lib.trid_pcrgauss_f32(p, p, p, p, p, 16, 1, 16) == EINVAL
     assert lib.trid_crpcrnbc_f64(p, p, p, p, p, 16, 1, 5) == EINVAL
+    assert lib.trid_gtsv_f32(p, p, p, p, null, 16, 1) == EINVAL
+    assert lib.trid_gtsv_f64(p, p, p, p, p, 0, 1) == EINVAL       # n < 1
+    assert lib.trid_gtsv_f32(p, p, p, p, p, 16, -1) == EINVAL
+    assert lib.trid_gtsv_f32(p, p, p, p, p, 1, 0) == 0            # empty batch
 
 
 def test_device_api_rejects_bad_arguments_without_a_device(trid):
@@ -79,6 +83,20 @@ def test_device_api_rejects_bad_arguments_without_a_device(trid):
         h.solve_interleaved(16, 32, 48, 64, 48, 8, 10, 10)
     with pytest.raises(trid.TridError):
         h.solve_interleaved(0, 32, 48, 64, 80, 8, 10, 10)
+    # pivoting solves: zero rows, x on a/b/c, interleaved stride below the
+    # system count or zero, a null input; an empty batch is a no-op
+    with pytest.raises(trid.TridError):
+        h.solve_pivoting(16, 32, 48, 64, 80, 0, 10)
+    with pytest.raises(trid.TridError):
+        h.solve_pivoting(16, 32, 48, 64, 32, 8, 10)
+    with pytest.raises(trid.TridError):
+        h.solve_pivoting(16, 32, 48, 64, 80, 8, 10, batchStride=9)
+    with pytest.raises(trid.TridError):
+        h.solve_pivoting(16, 32, 48, 64, 80, 8, 10, batchStride=0)
+    with pytest.raises(trid.TridError):
+        h.solve_pivoting(0, 32, 48, 64, 80, 8, 10)
+    h.solve_pivoting(16, 32, 48, 64, 80, 8, 0)
+    h.solve_pivoting(16, 32, 48, 64, 64, 1, 0, dtype="f64")
     h.close()
 
 

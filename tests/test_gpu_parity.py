@@ -176,7 +176,8 @@ def test_zero_pivot_gives_nonfinite_not_crash(oracle, trid):
 def test_device_generator_bitwise_equals_oracle(oracle, trid):
     import torch
     for dtype, tdt in ((np.float32, torch.float32), (np.float64, torch.float64)):
-        for kind, kname in ((0, "dominant"), (1, "weak"), (2, "refgen")):
+        for kind, kname in ((0, "dominant"), (1, "weak"), (2, "refgen"), (3, "paired"),
+                            (4, "general")):
             n, N = 1000, 33
             dev = [torch.empty(n * N, dtype=tdt, device="cuda") for _ in range(4)]
             trid.generate(kind, 42, *dev, n, N)

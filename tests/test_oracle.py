@@ -32,7 +32,7 @@ def relerr(x, ref, n):
 # generator
 # --------------------------------------------------------------------------
 
-@pytest.mark.parametrize("kind", ["dominant", "weak", "refgen"])
+@pytest.mark.parametrize("kind", ["dominant", "weak", "refgen", "paired", "general"])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_generator_c_equals_numpy(oracle, kind, dtype):
     for n, N in ((2, 3), (17, 5), (1024, 4)):

@@ -16,9 +16,11 @@ from .solvers import (  # noqa: F401
     crpcrNBC,
     device_count,
     generate,
+    gtsv,
     make_opts,
     pcr,
     pcrGaussWrapper,
     plan_query,
+    tridiagonalMKL,
 )
 from ._lib import LIB_PATH, lib  # noqa: F401

@@ -29,6 +29,9 @@ EXPORTS = (
     "trid_generate_f32", "trid_generate_f64", "trid_plan_query",
     "trid_solve_refined_f64", "trid_residual_f64",
     "trid_solve_interleaved_f32", "trid_solve_interleaved_f64",
+    "trid_solve_pivoting_f32", "trid_solve_pivoting_f64",
+    "trid_solve_pivoting_interleaved_f32", "trid_solve_pivoting_interleaved_f64",
+    "trid_gtsv_f32", "trid_gtsv_f64",
     "trid_ragged_plan_create", "trid_ragged_plan_destroy", "trid_ragged_plan_info",
     "trid_ragged_solve_f64",
     "trid_server_create", "trid_server_destroy", "trid_server_register_operator",
@@ -141,6 +144,16 @@ def lib() -> ctypes.CDLL:
             f = getattr(L, f"trid_{name}_{t}")
             f.argtypes = [P, P, P, P, P, i32, i32]
             f.restype = i32
+    for t in ("f32", "f64"):
+        f = getattr(L, f"trid_gtsv_{t}")
+        f.argtypes = [P, P, P, P, P, i32, i32]
+        f.restype = i32
+        f = getattr(L, f"trid_solve_pivoting_{t}")
+        f.argtypes = [P, P, P, P, P, P, i64, i64, P, P]
+        f.restype = i32
+        f = getattr(L, f"trid_solve_pivoting_interleaved_{t}")
+        f.argtypes = [P, P, P, P, P, P, i64, i64, i64, P, P]
+        f.restype = i32
     for name in ("crpcr", "crpcrnbc", "pcrgauss"):
         for t in ("f32", "f64"):
             f = getattr(L, f"trid_{name}_{t}")

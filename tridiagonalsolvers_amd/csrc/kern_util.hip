@@ -13,6 +13,12 @@
 //   0  a,c ~ U[-1,1], b = |a|+|c| + U[1,2],      d ~ U[-1,1]
 //   1  a,c ~ U[-1,1], b = |a|+|c| + U[0.01,0.1], d ~ U[-1,1]
 //   2  b = 8+U, a = 3+U, c = 2+U, d = 5+U
+//   3  not diagonally dominant, well conditioned once rows are interchanged:
+//      rows 2k, 2k+1 are coupled by c_{2k} = 1+U and a_{2k+1} = 1+U, the
+//      diagonals are tiny (b = 0.001 U[-1,1]) and the other couplings small
+//      (0.25 U[-1,1]); the last row of an odd-sized system has b = 1+U;
+//      d ~ U[-1,1]. Elimination without pivoting meets tiny pivots on it.
+//   4  a, b, c, d ~ U[-1,1]: general, not dominant, may be ill-conditioned
 // with a = 0 on the first and c = 0 on the last row of every system.
 #include "trid_device.hpp"
 #include "trid_launch.hpp"
@@ -55,6 +61,21 @@ __global__ __launch_bounds__(256) void generate_kernel(int kind, uint64_t ka, ui
             bv = (T)(8.0 + ub);
             cv = (T)(2.0 + uc);
             dv = (T)(5.0 + ud);
+            if (j == 0) av = T(0);
+            if (j == n - 1) cv = T(0);
+        } else if (kind == 3) {
+            const bool even = (j & 1) == 0;
+            av = (T)(even ? 0.25 * (2.0 * ua - 1.0) : 1.0 + ua);
+            bv = (T)((even && j == n - 1) ? 1.0 + ub : 0.001 * (2.0 * ub - 1.0));
+            cv = (T)(even ? 1.0 + uc : 0.25 * (2.0 * uc - 1.0));
+            dv = (T)(2.0 * ud - 1.0);
+            if (j == 0) av = T(0);
+            if (j == n - 1) cv = T(0);
+        } else if (kind == 4) {
+            av = (T)(2.0 * ua - 1.0);
+            bv = (T)(2.0 * ub - 1.0);
+            cv = (T)(2.0 * uc - 1.0);
+            dv = (T)(2.0 * ud - 1.0);
             if (j == 0) av = T(0);
             if (j == n - 1) cv = T(0);
         } else {

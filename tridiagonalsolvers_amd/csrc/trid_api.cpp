@@ -241,6 +241,73 @@ int interleaved_impl(trid_context* h, const T* a, const T* b, const T* c, const 
     return from_hip(trid::launch_interleaved<T>(g));
 }
 
+// Partial-pivoting solve (LAPACK ?gtsv semantics). stride 0 = system-major.
+template <typename T>
+int pivot_impl(trid_context* h, const T* a, const T* b, const T* c, const T* d, T* x,
+               int64_t n, int64_t N, int64_t stride, int32_t* info, hipStream_t stream)
+{
+    if (!h || !a || !b || !c || !d || !x) return TRID_EINVAL;
+    if (n < 1 || N < 0 || stride < 0) return TRID_EINVAL;
+    if (x == a || x == b || x == c) return TRID_EINVAL;
+    if (N == 0) return TRID_OK;
+    if (stride > 0) {
+        if (stride < N) return TRID_EINVAL;
+        if (n > INT64_MAX / stride / (int64_t)sizeof(T)) return TRID_EINVAL;
+    } else if (n > INT64_MAX / N / (int64_t)sizeof(T)) {
+        return TRID_EINVAL;
+    }
+    int st = check_device();
+    if (st != TRID_OK) return st;
+    st = ensure_ws(h, trid::pivot_ws_bytes<T>(n, N, stride > 0));
+    if (st != TRID_OK) return st;
+    trid::PivotArgs<T> g{};
+    g.a = a;
+    g.b = b;
+    g.c = c;
+    g.d = d;
+    g.x = x;
+    g.n = n;
+    g.N = N;
+    g.stride = stride;
+    g.ws = static_cast<T*>(h->ws);
+    g.info = info;
+    g.stream = stream;
+    return from_hip(trid::launch_pivot<T>(g));
+}
+
+// Host-pointer pivoting drop-in: copy in, solve on the null stream, copy out.
+template <typename T>
+int host_gtsv(const T* a, const T* b, const T* c, const T* d, T* x, int n, int N)
+{
+    if (!a || !b || !c || !d || !x) return TRID_EINVAL;
+    if (n < 1 || N < 0) return TRID_EINVAL;
+    if (N == 0) return TRID_OK;
+    int st = check_device();
+    if (st != TRID_OK) return st;
+    const size_t bytes = (size_t)n * (size_t)N * sizeof(T);
+    const size_t slot = (bytes + 255) & ~(size_t)255;
+    char* base = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&base), slot * 5);
+    if (e != hipSuccess) return from_hip(e);
+    T* dv[5];
+    for (int k = 0; k < 5; ++k) dv[k] = reinterpret_cast<T*>(base + k * slot);
+    const T* hv[4] = {a, b, c, d};
+    int rc = TRID_OK;
+    for (int k = 0; k < 4 && rc == TRID_OK; ++k)
+        rc = from_hip(hipMemcpy(dv[k], hv[k], bytes, hipMemcpyHostToDevice));
+    trid_context ctx{};
+    if (rc == TRID_OK)
+        rc = pivot_impl<T>(&ctx, dv[0], dv[1], dv[2], dv[3], dv[4], n, N, 0, nullptr, nullptr);
+    if (rc == TRID_OK) {
+        e = hipStreamSynchronize(nullptr);
+        if (e == hipSuccess) e = hipMemcpy(x, dv[4], bytes, hipMemcpyDeviceToHost);
+        rc = from_hip(e);
+    }
+    if (ctx.ws) (void)hipFree(ctx.ws);
+    (void)hipFree(base);
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -259,6 +326,54 @@ int trid_solve_interleaved_f64(trid_handle handle, const double* a, const double
 {
     return set_status(interleaved_impl<double>(handle, a, b, c, d, x, systemSize, numSystems,
                                                batchStride, static_cast<hipStream_t>(stream)));
+}
+
+int trid_solve_pivoting_f32(trid_handle handle, const float* a, const float* b, const float* c,
+                            const float* d, float* x, int64_t systemSize, int64_t numSystems,
+                            int32_t* info, void* stream)
+{
+    return set_status(pivot_impl<float>(handle, a, b, c, d, x, systemSize, numSystems, 0, info,
+                                         static_cast<hipStream_t>(stream)));
+}
+
+int trid_solve_pivoting_f64(trid_handle handle, const double* a, const double* b,
+                            const double* c, const double* d, double* x, int64_t systemSize,
+                            int64_t numSystems, int32_t* info, void* stream)
+{
+    return set_status(pivot_impl<double>(handle, a, b, c, d, x, systemSize, numSystems, 0, info,
+                                          static_cast<hipStream_t>(stream)));
+}
+
+int trid_solve_pivoting_interleaved_f32(trid_handle handle, const float* a, const float* b,
+                                        const float* c, const float* d, float* x,
+                                        int64_t systemSize, int64_t numSystems,
+                                        int64_t batchStride, int32_t* info, void* stream)
+{
+    if (batchStride < 1) return set_status(TRID_EINVAL);
+    return set_status(pivot_impl<float>(handle, a, b, c, d, x, systemSize, numSystems,
+                                         batchStride, info, static_cast<hipStream_t>(stream)));
+}
+
+int trid_solve_pivoting_interleaved_f64(trid_handle handle, const double* a, const double* b,
+                                        const double* c, const double* d, double* x,
+                                        int64_t systemSize, int64_t numSystems,
+                                        int64_t batchStride, int32_t* info, void* stream)
+{
+    if (batchStride < 1) return set_status(TRID_EINVAL);
+    return set_status(pivot_impl<double>(handle, a, b, c, d, x, systemSize, numSystems,
+                                          batchStride, info, static_cast<hipStream_t>(stream)));
+}
+
+int trid_gtsv_f32(const float* a, const float* b, const float* c, const float* d, float* x,
+                  int systemSize, int numSystems)
+{
+    return set_status(host_gtsv<float>(a, b, c, d, x, systemSize, numSystems));
+}
+
+int trid_gtsv_f64(const double* a, const double* b, const double* c, const double* d,
+                  double* x, int systemSize, int numSystems)
+{
+    return set_status(host_gtsv<double>(a, b, c, d, x, systemSize, numSystems));
 }
 
 const char* trid_status_string(int status)
@@ -375,7 +490,7 @@ int trid_residual_f64(const double* a, const double* b, const double* c, const d
 int trid_generate_f32(int kind, uint64_t seed, float* a, float* b, float* c, float* d,
                       int64_t systemSize, int64_t numSystems, void* stream)
 {
-    if (!a || !b || !c || !d || systemSize < 1 || numSystems < 0 || kind < 0 || kind > 2)
+    if (!a || !b || !c || !d || systemSize < 1 || numSystems < 0 || kind < 0 || kind > 4)
         return set_status(TRID_EINVAL);
     int st = check_device();
     if (st != TRID_OK) return set_status(st);
@@ -386,7 +501,7 @@ int trid_generate_f32(int kind, uint64_t seed, float* a, float* b, float* c, flo
 int trid_generate_f64(int kind, uint64_t seed, double* a, double* b, double* c, double* d,
                       int64_t systemSize, int64_t numSystems, void* stream)
 {
-    if (!a || !b || !c || !d || systemSize < 1 || numSystems < 0 || kind < 0 || kind > 2)
+    if (!a || !b || !c || !d || systemSize < 1 || numSystems < 0 || kind < 0 || kind > 4)
         return set_status(TRID_EINVAL);
     int st = check_device();
     if (st != TRID_OK) return set_status(st);

@@ -96,6 +96,27 @@ struct InterleavedArgs {
 template <typename T>
 hipError_t launch_interleaved(const InterleavedArgs<T>& g);
 
+// Partial-pivoting solve, LAPACK ?gtsv semantics (kern_gtsv.hip).
+template <typename T>
+struct PivotArgs {
+    const T* a;
+    const T* b;
+    const T* c;
+    const T* d;
+    T* x;            // may equal d
+    int64_t n;       // rows per system
+    int64_t N;       // systems
+    int64_t stride;  // 0: system-major; > 0: interleaved, row i of system s at i*stride + s
+    T* ws;           // checkpoints, pivot_ws_bytes<T>(n, N, stride > 0) bytes
+    int32_t* info;   // per system: 0, or the 1-based row of the first zero pivot (may be null)
+    hipStream_t stream;
+};
+
+template <typename T>
+size_t pivot_ws_bytes(int64_t n, int64_t N, bool interleaved);
+template <typename T>
+hipError_t launch_pivot(const PivotArgs<T>& g);
+
 // FP32 solve + FP64 residual refinement (kern_refine.hip).
 struct RefineArgs {
     const double* a;

@@ -88,6 +88,18 @@ def pcrGaussWrapper(a, b, c, d, x, systemSize, numSystems, gaussian):
     _call8("pcrgauss", a, b, c, d, x, systemSize, numSystems, gaussian)
 
 
+def gtsv(a, b, c, d, x, systemSize, numSystems):
+    """Partial-pivoting solve (LAPACK ?gtsv semantics) of every system; the
+    counterpart of the reference's MKL baseline ``tridiagonalMKL``
+    (tridiagonalMKL.cpp:5), which solves in place and swaps the d/x
+    pointers. Here a, b, c, d are left untouched and x receives the
+    solution; a singular system's x is NaN."""
+    _call7("gtsv", a, b, c, d, x, systemSize, numSystems)
+
+
+tridiagonalMKL = gtsv
+
+
 # --- options, plans, device API ----------------------------------------------
 
 def make_opts(rest=0, gaussian=0, lds_pad=0, lanes=0, per_lane=0, io_mode=0,
@@ -171,6 +183,26 @@ class Handle:
                ctypes.c_void_p(_stream_ptr(stream)))
         check(st, "trid_solve_interleaved")
 
+    def solve_pivoting(self, a, b, c, d, x, systemSize: int, numSystems: int,
+                       info=None, batchStride: int | None = None, stream=None,
+                       dtype: str | None = None) -> None:
+        """Partial pivoting (LAPACK ?gtsv). System-major by default; with
+        batchStride, interleaved (row i of system j at i * batchStride + j).
+        info: optional device int32 array of numSystems (0, or the 1-based
+        row of the first zero pivot); x may be d."""
+        if dtype is None:
+            dtype = "f64" if str(getattr(a, "dtype", "")) == "torch.float64" else "f32"
+        ptrs = [ctypes.c_void_p(_ptr(v)) for v in (a, b, c, d, x)]
+        info_p = ctypes.c_void_p(_ptr(info) if info is not None else None)
+        s = ctypes.c_void_p(_stream_ptr(stream))
+        if batchStride is None:
+            st = getattr(lib(), f"trid_solve_pivoting_{dtype}")(
+                self._h, *ptrs, int(systemSize), int(numSystems), info_p, s)
+        else:
+            st = getattr(lib(), f"trid_solve_pivoting_interleaved_{dtype}")(
+                self._h, *ptrs, int(systemSize), int(numSystems), int(batchStride), info_p, s)
+        check(st, "trid_solve_pivoting")
+
     def solve_refined(self, method, a, b, c, d, x, systemSize: int, numSystems: int,
                       iters: int = 2, opts: Opts | None = None, stream=None) -> None:
         """FP64 arrays; FP32 solves plus `iters` FP64 residual corrections."""
@@ -198,5 +230,6 @@ def device_count() -> int:
 
 
 __all__ = ["cr", "pcr", "crpcr", "crNBC", "crpcrNBC", "crNBC32", "pcrGaussWrapper",
+           "gtsv", "tridiagonalMKL",
            "Handle", "generate", "plan_query", "make_opts", "device_count", "TridError",
            "METHODS", "_lib"]
