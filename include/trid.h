@@ -212,7 +212,7 @@ int trid_solve_interleaved_f64(trid_handle handle, const double* a, const double
  * 1-based row of the first exactly zero pivot (LAPACK's INFO); such a system
  * is singular and its x is set to NaN. a, b, c, d are not modified; x may be
  * d but no other input. Uses the handle's workspace (a checkpoint of 4
- * elements per 64 bytes of each system), so calls on one handle must not
+ * elements per 128 bytes of each system), so calls on one handle must not
  * overlap on different streams. The interleaved form takes row i of system s
  * at i * batchStride + s (batchStride >= numSystems), the layout of
  * rocSPARSE's gtsv_interleaved_batch.

@@ -35,19 +35,31 @@
 // HBM traffic is 9 elements per unknown (4 + 4 read, 1 written) plus a
 // 4-element checkpoint per chunk and system, where keeping the factored rows
 // in memory would cost 13 (4 read, 4 written and read back, 1 written).
-// * System-major data (element i of system s at s*n + i): a chunk is 64
-//   bytes of each system; four lanes read one system's chunk as 16-byte
-//   vectors (16 systems per load), the chunk is transposed in the wave's LDS
-//   (row pitch 68 elements: the transposing writes and the per-lane reads
+// * System-major data (element i of system s at s*n + i): a chunk is 128
+//   bytes of each system; eight lanes read one system's chunk as 16-byte
+//   vectors (8 systems per load), the chunk is transposed in the wave's LDS
+//   (row pitch 66 elements: the transposing writes and the per-lane reads
 //   hit distinct banks) and x goes back out the same way.
 // * Interleaved data (element i of system s at i*stride + s): each lane reads
 //   and writes its own system directly; every access is coalesced across the
-//   wave's 64 systems. Chunks of 8 rows.
+//   wave's 64 systems. Chunks of 16 rows.
 #include "reg_io.hpp"
 #include "trid_device.hpp"
 #include "trid_launch.hpp"
 
 #pragma clang fp contract(off)
+
+// Bytes of one system per array in a system-major chunk (64 or 128).
+// (128: a whole cache line per system and array; on 65,536 x 1,024 it runs
+// 16% (FP32) and 21% (FP64) faster than 64, more bytes being in flight per
+// wave at one wave per SIMD.)
+#ifndef TRID_GTSV_CHUNK_BYTES
+#define TRID_GTSV_CHUNK_BYTES 128
+#endif
+// Rows per interleaved chunk.
+#ifndef TRID_GTSV_IL_ROWS
+#define TRID_GTSV_IL_ROWS 16
+#endif
 
 namespace trid {
 
@@ -107,13 +119,18 @@ __device__ __forceinline__ void wave_sync()
 
 template <typename T, bool VEC>
 struct SysMajorIO {
-    static constexpr int kVec = 16 / (int)sizeof(T);    // elements per vector
-    static constexpr int kRows = 64 / (int)sizeof(T);   // rows per chunk
-    static constexpr int kPitch = kWave + 4;            // LDS row pitch
-    static constexpr int kLanesPerSys = kRows / kVec;   // 4
-    static constexpr int kSysPerLoad = kWave / kLanesPerSys;  // 16
-    static constexpr int kLoads = kWave / kSysPerLoad;        // 4
+    static constexpr int kVec = 16 / (int)sizeof(T);                      // elements per vector
+    static constexpr int kRows = TRID_GTSV_CHUNK_BYTES / (int)sizeof(T);  // rows per chunk
+    static constexpr int kLanesPerSys = kRows / kVec;                     // 4 or 8
+    static constexpr int kSysPerLoad = kWave / kLanesPerSys;              // 16 or 8
+    static constexpr int kLoads = kWave / kSysPerLoad;                    // 4 or 8
+    // LDS row pitch: the lanes of one transposing write hold rows
+    // rl = j*kVec (j < kLanesPerSys) of kSysPerLoad consecutive systems; a
+    // pitch of 64 + 16/kLanesPerSys elements puts the kLanesPerSys groups
+    // on disjoint banks.
+    static constexpr int kPitch = kWave + 16 / kLanesPerSys;
     static constexpr int kLdsElems = 4 * kRows * kPitch;
+    static_assert(kLanesPerSys == 4 || kLanesPerSys == 8, "64- or 128-byte chunks");
 
     const T* p[4];
     T* X;
@@ -208,7 +225,7 @@ struct SysMajorIO {
 
 template <typename T>
 struct InterleavedIO {
-    static constexpr int kRows = 8;
+    static constexpr int kRows = TRID_GTSV_IL_ROWS;
     static constexpr int kLdsElems = 0;
 
     const T* p[4];
