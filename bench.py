@@ -48,6 +48,9 @@ def parse():
     p.add_argument("--systems", type=int, default=65536)
     p.add_argument("--rest", type=int, default=64)
     p.add_argument("--dtype", default="f32", choices=["f32", "f64"])
+    p.add_argument("--kind", default="dominant", choices=["dominant", "weak"],
+                   help="uniform batches: strongly (b = |a|+|c|+U[1,2]) or weakly "
+                        "(b = |a|+|c|+U[0.01,0.1]) diagonally dominant inputs")
     p.add_argument("--lanes", type=int, default=0)
     p.add_argument("--per-lane", type=int, default=0)
     p.add_argument("--io", type=int, default=0)
@@ -420,7 +423,7 @@ def cpu_baseline(args, x_gpu_np, n, N):
     from oracle import oracle as O
 
     dt = np.float32 if args.dtype == "f32" else np.float64
-    a, b, c, d = O.generate("dominant", 1, n, N, dt)
+    a, b, c, d = O.generate(getattr(args, "kind", "dominant"), 1, n, N, dt)
     threads = O.max_threads()
     reps, t0 = 0, time.perf_counter()
     while True:
@@ -872,7 +875,7 @@ def main():
     a, b, c, d = (torch.empty(n * N, dtype=tdt, device=dev) for _ in range(4))
     x = torch.empty_like(a)
     stream = torch.cuda.current_stream()
-    T.generate(0, 1, a, b, c, d, n, N, stream=stream)
+    T.generate(0 if args.kind == "dominant" else 1, 1, a, b, c, d, n, N, stream=stream)
     h = T.Handle()
     opts = T.make_opts(rest=args.rest if args.method in ("crpcr", "crpcrnbc") else 0,
                        gaussian=args.rest if args.method == "pcrgauss" else 0,
@@ -906,6 +909,8 @@ def main():
     alg_bytes = 5 * n * esize * N
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     workload = f"{args.method} {args.dtype} {N} systems x {n} unknowns"
+    if args.kind != "dominant":
+        workload += f", {args.kind}ly dominant"
     traffic = traffic_from_profiles(workload)
     out = {
         "metric": METRIC,
@@ -919,7 +924,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": args.dtype,
-        "data": "synthetic (seeded device generator: a,c~U[-1,1], b=|a|+|c|+U[1,2], d~U[-1,1])",
+        "data": ("synthetic (seeded device generator: a,c~U[-1,1], b=|a|+|c|+"
+                 + ("U[1,2]" if args.kind == "dominant" else "U[0.01,0.1]") + ", d~U[-1,1])"),
         "config": {"workload": workload, "method": args.method, "systems": N,
                    "system_size": n, "rest": plan["rest"], "lanes": plan["lanes"],
                    "per_lane": plan["per_lane"], "family": plan["family"]},
