@@ -149,7 +149,9 @@ int make_plan(int method, int64_t n, int64_t N, int elem_size, const trid_opts& 
             if (method == TRID_CRNBC32) {
                 pad = 0;  // the unpadded control (crNBC32banks.cu's map is the identity)
             } else if (o.lds_pad == 0) {
-                pad = 64 * 4 / elem_size;  // one element per 64 banks of 4 bytes
+                // one pad element per 32: the fewest bank-conflict cycles of the
+                // swept pads (8..128) in both precisions on gfx950
+                pad = 32;
             } else if (o.lds_pad < 0) {
                 pad = 0;
             } else {
