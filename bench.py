@@ -413,6 +413,46 @@ def copy_ceiling(a, b, c, d, stream, reps=10):
             "bytes": moved, "ms": ms}
 
 
+def stream_ceiling(a, b, c, d, x, stream, reps=20):
+    """Time of a plain stream of the solve's bytes (read a, b, c, d; write x;
+    16-byte non-temporal accesses, no arithmetic; bench_stream.so, built by
+    __graft_entry__.build()): the speed of light of this traffic on the
+    device, against which the solve kernel's time is read."""
+    import ctypes
+
+    import torch
+
+    path = os.path.join(ROOT, "bench_stream.so")
+    if not os.path.exists(path):
+        return {"error": "bench_stream.so not built"}
+    lib = ctypes.CDLL(path)
+    f = lib.bench_stream_4r1w
+    f.argtypes = [ctypes.c_void_p] * 5 + [ctypes.c_int64, ctypes.c_void_p]
+    f.restype = ctypes.c_int
+    nbytes = a.numel() * a.element_size()
+    if nbytes % 16 or any(v.data_ptr() % 16 for v in (a, b, c, d, x)):
+        return {"error": "arrays not 16-byte aligned"}
+    args = [ctypes.c_void_p(v.data_ptr()) for v in (a, b, c, d)]
+    dst = torch.empty_like(x)
+    call = lambda: f(*args, ctypes.c_void_p(dst.data_ptr()), nbytes // 16,
+                     ctypes.c_void_p(stream.cuda_stream))
+    for _ in range(3):
+        if call() != 0:
+            return {"error": "launch failed"}
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        call()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    moved = 5 * nbytes
+    return {"us": ms * 1e3, "gbs": moved / (ms * 1e-3) / 1e9,
+            "frac_of_peak": moved / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, "bytes": moved}
+
+
 def cpu_baseline(args, x_gpu_np, n, N):
     """FP32 Thomas (same recurrence as tridiagonal_gold.h:26-44) with OpenMP
     over all host cores on the same inputs, repeated for about
@@ -937,6 +977,10 @@ def main():
     if not args.no_vendor:
         out["vendor_baseline"] = vendor_baseline(a, b, c, d, x, n, N, args.dtype, stream)
         out["copy_ceiling"] = copy_ceiling(a, b, c, d, stream)
+        out["stream_ceiling"] = stream_ceiling(a, b, c, d, x, stream)
+        if "us" in out["stream_ceiling"]:
+            out["stream_ceiling"]["solve_time_over_stream_time"] = (
+                avg_ms * 1e3 / out["stream_ceiling"]["us"])
     if args.copy_probe:
         # 4 * n * N * esize bytes read and written by a plain copy kernel
         dst = torch.empty_like(a)

@@ -41,4 +41,6 @@ def test_uniform_line_has_roofline_and_cpu_baseline():
     assert cb["kind"] in ("port", "reference") and cb["cores"] >= 1 and cb["value"] > 0
     assert cb["gpu_max_rel_err_vs_fp64_thomas"] <= 1e-4
     assert "workload" in j["config"]
+    sc = j["stream_ceiling"]           # plain stream of the same bytes
+    assert sc["us"] > 0 and sc["bytes"] == 5 * 1024 * 2048 * 4
 
