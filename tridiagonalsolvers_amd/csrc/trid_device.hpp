@@ -28,14 +28,22 @@ __device__ __forceinline__ Eq<T> identity_eq()
 }
 
 // Quotient p / q. FP32 uses the hardware reciprocal (1 ulp) and one
-// multiply; FP64 uses IEEE division.
+// multiply. FP64 refines the hardware reciprocal (about 23 correct bits) by
+// two Newton-Raphson steps, r <- r + r (1 - q r), each of which doubles the
+// correct bits, then multiplies: within about one ulp of p / q for finite,
+// nonzero q, in a third of the instructions of the IEEE-rounded division
+// sequence (on 4,096 systems of 8,192 unknowns the exact division costs 14%
+// of the FP64 solve). A zero q gives a non-finite quotient, as division does.
 __device__ __forceinline__ float tdiv(float p, float q)
 {
     return p * __builtin_amdgcn_rcpf(q);
 }
 __device__ __forceinline__ double tdiv(double p, double q)
 {
-    return p / q;
+    double r = __builtin_amdgcn_rcp(q);
+    r = __builtin_fma(__builtin_fma(-q, r, 1.0), r, r);
+    r = __builtin_fma(__builtin_fma(-q, r, 1.0), r, r);
+    return p * r;
 }
 
 // One CR/PCR reduction step (cr_kernel.cu:53-58, pcr_kernel.cu:39-56):
