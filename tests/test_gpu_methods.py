@@ -158,3 +158,27 @@ def test_long_batch_full_size(oracle, trid, dtype):
     worst = check(oracle, x.cpu().numpy(), a, b, c, d, n, dtype)
     print(f"4096x8192 {np.dtype(dtype).name}: {worst:.3e}", flush=True)
     h.close()
+
+
+def test_fp64_long_systems_batch_independent(oracle, trid):
+    """FP64 systems of 8,192 unknowns (16 per lane, 8 waves): every system of
+    a 1,000-system batch equals its solve in a 100-system batch bit for bit,
+    and the oracle within tolerance; both reduced-system paths (rest <= 8
+    waves, rest > 8) are covered."""
+    import torch
+    n, N, Nsmall = 8192, 1000, 100
+    a, b, c, d = oracle.generate("dominant", 33, n, N, np.float64)
+    dev = [torch.from_numpy(v).cuda() for v in (a, b, c, d)]
+    h = trid.Handle()
+    for rest in (2, 8, 64, 256):
+        o = trid.make_opts(rest=rest)
+        x = torch.full_like(dev[0], float("nan"))
+        print(f"fp64 8192 x {N} rest={rest}", flush=True)
+        h.solve("crpcr", *dev, x, n, N, o)
+        xs = torch.full((n * Nsmall,), float("nan"), dtype=torch.float64, device="cuda")
+        h.solve("crpcr", *(v[: n * Nsmall] for v in dev), xs, n, Nsmall, o)
+        torch.cuda.synchronize()
+        xb = x.cpu().numpy()
+        assert np.array_equal(xs.cpu().numpy(), xb[: n * Nsmall]), rest
+        check(oracle, xb, a, b, c, d, n, np.float64)
+    h.close()
