@@ -30,12 +30,13 @@ def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--workload", default="uniform",
-                   choices=["uniform", "ragged", "serve", "gauss-sweep", "pivoting"],
+                   choices=["uniform", "ragged", "serve", "gauss-sweep", "pivoting", "adi"],
                    help="uniform: the headline batch; ragged: 100,000 systems of "
                         "log-uniform sizes in [16, 8192], FP32 solve + FP64 refinement; "
                         "serve: request server with Poisson arrivals and resident ADI grids; "
                         "pivoting: the partial-pivoting (LAPACK ?gtsv) solve of a batch "
-                        "that needs row interchanges")
+                        "that needs row interchanges; adi: Peaceman-Rachford steps of a "
+                        "resident 2048^2 and 4096^2 FP64 grid")
     p.add_argument("--requests", type=int, default=64)
     p.add_argument("--ragged-mode", default="refine", choices=["refine", "fp64"],
                    help="ragged: FP32 solve + 2 FP64 refinements (the mixed-precision "
@@ -389,6 +390,83 @@ def bench_pivoting(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_adi(args):
+    """N4: Peaceman-Rachford ADI steps (r = 1.5) of the 2D heat equation on
+    resident FP64 grids of 2048^2 and 4096^2 (trid_adi_step: row rhs, row
+    solve, column rhs transposed, column solve, transpose back: 5 passes of
+    16 bytes per point). The 4096^2 grid is the reported workload. Unless
+    --no-cpu, one step on 2048^2 is checked against the oracle's FP64 ADI
+    step and the oracle step is timed on the host as the CPU baseline."""
+    import numpy as np
+    import torch
+
+    from tridiagonalsolvers_amd.adi import AdiGrid
+
+    stream = torch.cuda.current_stream()
+    per = {}
+    for n in (2048, 4096):
+        u = torch.rand(n * n, dtype=torch.float64, device="cuda",
+                       generator=torch.Generator(device="cuda").manual_seed(n))
+        g = AdiGrid(n, u)
+        g.step(1.5, max(1, args.warmup), stream=stream)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        g.step(1.5, args.steps, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / args.steps
+        ms = e0.elapsed_time(e1) / args.steps
+        g.close()
+        per[n] = {"step_ms": ms, "wall_ms_per_step": wall * 1e3,
+                  "points_per_s": n * n / (ms * 1e-3),
+                  "hbm_gbs_80B_per_point": 80 * n * n / (ms * 1e-3) / 1e9}
+    n = 4096
+    out = {"metric": METRIC, "value": n * n / (per[n]["wall_ms_per_step"] * 1e-3),
+           "unit": "grid points/s (one ADI step = two sweeps of n systems of n)",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": per[n]["wall_ms_per_step"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (U[0,1) grids, seeded torch generator), r = 1.5",
+           "config": {"workload": "adi f64 4096^2 resident grid, Peaceman-Rachford step",
+                      "grids": {str(k): v for k, v in per.items()}},
+           "roofline": {"bound": "hbm", "achieved": per[n]["hbm_gbs_80B_per_point"],
+                        "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": per[n]["hbm_gbs_80B_per_point"] / PEAK_HBM_GBS,
+                        "traffic": None,
+                        "note": "80 B per point and step: 5 passes (row rhs, row solve, "
+                                "column rhs + transpose, column solve, transpose back) of "
+                                "16 B each"}}
+    if not args.no_cpu:
+        from oracle import oracle as O
+
+        m = 2048
+        u0 = np.random.default_rng(5).random(m * m)
+        dev = torch.from_numpy(u0).cuda()
+        g = AdiGrid(m, dev)
+        g.step(1.5, 1, stream=stream)
+        torch.cuda.synchronize()
+        reps, t1 = 0, time.perf_counter()
+        while True:
+            ref = O.adi_step_np(u0.reshape(m, m), 1.5)
+            reps += 1
+            el = time.perf_counter() - t1
+            if el >= args.cpu_seconds:
+                break
+        got = dev.cpu().numpy().reshape(m, m)
+        err = float(np.abs(got - ref).max() / np.abs(ref).max())
+        g.close()
+        out["cpu_baseline"] = {
+            "value": reps * m * m / el, "unit": "grid points/s", "cores": O.max_threads(),
+            "kind": "port",
+            "sample": f"oracle FP64 ADI step on 2048^2 x {reps}, {el:.1f} s (Thomas per "
+                      f"sweep, OpenMP over rows)",
+            "gpu_max_rel_err_vs_oracle_step": err}
+    print(json.dumps(out), flush=True)
+
+
 def copy_ceiling(a, b, c, d, stream, reps=10):
     """Bandwidth of a plain device copy of the four input arrays (torch's copy
     kernel): read + write bytes over HIP-event time, the achievable HBM rate
@@ -521,8 +599,8 @@ def bench_ragged(args):
     data; one graph replay of the persistent ragged kernel against plain FP64
     launches, one per distinct size, on the same data (sorted by size so each
     size is contiguous). The ragged kernel is timed in both of its modes (one
-    FP64 solve; FP32 solve + 2 FP64 refinements); --ragged-mode picks the
-    reported value."""
+    FP64 solve; FP32 solve + 2 FP64 refinements) and with one refinement;
+    --ragged-mode picks the reported value."""
     import numpy as np
     import torch
 
@@ -582,8 +660,14 @@ def bench_ragged(args):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps, e0.elapsed_time(e1) / steps
 
+    x_one = torch.empty_like(a)
+
+    def one_step():
+        plan_refine.solve(a, b, c, d, x_one, iters=1, stream=stream)
+
     wall_r, gpu_r = timed(ragged_step, args.steps, args.warmup)
     wall_o, gpu_o = timed(other_step, args.steps, args.warmup)
+    wall_1, gpu_1 = timed(one_step, args.steps, args.warmup)
     wall_p, gpu_p = timed(plain_step, max(1, args.steps // 5), 1)
     d_offs = torch.from_numpy(offs).to(dev)
     rr = torch.empty(N, dtype=torch.float64, device=dev)
@@ -591,10 +675,14 @@ def bench_ragged(args):
     rr_r = float(rr.max())
     R.residual(a, b, c, d, x_other, d_offs, N, rr, stream=stream)
     rr_o = float(rr.max())
+    R.residual(a, b, c, d, x_one, d_offs, N, rr, stream=stream)
+    rr_1 = float(rr.max())
     names = {True: "fp64 (one FP64 solve)", False: "refine (FP32 solve + 2 FP64 refinements)"}
     modes = {names[fp64]: {"value": total / wall_r, "kernel_ms": gpu_r, "max_rel_residual": rr_r},
              names[not fp64]: {"value": total / wall_o, "kernel_ms": gpu_o,
-                               "max_rel_residual": rr_o}}
+                               "max_rel_residual": rr_o},
+             "refine (FP32 solve + 1 FP64 refinement)": {
+                 "value": total / wall_1, "kernel_ms": gpu_1, "max_rel_residual": rr_1}}
     R.residual(a, b, c, d, x_plain, d_offs, N, rr, stream=stream)
     rr_p = float(rr.max())
     alg = 5 * 8 * total
@@ -899,6 +987,9 @@ def main():
         return
     if args.workload == "pivoting":
         bench_pivoting(args)
+        return
+    if args.workload == "adi":
+        bench_adi(args)
         return
     if args.layout == "interleaved":
         bench_interleaved(args)

@@ -44,3 +44,18 @@ def test_uniform_line_has_roofline_and_cpu_baseline():
     sc = j["stream_ceiling"]           # plain stream of the same bytes
     assert sc["us"] > 0 and sc["bytes"] == 5 * 1024 * 2048 * 4
 
+
+
+def test_adi_and_pivoting_lines():
+    j = _json(_run(["--workload", "adi", "--steps", "2", "--warmup", "1",
+                    "--cpu-seconds", "0.2"]))
+    for k in KEYS:
+        assert k in j, k
+    assert j["dtype"] == "f64" and j["value"] > 0 and j["roofline"]["frac"] > 0
+    assert j["cpu_baseline"]["gpu_max_rel_err_vs_oracle_step"] <= 1e-11
+    j = _json(_run(["--workload", "pivoting", "--n", "512", "--systems", "4096",
+                    "--steps", "2", "--warmup", "1", "--cpu-seconds", "0.2"]))
+    for k in KEYS:
+        assert k in j, k
+    cb = j["cpu_baseline"]
+    assert cb["gpu_bitwise_equal_to_oracle"] and cb["gpu_info_equal_to_oracle"]
