@@ -53,6 +53,22 @@ def test_sizes_beyond_the_kernels_are_refused(trid):
     h.close()
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_pcrgauss_two_pass_beyond_65535_systems(oracle, trid, dtype):
+    """The two-pass pcrGauss form runs a one-dimensional grid of tiles x
+    systems, so batches of more than 65,535 systems solve (70,000 here)."""
+    import torch
+    n, N = 64, 70000
+    a, b, c, d = oracle.generate("dominant", 12, n, N, dtype)
+    dev = [torch.from_numpy(v).cuda() for v in (a, b, c, d)]
+    x = torch.full_like(dev[0], float("nan"))
+    h = trid.Handle()
+    h.solve("pcrgauss", *dev, x, n, N, trid.make_opts(gaussian=4, flags=1))
+    torch.cuda.synchronize()
+    check(oracle, x.cpu().numpy(), a, b, c, d, n, dtype)
+    h.close()
+
+
 def test_offsets_beyond_2_31_elements(oracle, trid):
     import torch
     n, N = 1024, (1 << 21) + 3          # 2,147,486,720 elements per array

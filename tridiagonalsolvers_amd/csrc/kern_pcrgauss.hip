@@ -158,7 +158,7 @@ template <typename T, int KPT>
 __global__ __launch_bounds__(TRID_PCRG_THREADS) void pcr_global_levels_kernel(
     const T* __restrict__ A, const T* __restrict__ B, const T* __restrict__ C,
     const T* __restrict__ D, T* __restrict__ A2, T* __restrict__ B2, T* __restrict__ C2,
-    T* __restrict__ D2, int64_t n, int G, int tile)
+    T* __restrict__ D2, int64_t n, int G, int tile, int64_t tiles)
 {
     extern __shared__ __align__(16) char trid_smem[];
     const int H = G - 1;
@@ -166,9 +166,10 @@ __global__ __launch_bounds__(TRID_PCRG_THREADS) void pcr_global_levels_kernel(
     Eq<T>* se = reinterpret_cast<Eq<T>*>(trid_smem);
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
-    const int64_t sys = blockIdx.y;
+    // one-dimensional grid of tiles x systems (no 65,535 cap on the systems)
+    const int64_t sys = (int64_t)blockIdx.x / tiles;
     const int64_t base = sys * n;
-    const int64_t t0 = (int64_t)blockIdx.x * tile;
+    const int64_t t0 = ((int64_t)blockIdx.x - sys * tiles) * tile;
 
     Eq<T> own[KPT];
 #pragma unroll
@@ -663,7 +664,7 @@ hipError_t launch_pcr_thomas(const PcrThomasArgs<T>& g)
     const size_t lds = (size_t)len * sizeof(Eq<T>);
     if (lds > 160 * 1024 || kpt > max_kpt) return hipErrorInvalidConfiguration;
     const int64_t tiles = (g.n + tile - 1) / tile;
-    if (tiles > 0x7fffffffLL || g.N > 65535) return hipErrorInvalidConfiguration;
+    if (tiles > 0x7fffffffLL / g.N) return hipErrorInvalidConfiguration;
     {
         auto go = [&](auto kern) -> hipError_t {
             if (lds > 64 * 1024) {
@@ -671,8 +672,8 @@ hipError_t launch_pcr_thomas(const PcrThomasArgs<T>& g)
                     (const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
                 if (e != hipSuccess) return e;
             }
-            hipLaunchKernelGGL(kern, dim3((unsigned)tiles, (unsigned)g.N), dim3(nt), lds,
-                               g.stream, g.a, g.b, g.c, g.d, A2, B2, C2, D2, g.n, G, tile);
+            hipLaunchKernelGGL(kern, dim3((unsigned)(tiles * g.N)), dim3(nt), lds, g.stream,
+                               g.a, g.b, g.c, g.d, A2, B2, C2, D2, g.n, G, tile, tiles);
             return hipGetLastError();
         };
         // G = 1 runs no PCR level: pass 1 only copies (and zeroes a[0], c[n-1])

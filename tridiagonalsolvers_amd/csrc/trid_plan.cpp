@@ -174,7 +174,6 @@ int make_plan(int method, int64_t n, int64_t N, int elem_size, const trid_opts& 
             p->two_pass = (!fits || (o.flags & TRID_FLAG_TWO_PASS)) ? 1 : 0;
             p->pad = p->two_pass ? 0 : pad;
             if (p->two_pass && G > (elem_size == 4 ? 1024 : 512)) return TRID_ENOTSUP;
-            if (p->two_pass && N > 65535) return TRID_ENOTSUP;
             return TRID_OK;
         }
         default:
