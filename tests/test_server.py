@@ -2,7 +2,8 @@
 device work, nothing computed): admission never exceeds the HBM budget,
 requests that can never fit are rejected, queued requests can be cancelled,
 a failing batch fails only its own requests, operators are reference
-counted, and concurrent submitters are all served."""
+counted, concurrent submitters are all served, and ADI steps do not queue
+behind solve batches."""
 import threading
 
 import numpy as np
@@ -149,6 +150,30 @@ def test_adi_requests_are_serialized_per_grid(S):
         for a, b in zip(rs, rs[1:]):
             assert b["latency_us"] - b["queue_us"] >= 0
     assert srv.release_adi(g1) == 0 and srv.release_adi(g2) == 0
+    srv.close()
+
+
+def test_adi_steps_do_not_queue_behind_solve_batches(S):
+    # two slots, 50 ms batches, one system per batch: the solves keep slot 1
+    # busy back to back, while an ADI step submitted behind them starts at
+    # once on slot 0 (kept for steps while a grid exists)
+    srv = S.Server(fake=True, slots=2, fake_batch_us=50000, max_batch_systems=1,
+                   batch_window_us=0)
+    g = srv.create_adi(np.zeros(64 * 64))
+    rng = np.random.default_rng(3)
+    solves = []
+    for _ in range(4):
+        offs, (a, b, c, d), x = _req(rng, most=2)
+        solves.append(srv.submit(d, x, offsets=offs, a=a, b=b, c=c))
+    step = srv.submit_adi(g, 1.0)
+    r = srv.wait(step)
+    assert r["state"] == "done" and r["queue_us"] < 20000, r
+    rs = [srv.wait(t) for t in solves]
+    assert all(q["state"] == "done" for q in rs)
+    # the solves ran one after another on the one slot left to them
+    assert sorted(q["batch"] for q in rs) == [q["batch"] for q in rs]
+    assert max(q["queue_us"] for q in rs) > 100000
+    assert srv.release_adi(g) == 0
     srv.close()
 
 

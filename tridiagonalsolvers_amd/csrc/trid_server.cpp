@@ -17,7 +17,10 @@
 //     copied device-to-device into the batch;
 //   * keeps ADI grids resident across time steps: an ADI request advances
 //     its grid on a slot of its own (steps on one grid never overlap and keep
-//     their submission order), interleaved with the solve batches;
+//     their submission order), interleaved with the solve batches; steps go
+//     ahead of solve batches and skip the batching window, and while grids
+//     exist (and there are two slots or more) slot 0 takes no solve batch, so
+//     a step does not queue behind solves;
 //   * reserves its whole budget once (one device allocation carved by a
 //     first-fit allocator for batches and operators) and one pinned staging
 //     buffer per slot, so serving allocates nothing per request;
@@ -470,9 +473,55 @@ void trid_server_s::run()
         if (stopping && pending.empty() && !any_busy) break;
 
         // ---- form and launch batches on free slots ------------------------
-        for (Slot& s : slots) {
+        for (size_t si = 0; si < slots.size(); ++si) {
+            Slot& s = slots[si];
             if (s.busy || pending.empty()) continue;
             const Clock::time_point now = Clock::now();
+            // ADI steps first: each runs alone, so it does not wait for a
+            // batching window; the first pending step of a grid that is not
+            // running goes (one at a time per grid, in submission order)
+            Req* adi_q = nullptr;
+            {
+                std::vector<int> seen;
+                for (auto it = pending.begin(); it != pending.end(); ++it) {
+                    const int g = (*it)->r.adi;
+                    if (g < 0 || std::find(seen.begin(), seen.end(), g) != seen.end()) continue;
+                    seen.push_back(g);
+                    if (!adis[g].running) {
+                        adi_q = *it;
+                        pending.erase(it);
+                        break;
+                    }
+                }
+            }
+            if (adi_q) {
+                s.busy = true;
+                s.reqs.assign(1, adi_q);
+                s.total = 0;
+                s.N = 0;
+                s.dev_off = 0;
+                s.dev_bytes = 0;
+                s.batch = next_batch++;
+                st.batches++;
+                adi_q->state = kRunning;
+                adi_q->t_launch = now;
+                adi_q->batch = s.batch;
+                adi_q->batch_systems = 2 * adis[adi_q->r.adi].n;
+                const int rc = launch(s);
+                if (rc != TRID_OK) {
+                    finish(adi_q, kFailed, rc);
+                    s.reqs.clear();
+                    s.busy = false;
+                    cv_done.notify_all();
+                }
+                continue;
+            }
+            // with resident ADI grids and more than one slot, slot 0 is kept
+            // for their steps, so a step never queues behind solve batches
+            if (si == 0 && slots.size() >= 2 && !adis.empty()) continue;
+            bool any_solve = false;
+            for (Req* q : pending) any_solve = any_solve || q->r.adi < 0;
+            if (!any_solve) continue;
             const bool device_busy = std::any_of(slots.begin(), slots.end(),
                                                  [](const Slot& x) { return x.busy; });
             // while the device is busy, let arrivals accumulate for up to the
@@ -483,26 +532,13 @@ void trid_server_s::run()
             if (device_busy && age < o.batch_window_us && waiting < o.max_batch_unknowns &&
                 !stopping)
                 continue;
-            // FIFO selection within limits and the memory budget
+            // FIFO selection of solve requests within limits and the memory budget
             int64_t total = 0, N = 0;
             std::vector<Req*> pick;
             const size_t free_b = budget_free();
-            bool adi_pick = false;
             for (auto it = pending.begin(); it != pending.end();) {
                 Req* q = *it;
-                if (q->r.adi >= 0) {
-                    // an ADI step runs alone on its slot, one at a time per grid,
-                    // in submission order
-                    const Adi& a = adis[q->r.adi];
-                    bool earlier = false;
-                    for (auto jt = pending.begin(); jt != it; ++jt)
-                        if ((*jt)->r.adi == q->r.adi) earlier = true;
-                    if (pick.empty() && !a.running && !earlier) {
-                        pick.push_back(q);
-                        pending.erase(it);
-                        adi_pick = true;
-                        break;
-                    }
+                if (q->r.adi >= 0) {  // waits for its grid (taken above when free)
                     ++it;
                     continue;
                 }
@@ -524,28 +560,6 @@ void trid_server_s::run()
             if (pick.empty()) break;
             const BatchLayout L(total, N);
             size_t off = 0;
-            if (adi_pick) {
-                s.busy = true;
-                s.reqs = pick;
-                s.total = 0;
-                s.N = 0;
-                s.dev_off = 0;
-                s.dev_bytes = 0;
-                s.batch = next_batch++;
-                st.batches++;
-                pick[0]->state = kRunning;
-                pick[0]->t_launch = now;
-                pick[0]->batch = s.batch;
-                pick[0]->batch_systems = 2 * adis[pick[0]->r.adi].n;
-                const int rc = launch(s);
-                if (rc != TRID_OK) {
-                    finish(pick[0], kFailed, rc);
-                    s.reqs.clear();
-                    s.busy = false;
-                    cv_done.notify_all();
-                }
-                continue;
-            }
             if (!pool.alloc(L.bytes, &off)) {
                 // fragmented: put them back in order and wait for retirements
                 for (auto r = pick.rbegin(); r != pick.rend(); ++r) pending.push_front(*r);
