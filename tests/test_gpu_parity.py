@@ -159,9 +159,11 @@ def test_batch_position_invariance(oracle, trid):
             assert np.array_equal(xs, x[sl])
 
 
-def test_zero_pivot_gives_nonfinite_not_crash(oracle, trid):
-    n, N = 64, 4
-    a, b, c, d = oracle.generate("dominant", 2, n, N, np.float32)
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n", [64, 8192])
+def test_zero_pivot_gives_nonfinite_not_crash(oracle, trid, dtype, n):
+    N = 4
+    a, b, c, d = oracle.generate("dominant", 2, n, N, dtype)
     b[n + 5] = 0.0
     a[n + 5] = 0.0
     c[n + 5] = 0.0   # system 1 is singular
@@ -170,7 +172,7 @@ def test_zero_pivot_gives_nonfinite_not_crash(oracle, trid):
     assert not np.all(np.isfinite(X[1]))
     check(oracle, np.concatenate([X[0], X[2], X[3]]),
           *(np.concatenate([v.reshape(N, n)[i] for i in (0, 2, 3)]) for v in (a, b, c, d)),
-          n, np.float32)
+          n, dtype)
 
 
 def test_device_generator_bitwise_equals_oracle(oracle, trid):
