@@ -190,8 +190,11 @@ def bench_interleaved(args):
            "roofline": {"bound": "hbm", "achieved": alg / (avg * 1e-3) / 1e9,
                         "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "frac": alg / (avg * 1e-3) / 1e9 / PEAK_HBM_GBS, "traffic": None,
-                        "note": "the two-ended elimination moves 9 x T bytes per unknown "
-                                "(reads a, b, c, d; writes and re-reads c', d'; writes x)"}}
+                        "note": ("512 < n <= 1024: register CR-PCR on systems gathered "
+                                 "through LDS, 5 x T bytes per unknown moved"
+                                 if 512 < n <= 1024 else
+                                 "the two-ended elimination moves 9 x T bytes per unknown "
+                                 "(reads a, b, c, d; writes and re-reads c', d'; writes x)")}}
     if not args.no_vendor:
         rs = ctypes.CDLL("librocsparse.so")
         vp = ctypes.c_void_p

@@ -189,12 +189,14 @@ int trid_solve_refined_f64(trid_handle handle, int method,
 /*
  * Interleaved batch: row i of system j at i * batchStride + j (batchStride >=
  * numSystems), the layout of rocSPARSE's gtsv_interleaved_batch. No
- * counterpart in the reference (system-major only). Each system is solved by
- * Gaussian elimination from both ends at once (two lanes per system, every
- * access coalesced across systems); the a of row 0 and the c of the last
- * row are ignored; x may be d (in place) but no other input. Uses the
- * handle's workspace (systemSize * batchStride elements), so calls on one
- * handle must not overlap on different streams.
+ * counterpart in the reference (system-major only). Systems of 513..1024
+ * unknowns are gathered through LDS, a few per workgroup, and solved by the
+ * register CR-PCR kernel (one wave each); other sizes by Gaussian elimination
+ * from both ends at once (two lanes per system, every access coalesced
+ * across systems). The a of row 0 and the c of the last row are ignored; x
+ * may be d (in place) but no other input. Uses the handle's workspace
+ * (systemSize * batchStride elements), so calls on one handle must not
+ * overlap on different streams.
  */
 int trid_solve_interleaved_f32(trid_handle handle, const float* a, const float* b,
                                const float* c, const float* d, float* x, int64_t systemSize,

@@ -62,3 +62,37 @@ def test_interleaved_in_place(oracle, trid, dtype):
     assert torch.equal(x_sep, dev[3])
     check(oracle, _deinterleave(x_sep.cpu().numpy(), n, N, N), a, b, c, d, n, dtype)
     h.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_interleaved_register_form(oracle, trid, dtype):
+    """512 < n <= 1024 runs one wave per system on the register CR-PCR core
+    (the systems gathered through LDS): partial groups of systems, strides
+    that break 16-byte vectors, an odd number of rows, unaligned arrays, x = d
+    in place, and the whole 65,536 x 1,024 batch."""
+    import torch
+    h = trid.Handle()
+    cases = ((1024, 17, 20, 0), (513, 100, 101, 0), (1000, 3000, 3100, 0), (1024, 4096, 4096, 1),
+             (777, 250, 256, 0), (1024, 65536, 65536, 0))
+    for n, N, stride, offset in cases:
+        a, b, c, d = oracle.generate("dominant", 40 + n, n, N, dtype)
+        dev = []
+        for v in (a, b, c, d):
+            t = torch.empty(n * stride + offset, dtype=torch.from_numpy(v[:1]).dtype,
+                            device="cuda")[offset:]
+            t.copy_(torch.from_numpy(_interleave(v, n, N, stride)))
+            dev.append(t)
+        x = torch.full_like(dev[0], float("nan"))
+        print(f"{np.dtype(dtype).name} register form n={n} N={N} stride={stride} "
+              f"offset={offset}", flush=True)
+        h.solve_interleaved(*dev, x, n, N, stride)
+        torch.cuda.synchronize()
+        xs = _deinterleave(x.cpu().numpy(), n, N, stride)
+        check(oracle, xs, a, b, c, d, n, dtype)
+        if stride > N:
+            assert np.all(np.isnan(x.cpu().numpy().reshape(n, stride)[:, N:]))
+        # in place: x = d gives the same bits
+        h.solve_interleaved(*dev, dev[3], n, N, stride)
+        torch.cuda.synchronize()
+        assert torch.equal(dev[3].view(n, stride)[:, :N], x.view(n, stride)[:, :N])
+    h.close()

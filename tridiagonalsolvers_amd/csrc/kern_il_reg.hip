@@ -1,0 +1,221 @@
+// kern_il_reg.hip -- interleaved batches (row i of system s at i*stride + s)
+// solved by the register-resident CR-PCR core instead of a lane-per-system
+// elimination.
+//
+// No counterpart in the reference (system-major only). A lane-per-system
+// elimination reads every row coalesced across systems but must keep its
+// factors in memory: 9 elements of traffic per unknown. Here a workgroup of
+// S waves takes S consecutive systems (S = 16 FP32, 8 FP64) and gives each
+// wave one system in the layout of the register core (lane l holds unknowns
+// [16 l, 16 l + 16)), so the solve moves only the 5 elements per unknown it
+// must (read a, b, c, d; write x). The gather is the transposition: a row of
+// the S systems is 64 contiguous bytes, read as 16-byte vectors, one per
+// thread and array per chunk of 256 (FP32) / 128 (FP64) rows; the chunk is scattered
+// into LDS by system and lane group, and the 16 (8) lanes of every wave that
+// own those rows read their 16 consecutive rows back as 16-byte vectors. The
+// solution leaves the same way. LDS layout per array: [system][lane group]
+// [16 rows + 16 bytes of padding], plus 16 bytes between systems: the per-lane
+// vector reads fall on distinct banks and the scattering writes of a
+// half-wave conflict at most 2-way.
+//
+// Rows past n are identity rows and systems past N are never stored, so any
+// 512 < n <= 1024 and any N work; a[first] and c[last] are ignored. x may be
+// d: a workgroup reads all of its systems' rows before it writes any x.
+#include "reg_core.hpp"
+#include "reg_io.hpp"
+#include "trid_device.hpp"
+#include "trid_launch.hpp"
+
+namespace trid {
+
+namespace {
+
+constexpr int kIlM = 16;  // unknowns per lane: n_pad = 1024
+
+// S systems (waves) per workgroup: 16 in FP32 (1024 threads, 109 VGPRs), 8 in
+// FP64 (512 threads: the FP64 core needs about 210 VGPRs). Either way a row of
+// the workgroup's systems is 64 bytes.
+template <typename T>
+struct IlGeom {
+    static constexpr int S = sizeof(T) == 4 ? 16 : 8;
+    static constexpr int EV = 16 / (int)sizeof(T);           // elements per vector
+    static constexpr int VPR = S / EV;                        // vectors per row
+    static constexpr int RC = S * kWave / VPR;                // rows per chunk
+    static constexpr int LC = RC / kIlM;                      // lanes per chunk
+    static constexpr int NC = kWave / LC;                     // chunks per system
+    static constexpr int GP = kIlM + EV;                      // padded lane group
+    static constexpr int SP = LC * GP + EV;                   // system pitch
+    static constexpr int AP = S * SP;                         // array pitch
+    static_assert(LC * NC == kWave && S * (int)sizeof(T) == 64, "chunk geometry");
+};
+
+// Workgroup -> group of S systems. A 64-byte row is half a 128-byte line; the
+// other half belongs to the neighbouring group. Workgroups are dealt to the 8
+// XCDs round-robin, so b and b + 8 run at about the same time on one XCD:
+// they take the two halves of the same lines, and the second read of a line
+// hits that XCD's L2. (The grid is a multiple of 16 so the map is one-to-one.)
+__device__ __forceinline__ int64_t il_group(int64_t b)
+{
+    const int64_t x = b % 8, k = b / 8;
+    return 2 * ((k / 2) * 8 + x) + (k % 2);
+}
+
+template <typename T>
+__device__ __forceinline__ int il_addr(int sys, int row_in_chunk)
+{
+    using G = IlGeom<T>;
+    return sys * G::SP + (row_in_chunk / kIlM) * G::GP + (row_in_chunk % kIlM);
+}
+
+// One vector of array `arr` for chunk `c`: row c*RC + t/VPR, systems
+// s0 + (t%VPR)*EV .. +EV-1. Identity values past n; nothing read past N.
+template <typename T>
+__device__ __forceinline__ void il_fetch(const T* __restrict__ P, int arr, int c, int t,
+                                         int64_t n, int64_t N, int64_t stride, int64_t s0,
+                                         bool vec, T (&v)[IlGeom<T>::EV])
+{
+    using G = IlGeom<T>;
+    const int64_t row = (int64_t)c * G::RC + t / G::VPR;
+    const int64_t s = s0 + (t % G::VPR) * G::EV;
+    const T fill = arr == 1 ? T(1) : T(0);
+    const int64_t off = row * stride + s;
+    if (row < n && vec && s + G::EV <= N) {
+        const u32x4 q = gload16<false>(P + off);
+        __builtin_memcpy(v, &q, 16);
+    } else {
+#pragma unroll
+        for (int e = 0; e < G::EV; ++e) v[e] = (row < n && s + e < N) ? P[off + e] : fill;
+    }
+    if (arr == 0 && row == 0) {
+#pragma unroll
+        for (int e = 0; e < G::EV; ++e) v[e] = T(0);
+    }
+    if (arr == 2 && row == n - 1) {
+#pragma unroll
+        for (int e = 0; e < G::EV; ++e) v[e] = T(0);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(IlGeom<T>::S * kWave) void il_reg_kernel(
+    const T* __restrict__ A, const T* __restrict__ B, const T* __restrict__ C,
+    const T* __restrict__ D, T* X, int64_t n, int64_t N, int64_t stride, int rest, int vec)
+{
+    using G = IlGeom<T>;
+    __shared__ __align__(16) T tile[4 * G::AP];
+    const int t = threadIdx.x;
+    const int w = t / kWave;
+    const int lane = t & (kWave - 1);
+    const int64_t s0 = il_group(blockIdx.x) * G::S;
+    if (s0 >= N) return;  // padding workgroup: the whole block leaves
+    const T* P[4] = {A, B, C, D};
+
+    // NB chunks in flight: chunk c + NB is fetched into the buffer chunk c
+    // has just left (FP64: 2; FP32: 1, a second buffer would spill)
+    constexpr int NB = sizeof(T) == 8 ? 2 : 1;
+    T pre[NB][4][G::EV];
+#pragma unroll
+    for (int c = 0; c < NB && c < G::NC; ++c)
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            il_fetch<T>(P[k], k, c, t, n, N, stride, s0, vec != 0, pre[c][k]);
+
+    Eq<T> e[kIlM];
+#pragma unroll
+    for (int c = 0; c < G::NC; ++c) {
+        __syncthreads();  // the previous chunk has been read back
+        {
+            const int r = t / G::VPR;
+            const int sb = (t % G::VPR) * G::EV;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int q = 0; q < G::EV; ++q)
+                    tile[k * G::AP + il_addr<T>(sb + q, r)] = pre[c % NB][k][q];
+        }
+        __syncthreads();
+        if (c + NB < G::NC) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                il_fetch<T>(P[k], k, c + NB, t, n, N, stride, s0, vec != 0, pre[c % NB][k]);
+        }
+        if (lane / G::LC == c) {
+            const int base = w * G::SP + (lane % G::LC) * G::GP;
+            T v[4][kIlM];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int q = 0; q < kIlM / G::EV; ++q) {
+                    const u32x4 x4 =
+                        *reinterpret_cast<const u32x4*>(&tile[k * G::AP + base + q * G::EV]);
+                    __builtin_memcpy(&v[k][q * G::EV], &x4, 16);
+                }
+#pragma unroll
+            for (int i = 0; i < kIlM; ++i) e[i] = Eq<T>{v[0][i], v[1][i], v[2][i], v[3][i]};
+        }
+    }
+
+    T xv[kIlM];
+    wave_core<T, kIlM, 1>(e, xv, lane, lane, kWave, rest);
+
+    for (int c = 0; c < G::NC; ++c) {
+        __syncthreads();
+        if (lane / G::LC == c) {
+            const int base = w * G::SP + (lane % G::LC) * G::GP;
+#pragma unroll
+            for (int q = 0; q < kIlM / G::EV; ++q) {
+                u32x4 x4;
+                __builtin_memcpy(&x4, &xv[q * G::EV], 16);
+                *reinterpret_cast<u32x4*>(&tile[base + q * G::EV]) = x4;
+            }
+        }
+        __syncthreads();
+        const int r = t / G::VPR;
+        const int sb = (t % G::VPR) * G::EV;
+        const int64_t row = (int64_t)c * G::RC + r;
+        const int64_t s = s0 + sb;
+        if (row < n) {
+            T v[G::EV];
+#pragma unroll
+            for (int q = 0; q < G::EV; ++q) v[q] = tile[il_addr<T>(sb + q, r)];
+            const int64_t off = row * stride + s;
+            if (vec && s + G::EV <= N) {
+                u32x4 x4;
+                __builtin_memcpy(&x4, v, 16);
+                gstore16<false>(X + off, x4);
+            } else {
+#pragma unroll
+                for (int q = 0; q < G::EV; ++q)
+                    if (s + q < N) X[off + q] = v[q];
+            }
+        }
+    }
+}
+
+bool il_aligned(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+}  // namespace
+
+template <typename T>
+hipError_t launch_interleaved_reg(const InterleavedArgs<T>& g, bool* taken)
+{
+    using G = IlGeom<T>;
+    *taken = false;
+    if (g.n <= kWave * kIlM / 2 || g.n > kWave * kIlM || g.N <= 0) return hipSuccess;
+    const int64_t groups = (g.N + G::S - 1) / G::S;
+    const int64_t grid = (groups + 15) / 16 * 16;  // il_group is one-to-one on these
+    if (grid > 0x7fffffffLL) return hipSuccess;
+    const int vec = (g.stride % G::EV == 0 && il_aligned(g.a) && il_aligned(g.b) &&
+                     il_aligned(g.c) && il_aligned(g.d) && il_aligned(g.x))
+                        ? 1
+                        : 0;
+    *taken = true;
+    hipLaunchKernelGGL((il_reg_kernel<T>), dim3((unsigned)grid), dim3(G::S * kWave), 0,
+                       g.stream, g.a, g.b, g.c, g.d, g.x, g.n, g.N, g.stride, kWave, vec);
+    return hipGetLastError();
+}
+
+template hipError_t launch_interleaved_reg<float>(const InterleavedArgs<float>&, bool*);
+template hipError_t launch_interleaved_reg<double>(const InterleavedArgs<double>&, bool*);
+
+}  // namespace trid

@@ -610,8 +610,13 @@ template <typename T>
 hipError_t launch_interleaved(const InterleavedArgs<T>& g)
 {
     if (g.N <= 0) return hipSuccess;
-    // one block of n * stride elements whose N chains are the systems; the
-    // c' factors go to the workspace, the d' factors to x
+    // 512 < n <= 1024: one wave per system on the register CR-PCR core,
+    // gathered through LDS (kern_il_reg.hip), 5 elements of traffic per unknown
+    bool taken = false;
+    const hipError_t e = launch_interleaved_reg<T>(g, &taken);
+    if (taken || e != hipSuccess) return e;
+    // otherwise one block of n * stride elements whose N chains are the
+    // systems; the c' factors go to the workspace, the d' factors to x
     return launch_strided<T>(g.a, g.b, g.c, g.d, g.ws, g.x, g.x, g.n * g.stride, 1, g.N,
                              g.stride, g.stream);
 }
