@@ -49,6 +49,9 @@ def parse():
     p.add_argument("--systems", type=int, default=65536)
     p.add_argument("--rest", type=int, default=64)
     p.add_argument("--dtype", default="f32", choices=["f32", "f64"])
+    p.add_argument("--batch-stride", type=int, default=0,
+                   help="uniform batches: system s at s * stride (>= n) through "
+                        "trid_solve_strided (rocSPARSE's strided-batch layout); timing only")
     p.add_argument("--kind", default="dominant", choices=["dominant", "weak"],
                    help="uniform batches: strongly (b = |a|+|c|+U[1,2]) or weakly "
                         "(b = |a|+|c|+U[0.01,0.1]) diagonally dominant inputs")
@@ -1006,10 +1009,14 @@ def main():
     esize = 4 if args.dtype == "f32" else 8
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    a, b, c, d = (torch.empty(n * N, dtype=tdt, device=dev) for _ in range(4))
+    bs = args.batch_stride if args.batch_stride > 0 else n
+    if bs < n:
+        raise SystemExit("--batch-stride must be at least --n")
+    a, b, c, d = (torch.empty(bs * N, dtype=tdt, device=dev) for _ in range(4))
     x = torch.empty_like(a)
     stream = torch.cuda.current_stream()
-    T.generate(0 if args.kind == "dominant" else 1, 1, a, b, c, d, n, N, stream=stream)
+    # strided: rows of systems of size bs, of which the first n form each system
+    T.generate(0 if args.kind == "dominant" else 1, 1, a, b, c, d, bs, N, stream=stream)
     h = T.Handle()
     opts = T.make_opts(rest=args.rest if args.method in ("crpcr", "crpcrnbc") else 0,
                        gaussian=args.rest if args.method == "pcrgauss" else 0,
@@ -1018,7 +1025,10 @@ def main():
     plan = T.plan_query(args.method, n, N, esize, opts)
 
     def step():
-        h.solve(args.method, a, b, c, d, x, n, N, opts, stream=stream)
+        if bs != n:
+            h.solve_strided(args.method, a, b, c, d, x, n, N, bs, opts, stream=stream)
+        else:
+            h.solve(args.method, a, b, c, d, x, n, N, opts, stream=stream)
 
     for _ in range(args.warmup):
         step()
@@ -1045,6 +1055,8 @@ def main():
     workload = f"{args.method} {args.dtype} {N} systems x {n} unknowns"
     if args.kind != "dominant":
         workload += f", {args.kind}ly dominant"
+    if bs != n:
+        workload += f", batch stride {bs}"
     traffic = traffic_from_profiles(workload)
     out = {
         "metric": METRIC,
@@ -1068,6 +1080,8 @@ def main():
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": traffic},
     }
+    if bs != n:
+        args.no_vendor = args.no_cpu = True  # the checks below assume a contiguous batch
     if not args.no_vendor:
         out["vendor_baseline"] = vendor_baseline(a, b, c, d, x, n, N, args.dtype, stream)
         out["copy_ceiling"] = copy_ceiling(a, b, c, d, stream)

@@ -187,6 +187,25 @@ int trid_solve_refined_f64(trid_handle handle, int method,
                            const trid_opts* opts, void* stream);
 
 /*
+ * Strided batch: system s occupies [s * batchStride, s * batchStride +
+ * systemSize) of each array (batchStride >= systemSize), the layout of
+ * rocSPARSE's gtsv_no_pivot_strided_batch; batchStride == systemSize is the
+ * contiguous batch of trid_solve_*. No counterpart in the reference
+ * (contiguous only). Methods TRID_CR, TRID_PCR, TRID_CRPCR and TRID_AUTO on
+ * the register kernels (systemSize up to 16384 FP32 / 8192 FP64); other
+ * methods or sizes return TRID_ENOTSUP. Elements between systems are neither
+ * read nor written.
+ */
+int trid_solve_strided_f32(trid_handle handle, int method, const float* a, const float* b,
+                           const float* c, const float* d, float* x, int64_t systemSize,
+                           int64_t numSystems, int64_t batchStride, const trid_opts* opts,
+                           void* stream);
+int trid_solve_strided_f64(trid_handle handle, int method, const double* a, const double* b,
+                           const double* c, const double* d, double* x, int64_t systemSize,
+                           int64_t numSystems, int64_t batchStride, const trid_opts* opts,
+                           void* stream);
+
+/*
  * Interleaved batch: row i of system j at i * batchStride + j (batchStride >=
  * numSystems), the layout of rocSPARSE's gtsv_interleaved_batch. No
  * counterpart in the reference (system-major only). Systems of 513..1024

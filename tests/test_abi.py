@@ -95,6 +95,17 @@ def test_device_api_rejects_bad_arguments_without_a_device(trid):
         h.solve_pivoting(16, 32, 48, 64, 80, 8, 10, batchStride=0)
     with pytest.raises(trid.TridError):
         h.solve_pivoting(0, 32, 48, 64, 80, 8, 10)
+    # strided batches: stride below the system size, a method outside the
+    # register family, flags; an empty batch is a no-op
+    with pytest.raises(trid.TridError) as e:
+        h.solve_strided("crpcr", 16, 32, 48, 64, 80, 64, 4, 63)
+    assert e.value.status == 1
+    with pytest.raises(trid.TridError) as e:
+        h.solve_strided("crnbc", 16, 32, 48, 64, 80, 64, 4, 64)
+    assert e.value.status == 4
+    with pytest.raises(trid.TridError):
+        h.solve_strided("crpcr", 16, 32, 48, 64, 80, 64, 4, 80, trid.make_opts(flags=1))
+    h.solve_strided("crpcr", 16, 32, 48, 64, 80, 64, 0, 80)
     h.solve_pivoting(16, 32, 48, 64, 80, 8, 0)
     h.solve_pivoting(16, 32, 48, 64, 64, 1, 0, dtype="f64")
     h.close()

@@ -29,6 +29,7 @@ EXPORTS = (
     "trid_generate_f32", "trid_generate_f64", "trid_plan_query",
     "trid_solve_refined_f64", "trid_residual_f64",
     "trid_solve_interleaved_f32", "trid_solve_interleaved_f64",
+    "trid_solve_strided_f32", "trid_solve_strided_f64",
     "trid_solve_pivoting_f32", "trid_solve_pivoting_f64",
     "trid_solve_pivoting_interleaved_f32", "trid_solve_pivoting_interleaved_f64",
     "trid_gtsv_f32", "trid_gtsv_f64",
@@ -172,6 +173,9 @@ def lib() -> ctypes.CDLL:
         g.restype = i32
         v = getattr(L, f"trid_solve_interleaved_{t}")
         v.argtypes = [P, P, P, P, P, P, i64, i64, i64, P]
+        v.restype = i32
+        v = getattr(L, f"trid_solve_strided_{t}")
+        v.argtypes = [P, i32, P, P, P, P, P, i64, i64, i64, ctypes.POINTER(Opts), P]
         v.restype = i32
     L.trid_solve_refined_f64.argtypes = [P, i32, P, P, P, P, P, i64, i64, i32,
                                          ctypes.POINTER(Opts), P]

@@ -41,21 +41,24 @@ __global__ __launch_bounds__(256) void reg_crpcr_kernel(const T* __restrict__ A,
                                                         const T* __restrict__ D,
                                                         T* __restrict__ X, int64_t n,
                                                         int64_t N, int L, int logL, int rest,
-                                                        int vec, ConstCoef<T> cc)
+                                                        int vec, ConstCoef<T> cc, int64_t bs)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int ll = lane & (L - 1);
     const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t sys = gtid >> logL;
     const bool live = sys < N;
-    const int64_t base = sys * n;
+    const int64_t base = sys * bs;  // bs = n for a contiguous batch
     const int64_t j0 = (int64_t)ll * M;
 
     constexpr int E = 16 / (int)sizeof(T);
     extern __shared__ __align__(16) char trid_smem[];
     char* lds = trid_smem + (threadIdx.x / kWave) * Stage<T, M>::kWaveBytes;
-    const int64_t wave_base = (gtid / kWave) * (int64_t)(kWave * M);  // dense: n == L*M
-    const int64_t total = n * N;
+    // dense: a wave's systems are one block of 64*M elements (contiguous
+    // batches), or the wave holds one system (L = 64, any batch stride)
+    const int64_t wave_base =
+        bs == n ? (gtid / kWave) * (int64_t)(kWave * M) : ((gtid / kWave) * kWave >> logL) * bs;
+    const int64_t total = (N - 1) * bs + n;
 
     Eq<T> e[M];
     if constexpr (M % E == 0 && IO >= 2) {
@@ -119,7 +122,7 @@ static hipError_t launch_reg_io(const RegArgs<T>& g, int R, dim3 grid, dim3 bloc
         if constexpr (RR <= M) {                                                             \
             hipLaunchKernelGGL((reg_crpcr_kernel<T, M, RR, IO>), grid, block, lds, g.stream, \
                                g.a, g.b, g.c, g.d, g.x, g.n, g.N, g.L, logL, g.rest, g.vec,    \
-                               ConstCoef<T>{g.cc != 0, g.a0, g.b0, g.c0});                   \
+                               ConstCoef<T>{g.cc != 0, g.a0, g.b0, g.c0}, g.bs ? g.bs : g.n);   \
             return hipGetLastError();                                                        \
         }                                                                                    \
         break;

@@ -32,14 +32,15 @@ template <typename T, int M, int IO>
 __device__ __forceinline__ void reg_mw_body(const T* __restrict__ A, const T* __restrict__ B,
                                             const T* __restrict__ C, const T* __restrict__ D,
                                             T* __restrict__ X, int64_t n, int rest, int vec,
-                                            int64_t N, ConstCoef<T> cc)
+                                            int64_t N, ConstCoef<T> cc, int64_t bs)
 {
     constexpr int E = 16 / (int)sizeof(T);
     const int W = blockDim.x / kWave;
     const int w = threadIdx.x / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t sys = blockIdx.x;
-    const int64_t base = sys * n;
+    const int64_t base = sys * bs;  // bs = n for a contiguous batch
+    const int64_t total = (N - 1) * bs + n;
     const int64_t j0 = (int64_t)(w * kWave + lane) * M;
 
     // LDS: the staging rows (load and store) share space with the exchange
@@ -52,7 +53,7 @@ __device__ __forceinline__ void reg_mw_body(const T* __restrict__ A, const T* __
     bool loaded = false;
     if constexpr (M % E == 0 && IO >= 2) {
         if (vec) {
-            load_staged<T, M, IO == 3>(A, B, C, D, base + (int64_t)w * kWave * M, n * N, lane,
+            load_staged<T, M, IO == 3>(A, B, C, D, base + (int64_t)w * kWave * M, total, lane,
                                        lds_stage, e, cc);
             loaded = true;
         }
@@ -83,7 +84,7 @@ __device__ __forceinline__ void reg_mw_body(const T* __restrict__ A, const T* __
     __syncthreads();  // exchange area becomes staging rows again
     if constexpr (M % E == 0 && IO >= 2) {
         if (vec) {
-            store_staged<T, M, IO == 3>(X, base + (int64_t)w * kWave * M, n * N, lane, lds_stage,
+            store_staged<T, M, IO == 3>(X, base + (int64_t)w * kWave * M, total, lane, lds_stage,
                                         xv);
             return;
         }
@@ -108,9 +109,10 @@ __global__ __launch_bounds__(1024) void reg_mw_kernel(const T* __restrict__ A,
                                                       const T* __restrict__ C,
                                                       const T* __restrict__ D,
                                                       T* __restrict__ X, int64_t n, int rest,
-                                                      int vec, int64_t N, ConstCoef<T> cc)
+                                                      int vec, int64_t N, ConstCoef<T> cc,
+                                                      int64_t bs)
 {
-    reg_mw_body<T, M, IO>(A, B, C, D, X, n, rest, vec, N, cc);
+    reg_mw_body<T, M, IO>(A, B, C, D, X, n, rest, vec, N, cc, bs);
 }
 
 // ... or up to 8 waves with twice the registers: FP64 at 16 unknowns per lane.
@@ -120,9 +122,10 @@ __global__ __launch_bounds__(512) void reg_mw512_kernel(const T* __restrict__ A,
                                                         const T* __restrict__ C,
                                                         const T* __restrict__ D,
                                                         T* __restrict__ X, int64_t n, int rest,
-                                                        int vec, int64_t N, ConstCoef<T> cc)
+                                                        int vec, int64_t N, ConstCoef<T> cc,
+                                                        int64_t bs)
 {
-    reg_mw_body<T, M, IO>(A, B, C, D, X, n, rest, vec, N, cc);
+    reg_mw_body<T, M, IO>(A, B, C, D, X, n, rest, vec, N, cc, bs);
 }
 
 template <typename T, int M>
@@ -153,10 +156,12 @@ static hipError_t launch_mw_io(const RegArgs<T>& g)
     const ConstCoef<T> cc{g.cc != 0, g.a0, g.b0, g.c0};
     if constexpr (wide)
         hipLaunchKernelGGL((reg_mw512_kernel<T, M, IO>), dim3((unsigned)g.N), dim3(kWave * W),
-                           lds, g.stream, g.a, g.b, g.c, g.d, g.x, g.n, g.rest, g.vec, g.N, cc);
+                           lds, g.stream, g.a, g.b, g.c, g.d, g.x, g.n, g.rest, g.vec, g.N, cc,
+                           g.bs ? g.bs : g.n);
     else
         hipLaunchKernelGGL((reg_mw_kernel<T, M, IO>), dim3((unsigned)g.N), dim3(kWave * W), lds,
-                           g.stream, g.a, g.b, g.c, g.d, g.x, g.n, g.rest, g.vec, g.N, cc);
+                           g.stream, g.a, g.b, g.c, g.d, g.x, g.n, g.rest, g.vec, g.N, cc,
+                           g.bs ? g.bs : g.n);
     return hipGetLastError();
 }
 

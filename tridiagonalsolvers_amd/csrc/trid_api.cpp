@@ -165,6 +165,53 @@ int solve_impl(trid_handle h, int method, const T* a, const T* b, const T* c, co
     }
 }
 
+// Strided batch: system s at s * bs (bs >= n), the register family only.
+template <typename T>
+int strided_impl(trid_context* h, int method, const T* a, const T* b, const T* c, const T* d,
+                 T* x, int64_t n, int64_t N, int64_t bs, const trid_opts* opts,
+                 hipStream_t stream)
+{
+    trid_opts o{};
+    if (opts) o = *opts;
+    if (!h || !a || !b || !c || !d || !x) return TRID_EINVAL;
+    if (n < 2 || N < 0 || bs < n) return TRID_EINVAL;
+    if (o.flags != 0) return TRID_EINVAL;
+    if (method != TRID_CR && method != TRID_PCR && method != TRID_CRPCR && method != TRID_AUTO)
+        return TRID_ENOTSUP;
+    if (N == 0) return TRID_OK;
+    if (N > 1 && bs > (INT64_MAX - n) / (N - 1)) return TRID_EINVAL;
+    if (bs == n) return solve_impl<T>(h, method, a, b, c, d, x, n, N, &o, stream);
+    trid::Plan p;
+    int st = trid::make_plan(method, n, N, (int)sizeof(T), o, &p);
+    if (st != TRID_OK) return st;
+    if (p.family != trid::Family::Reg) return TRID_ENOTSUP;
+    st = check_device();
+    if (st != TRID_OK) return st;
+    trid::RegArgs<T> g{};
+    g.a = a;
+    g.b = b;
+    g.c = c;
+    g.d = d;
+    g.x = x;
+    g.n = n;
+    g.N = N;
+    g.M = p.M;
+    g.L = p.L;
+    g.rest = p.rest;
+    g.bs = bs;
+    // 16-byte vectors need every system to start 16-byte aligned; the
+    // wave-staged form needs one system per wave (L >= 64), otherwise the
+    // lanes move their 16-byte vectors directly
+    g.vec = (p.dense && (bs * (int64_t)sizeof(T)) % 16 == 0 && aligned16(a) && aligned16(b) &&
+             aligned16(c) && aligned16(d) && aligned16(x))
+                ? 1
+                : 0;
+    g.io = p.io;
+    if (g.io >= 2 && g.L < 64) g.io = 1;
+    g.stream = stream;
+    return from_hip(g.L <= 64 ? trid::launch_reg_crpcr<T>(g) : trid::launch_reg_mw<T>(g));
+}
+
 // Host-pointer drop-in: copy in, one solve on the null stream, copy out.
 template <typename T>
 int host_solve(int method, const T* a, const T* b, const T* c, const T* d, T* x, int n, int N,
@@ -326,6 +373,25 @@ int trid_solve_interleaved_f64(trid_handle handle, const double* a, const double
 {
     return set_status(interleaved_impl<double>(handle, a, b, c, d, x, systemSize, numSystems,
                                                batchStride, static_cast<hipStream_t>(stream)));
+}
+
+int trid_solve_strided_f32(trid_handle handle, int method, const float* a, const float* b,
+                           const float* c, const float* d, float* x, int64_t systemSize,
+                           int64_t numSystems, int64_t batchStride, const trid_opts* opts,
+                           void* stream)
+{
+    return set_status(strided_impl<float>(handle, method, a, b, c, d, x, systemSize, numSystems,
+                                          batchStride, opts, static_cast<hipStream_t>(stream)));
+}
+
+int trid_solve_strided_f64(trid_handle handle, int method, const double* a, const double* b,
+                           const double* c, const double* d, double* x, int64_t systemSize,
+                           int64_t numSystems, int64_t batchStride, const trid_opts* opts,
+                           void* stream)
+{
+    return set_status(strided_impl<double>(handle, method, a, b, c, d, x, systemSize,
+                                           numSystems, batchStride, opts,
+                                           static_cast<hipStream_t>(stream)));
 }
 
 int trid_solve_pivoting_f32(trid_handle handle, const float* a, const float* b, const float* c,

@@ -29,6 +29,8 @@ struct RegArgs {
     hipStream_t stream;
     int cc = 0;      // constant coefficients a0, b0, c0 on every row (d only)
     T a0 = 0, b0 = 1, c0 = 0;
+    int64_t bs = 0;  // elements from one system's first row to the next's (0: n);
+                     // with bs != n the wave-staged I/O needs L >= 64
 };
 
 template <typename T>

@@ -169,6 +169,20 @@ class Handle:
                ctypes.c_void_p(_stream_ptr(stream)))
         check(st, "trid_solve")
 
+    def solve_strided(self, method, a, b, c, d, x, systemSize: int, numSystems: int,
+                      batchStride: int, opts: Opts | None = None, stream=None,
+                      dtype: str | None = None) -> None:
+        """Strided batch: system s at s * batchStride (>= systemSize), the
+        layout of rocSPARSE's gtsv_no_pivot_strided_batch; register methods."""
+        if dtype is None:
+            dtype = "f64" if str(getattr(a, "dtype", "")) == "torch.float64" else "f32"
+        f = getattr(lib(), f"trid_solve_strided_{dtype}")
+        o = opts if opts is not None else Opts()
+        st = f(self._h, method_id(method), *(ctypes.c_void_p(_ptr(v)) for v in (a, b, c, d, x)),
+               int(systemSize), int(numSystems), int(batchStride), ctypes.byref(o),
+               ctypes.c_void_p(_stream_ptr(stream)))
+        check(st, "trid_solve_strided")
+
     def solve_interleaved(self, a, b, c, d, x, systemSize: int, numSystems: int,
                           batchStride: int | None = None, stream=None,
                           dtype: str | None = None) -> None:
