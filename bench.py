@@ -47,7 +47,9 @@ def parse():
     p.add_argument("--method", default="crpcr")
     p.add_argument("--n", type=int, default=1024)
     p.add_argument("--systems", type=int, default=65536)
-    p.add_argument("--rest", type=int, default=64)
+    p.add_argument("--rest", type=int, default=0,
+                   help="crpcr: CR->PCR switch size (0 = the plan's choice, one PCR "
+                        "equation per lane: 64 for the headline); pcrgauss: G")
     p.add_argument("--dtype", default="f32", choices=["f32", "f64"])
     p.add_argument("--batch-stride", type=int, default=0,
                    help="uniform batches: system s at s * stride (>= n) through "
