@@ -55,15 +55,17 @@ __global__ __launch_bounds__(256) void reg_crpcr_kernel(const T* __restrict__ A,
     extern __shared__ __align__(16) char trid_smem[];
     char* lds = trid_smem + (threadIdx.x / kWave) * Stage<T, M>::kWaveBytes;
     // dense: a wave's systems are one block of 64*M elements (contiguous
-    // batches), or the wave holds one system (L = 64, any batch stride)
+    // batches), or 64/L systems of n = L*M elements, bs apart (strided)
     const int64_t wave_base =
         bs == n ? (gtid / kWave) * (int64_t)(kWave * M) : ((gtid / kWave) * kWave >> logL) * bs;
     const int64_t total = (N - 1) * bs + n;
+    const int seg_log2 = (bs == n || L == kWave) ? -1 : logL + Log2<M>::value;
 
     Eq<T> e[M];
     if constexpr (M % E == 0 && IO >= 2) {
         if (vec) {
-            load_staged<T, M, IO == 3>(A, B, C, D, wave_base, total, lane, lds, e, cc);
+            load_staged<T, M, IO == 3>(A, B, C, D, wave_base, total, lane, lds, e, cc, seg_log2,
+                                       bs);
             if (!live) set_identity<T, M>(e);
         }
     } else if constexpr (M % E == 0) {
@@ -94,7 +96,7 @@ __global__ __launch_bounds__(256) void reg_crpcr_kernel(const T* __restrict__ A,
 
     if constexpr (M % E == 0 && IO >= 2) {
         if (vec) {
-            store_staged<T, M, IO == 3>(X, wave_base, total, lane, lds, xv);
+            store_staged<T, M, IO == 3>(X, wave_base, total, lane, lds, xv, seg_log2, bs);
             return;
         }
     } else if constexpr (M % E == 0) {

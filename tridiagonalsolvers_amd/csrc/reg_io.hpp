@@ -97,14 +97,26 @@ struct Stage {
     }
 };
 
+// Element offset of position o of a wave's block of 64*M unknowns. The block
+// is contiguous (seg_log2 < 0), or (strided batches) made of systems of
+// 2^seg_log2 unknowns that start `seg_stride` elements apart.
+static __device__ __forceinline__ int64_t block_elem(int64_t wave_base, int64_t o, int seg_log2,
+                                                     int64_t seg_stride)
+{
+    if (seg_log2 < 0) return wave_base + o;
+    return wave_base + (o >> seg_log2) * seg_stride + (o & ((int64_t(1) << seg_log2) - 1));
+}
+
 // Wave-coalesced loads: instruction q of the wave reads the q-th contiguous
-// 1 KiB of the wave's block; the chunks are transposed through LDS so that
-// lane l ends up with unknowns [l*M, l*M+M) of the block.
+// 1 KiB of the wave's block (for a strided batch: 16-byte pieces of its
+// systems); the chunks are transposed through LDS so that lane l ends up with
+// unknowns [l*M, l*M+M) of the block.
 template <typename T, int M, bool NT>
 static __device__ __forceinline__ void load_staged(const T* __restrict__ A, const T* __restrict__ B,
                                             const T* __restrict__ C, const T* __restrict__ D,
                                             int64_t wave_base, int64_t total, int lane,
-                                            char* lds, Eq<T> (&e)[M], ConstCoef<T> cc)
+                                            char* lds, Eq<T> (&e)[M], ConstCoef<T> cc,
+                                            int seg_log2 = -1, int64_t seg_stride = 0)
 {
     using S = Stage<T, M>;
     constexpr int E = S::E, Q = S::Q;
@@ -113,7 +125,8 @@ static __device__ __forceinline__ void load_staged(const T* __restrict__ A, cons
     const int first = cc.on ? 3 : 0;  // constant coefficients: d only
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        const int64_t el = wave_base + (int64_t)(kWave * q + lane) * E;
+        const int64_t el =
+            block_elem(wave_base, (int64_t)(kWave * q + lane) * E, seg_log2, seg_stride);
         const bool ok = el < total;
 #pragma unroll
         for (int arr = 0; arr < 4; ++arr)
@@ -148,7 +161,8 @@ static __device__ __forceinline__ void load_staged(const T* __restrict__ A, cons
 
 template <typename T, int M, bool NT>
 static __device__ __forceinline__ void store_staged(T* __restrict__ X, int64_t wave_base, int64_t total,
-                                             int lane, char* lds, const T (&xv)[M])
+                                             int lane, char* lds, const T (&xv)[M],
+                                             int seg_log2 = -1, int64_t seg_stride = 0)
 {
     using S = Stage<T, M>;
     constexpr int E = S::E, Q = S::Q;
@@ -162,7 +176,8 @@ static __device__ __forceinline__ void store_staged(T* __restrict__ X, int64_t w
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const u32x4 t = *reinterpret_cast<const u32x4*>(lds + S::chunk_addr(kWave * q + lane));
-        const int64_t el = wave_base + (int64_t)(kWave * q + lane) * E;
+        const int64_t el =
+            block_elem(wave_base, (int64_t)(kWave * q + lane) * E, seg_log2, seg_stride);
         if (el < total) gstore16<NT>(X + el, t);
     }
 }

@@ -199,15 +199,13 @@ int strided_impl(trid_context* h, int method, const T* a, const T* b, const T* c
     g.L = p.L;
     g.rest = p.rest;
     g.bs = bs;
-    // 16-byte vectors need every system to start 16-byte aligned; the
-    // wave-staged form needs one system per wave (L >= 64), otherwise the
-    // lanes move their 16-byte vectors directly
+    // 16-byte vectors need every system to start 16-byte aligned (the
+    // wave-staged form then reads each system's 16-byte pieces in place)
     g.vec = (p.dense && (bs * (int64_t)sizeof(T)) % 16 == 0 && aligned16(a) && aligned16(b) &&
              aligned16(c) && aligned16(d) && aligned16(x))
                 ? 1
                 : 0;
     g.io = p.io;
-    if (g.io >= 2 && g.L < 64) g.io = 1;
     g.stream = stream;
     return from_hip(g.L <= 64 ? trid::launch_reg_crpcr<T>(g) : trid::launch_reg_mw<T>(g));
 }
