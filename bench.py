@@ -1,4 +1,4 @@
-"""Headline benchmark: batched CR-PCR, FP32, 65,536 systems x 1,024 unknowns.
+"""Benchmark: batched CR-PCR, FP32, 65,536 systems x 1,024 unknowns (default).
 
 One step = one solve of the whole batch (inputs already resident in HBM,
 written by the device generator, seed 1, diagonally dominant kind). Prints
@@ -31,7 +31,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--workload", default="uniform",
                    choices=["uniform", "ragged", "serve", "gauss-sweep", "pivoting", "adi"],
-                   help="uniform: the headline batch; ragged: 100,000 systems of "
+                   help="uniform: 65,536 x 1,024 by default; ragged: 100,000 systems of "
                         "log-uniform sizes in [16, 8192], FP32 solve + FP64 refinement; "
                         "serve: request server with Poisson arrivals and resident ADI grids; "
                         "pivoting: the partial-pivoting (LAPACK ?gtsv) solve of a batch "
@@ -49,7 +49,7 @@ def parse():
     p.add_argument("--systems", type=int, default=65536)
     p.add_argument("--rest", type=int, default=0,
                    help="crpcr: CR->PCR switch size (0 = the plan's choice, one PCR "
-                        "equation per lane: 64 for the headline); pcrgauss: G")
+                        "equation per lane: 64 at 1,024 unknowns); pcrgauss: G")
     p.add_argument("--dtype", default="f32", choices=["f32", "f64"])
     p.add_argument("--batch-stride", type=int, default=0,
                    help="uniform batches: system s at s * stride (>= n) through "

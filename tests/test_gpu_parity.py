@@ -206,8 +206,8 @@ def test_device_api_matches_host_dropin(oracle, trid):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_headline_full_size(oracle, trid, dtype):
-    # 65,536 systems x 1,024 unknowns (the headline batch), whole batch checked
+def test_65536x1024_full_size(oracle, trid, dtype):
+    # 65,536 systems x 1,024 unknowns, whole batch checked
     import torch
     n, N = 1024, 65536
     tdt = torch.float32 if dtype == np.float32 else torch.float64
@@ -219,7 +219,7 @@ def test_headline_full_size(oracle, trid, dtype):
     torch.cuda.synchronize()
     a, b, c, d = oracle.generate("dominant", 1, n, N, dtype)
     worst = check(oracle, x.cpu().numpy(), a, b, c, d, n, dtype)
-    print(f"headline {np.dtype(dtype).name}: max rel err {worst:.3e}", flush=True)
+    print(f"65536x1024 {np.dtype(dtype).name}: max rel err {worst:.3e}", flush=True)
     h.close()
 
 

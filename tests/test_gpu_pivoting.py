@@ -128,7 +128,7 @@ def test_gtsv_host_dropin(oracle, trid, dtype):
         assert np.array_equal(u, v)
 
 
-def test_pivoting_headline_shape_bitwise(oracle, trid):
+def test_pivoting_65536x1024_bitwise(oracle, trid):
     """65,536 systems of 1,024 unknowns, FP32, the "paired" kind (pivoting
     needed): every element equals the oracle's sgtsv restatement."""
     import torch

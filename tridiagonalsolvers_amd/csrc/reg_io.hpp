@@ -82,7 +82,7 @@ static __device__ __forceinline__ void store_direct(T* __restrict__ X, int64_t o
 // LDS row of one lane: M elements plus 16 bytes of padding. Unpadded rows
 // (M*sizeof(T) in {32, 64, 128} bytes) would put the row-wise 16-byte reads
 // of neighbouring lanes on the same banks; the extra 16 bytes shift each
-// row by four banks. (Measured on the headline kernel: bank-conflict cycles
+// row by four banks. (Measured on 65,536 x 1,024 FP32: bank-conflict cycles
 // are a quarter of its LDS-busy cycles, and its time is set by HBM.)
 template <typename T, int M>
 struct Stage {
