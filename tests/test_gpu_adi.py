@@ -44,3 +44,43 @@ def test_adi_random_grid_matches_oracle(oracle, trid, n):
     err = np.max(np.abs(u - ref)) / np.max(np.abs(ref))
     print(f"n={n}: rel err {err:.2e}")
     assert err <= 1e-12
+
+
+@pytest.mark.parametrize("n", [37, 256, 2048])
+def test_adi_several_steps_in_one_call(oracle, trid, n):
+    """One call of several steps keeps u' transposed between steps and
+    transposes back only at the end; it must equal the steps one by one."""
+    import torch
+    from tridiagonalsolvers_amd.adi import AdiGrid
+    rng = np.random.default_rng(7 + n)
+    u0 = rng.uniform(-1, 1, (n, n))
+    u = torch.from_numpy(u0.copy()).cuda()
+    g = AdiGrid(n, u)
+    g.step(2.5, 4)
+    torch.cuda.synchronize()
+    got = u.cpu().numpy().reshape(n, n)
+    g.close()
+    ref = u0
+    for _ in range(4):
+        ref = oracle.adi_step_np(ref, 2.5)
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    print(f"n={n}: 4 steps, rel err {err:.2e}", flush=True)
+    assert err <= 1e-12
+    # a sine mode decays by its amplification factor per step
+    i = np.arange(1, n + 1)
+    s0 = np.outer(np.sin(3 * np.pi * i / (n + 1)), np.sin(2 * np.pi * i / (n + 1)))
+    u1 = _run_steps(n, s0, 0.7, 3)
+    f = oracle.adi_mode_factor(n, 3, 2, 0.7)
+    assert np.max(np.abs(u1 - f ** 3 * s0)) <= 1e-12
+
+
+def _run_steps(n, u0, r, steps):
+    import torch
+    from tridiagonalsolvers_amd.adi import AdiGrid
+    u = torch.from_numpy(np.ascontiguousarray(u0, dtype=np.float64)).cuda()
+    g = AdiGrid(n, u)
+    g.step(r, steps)
+    torch.cuda.synchronize()
+    out = u.cpu().numpy().reshape(n, n)
+    g.close()
+    return out

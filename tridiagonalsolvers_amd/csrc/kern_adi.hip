@@ -17,6 +17,11 @@
 //   and a tiled transpose writes u'. A direct column solve (one lane per
 //   column, n/64 waves) leaves the GPU mostly idle; the transposes cost two
 //   streaming passes and keep every pass at full width.
+// * several steps in one call: u' of a step stays transposed, and the next
+//   row rhs is formed from it by the same transposing stencil kernel (the
+//   stencil along the transposed grid's rows is the stencil along u's
+//   columns); only the last step transposes u' back into u: 4 passes a step
+//   instead of 5.
 #include "trid_device.hpp"
 #include "trid_launch.hpp"
 
@@ -93,7 +98,7 @@ __global__ __launch_bounds__(256) void adi_transpose_kernel(const double* __rest
     }
 }
 
-hipError_t launch_adi_step(const AdiArgs& g)
+hipError_t launch_adi_steps(const AdiArgs& g, int steps)
 {
     const int64_t n = g.n;
     const int64_t total = n * n;
@@ -121,19 +126,31 @@ hipError_t launch_adi_step(const AdiArgs& g)
         ra.x = x;
         return ra.L <= kWave ? launch_reg_crpcr<double>(ra) : launch_reg_mw<double>(ra);
     };
-    // row sweep: rhs into T, solve T -> U (u*)
-    hipLaunchKernelGGL(adi_rhs_rows_kernel, dim3((unsigned)blocks), dim3(256), 0, g.stream, g.u,
-                       g.t, n, g.r);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = solve(g.t, g.u)) != hipSuccess) return e;
-    // column sweep: rhs transposed into T, solve T -> T2, transpose T2 -> U
-    hipLaunchKernelGGL(adi_rhs_cols_t_kernel, dim3((unsigned)tiles, (unsigned)tiles), dim3(256),
-                       0, g.stream, g.u, g.t, n, g.r);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = solve(g.t, g.t2)) != hipSuccess) return e;
-    hipLaunchKernelGGL(adi_transpose_kernel, dim3((unsigned)tiles, (unsigned)tiles), dim3(256), 0,
-                       g.stream, g.t2, g.u, n);
-    return hipGetLastError();
+    const dim3 tgrid((unsigned)tiles, (unsigned)tiles);
+    for (int k = 0; k < steps; ++k) {
+        // row sweep: rhs into T, solve T -> U (u*). After a step whose u' was
+        // left transposed in T2, the same tiled kernel forms the row rhs from
+        // T2 (transposing it back on the way), saving a transpose and a pass.
+        if (k == 0)
+            hipLaunchKernelGGL(adi_rhs_rows_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                               g.stream, g.u, g.t, n, g.r);
+        else
+            hipLaunchKernelGGL(adi_rhs_cols_t_kernel, tgrid, dim3(256), 0, g.stream, g.t2, g.t,
+                               n, g.r);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = solve(g.t, g.u)) != hipSuccess) return e;
+        // column sweep: rhs transposed into T, solve T -> T2 (u' transposed)
+        hipLaunchKernelGGL(adi_rhs_cols_t_kernel, tgrid, dim3(256), 0, g.stream, g.u, g.t, n,
+                           g.r);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = solve(g.t, g.t2)) != hipSuccess) return e;
+    }
+    // the grid itself after the last step: transpose T2 -> U
+    if (steps > 0) {
+        hipLaunchKernelGGL(adi_transpose_kernel, tgrid, dim3(256), 0, g.stream, g.t2, g.u, n);
+        return hipGetLastError();
+    }
+    return hipSuccess;
 }
 
 }  // namespace trid

@@ -82,10 +82,7 @@ int trid_adi_step(trid_adi a, double r, int steps, void* stream)
                                reinterpret_cast<uintptr_t>(a->t) |
                                reinterpret_cast<uintptr_t>(a->t2)) & 15u) == 0;
     g.stream = static_cast<hipStream_t>(stream);
-    for (int s = 0; s < steps; ++s) {
-        if (trid::launch_adi_step(g) != hipSuccess) return TRID_EHIP;
-    }
-    return TRID_OK;
+    return trid::launch_adi_steps(g, steps) == hipSuccess ? TRID_OK : TRID_EHIP;
 }
 
 }  // extern "C"

@@ -204,7 +204,7 @@ template <typename T>
 hipError_t launch_generate(int kind, uint64_t seed, T* a, T* b, T* c, T* d, int64_t n,
                            int64_t N, hipStream_t stream);
 
-// One Peaceman-Rachford ADI step on an n x n FP64 grid (kern_adi.hip).
+// Peaceman-Rachford ADI steps on an n x n FP64 grid (kern_adi.hip).
 struct AdiArgs {
     double* u;      // grid, row-major, n*n (input and output)
     double* t;      // scratch grid, n*n
@@ -215,7 +215,8 @@ struct AdiArgs {
     hipStream_t stream;
 };
 
-hipError_t launch_adi_step(const AdiArgs& g);
+// `steps` Peaceman-Rachford steps (the grid in U is current when they end).
+hipError_t launch_adi_steps(const AdiArgs& g, int steps);
 
 // Relative residual per system (kern_util.hip).
 hipError_t launch_residual(const double* a, const double* b, const double* c, const double* d,

@@ -308,8 +308,7 @@ int trid_server_s::launch(Slot& s)
         g.rest = a.plan.rest;
         g.vec = a.plan.dense;
         g.stream = s.stream;
-        for (int k = 0; k < q->r.adi_steps; ++k)
-            if (trid::launch_adi_step(g) != hipSuccess) return TRID_EHIP;
+        if (trid::launch_adi_steps(g, q->r.adi_steps) != hipSuccess) return TRID_EHIP;
         return hipEventRecord(s.done, s.stream) == hipSuccess ? TRID_OK : TRID_EHIP;
     }
     const int64_t total = s.total, N = s.N;
