@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void adi_rhs_rows_kernel(const double* __restr
 // Dt[j][i] = u[i][j] + r * (u[i][j-1] - 2 u[i][j] + u[i][j+1]) for one 64 x 64
 // tile: rows i0.., columns j0-1 .. j0+64 are read (coalesced along j) into
 // LDS, the tile is written transposed (coalesced along i). Row pitch 67
-// doubles: the 32 lanes of one LDS pass read tile[tx][c] on disjoint banks.
+// doubles spreads the column reads tile[tx][c] over the banks.
 __global__ __launch_bounds__(256) void adi_rhs_cols_t_kernel(const double* __restrict__ U,
                                                              double* __restrict__ Dt, int64_t n,
                                                              double r)
@@ -60,11 +60,16 @@ __global__ __launch_bounds__(256) void adi_rhs_cols_t_kernel(const double* __res
     const int ty = threadIdx.x / kTile;  // 0..3
     const int64_t i0 = (int64_t)blockIdx.y * kTile;
     const int64_t j0 = (int64_t)blockIdx.x * kTile;
+    // the 64 columns of the tile as one aligned 512-byte row per wave; the
+    // two halo columns j0-1 and j0+64 by two lanes
     for (int row = ty; row < kTile; row += 4) {
         const int64_t i = i0 + row;
-        for (int c = tx; c < kTile + 2; c += kTile) {
-            const int64_t j = j0 - 1 + c;
-            tile[row][c] = (i < n && j >= 0 && j < n) ? U[i * n + j] : 0.0;
+        const int64_t j = j0 + tx;
+        tile[row][tx + 1] = (i < n && j < n) ? U[i * n + j] : 0.0;
+        if (tx < 2) {
+            const int64_t jh = tx == 0 ? j0 - 1 : j0 + kTile;
+            tile[row][tx == 0 ? 0 : kTile + 1] =
+                (i < n && jh >= 0 && jh < n) ? U[i * n + jh] : 0.0;
         }
     }
     __syncthreads();
