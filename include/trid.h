@@ -471,7 +471,9 @@ int trid_server_get_stats(trid_server server, trid_server_stats* stats);
  * systemSize (one block of n/2 or n threads per system, cr_app.cu:37-38).
  * Validates a request exactly as trid_solve_* would and reports the kernel
  * family and launch shape it would use: family 0 = register-resident
- * CR/CR-PCR/PCR, 1 = LDS-resident CR/CR-PCR with pad map, 2 = PCR + Thomas.
+ * CR/CR-PCR/PCR, 1 = LDS-resident CR/CR-PCR with pad map, 2 = PCR + Thomas,
+ * 3 = CR/CR-PCR by separators (systems longer than 16,384 unknowns in FP32
+ * or 8,192 in FP64: chunks of 1,024 rows with per_lane 16, lanes 64).
  */
 typedef struct trid_plan_info {
     int32_t family;

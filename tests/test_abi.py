@@ -135,6 +135,27 @@ def test_plan_query_register_family(trid):
         trid.plan_query("crpcr", 64, 1, 3)
 
 
+def test_plan_query_separator_family(trid):
+    # cr / crpcr past the register sizes: chunks of 1,024 rows (16 x 64 lanes)
+    for n, es in ((16385, 4), (1 << 20, 4), (1 << 30, 4), (8193, 8), (1 << 20, 8)):
+        for m in ("cr", "crpcr", "auto"):
+            q = trid.plan_query(m, n, 3, es)
+            assert q["family"] == 3 and q["per_lane"] == 16 and q["lanes"] == 64, (m, n, es)
+            assert q["rest"] == (2 if m == "cr" else 64)
+    assert trid.plan_query("crpcr", 16384, 3, 4)["family"] == 0
+    assert trid.plan_query("crpcr", 8192, 3, 8)["family"] == 0
+    q = trid.plan_query("crpcr", 1 << 20, 1, 4, trid.make_opts(rest=512))
+    assert q["family"] == 3 and q["rest"] == 512
+    for bad, status in ((trid.make_opts(rest=1024), 4), (trid.make_opts(rest=96), 1),
+                        (trid.make_opts(lanes=64, per_lane=16), 1)):
+        with pytest.raises(trid.TridError) as e:
+            trid.plan_query("crpcr", 1 << 20, 1, 4, bad)
+        assert e.value.status == status
+    with pytest.raises(trid.TridError) as e:     # PCR has no separator form
+        trid.plan_query("pcr", 1 << 20, 1, 4)
+    assert e.value.status == 4
+
+
 def test_plan_query_nbc_and_gauss(trid):
     q = trid.plan_query("crnbc", 1024, 1, 4)
     assert q["family"] == 1 and q["lds_pad"] > 0

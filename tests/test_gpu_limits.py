@@ -37,10 +37,10 @@ def test_sizes_beyond_the_kernels_are_refused(trid):
     h = trid.Handle()
     x64 = torch.zeros(32768, dtype=torch.float64, device="cuda")
     x32 = torch.zeros(65536, dtype=torch.float32, device="cuda")
+    # PCR beyond the register path (cr and crpcr go on by separators)
     cases = [
-        ("crpcr", x64, 16384),   # FP64 register path ends at 8192
-        ("crpcr", x32, 32768),   # FP32 register path ends at 16384
-        ("cr", x32, 65536),
+        ("pcr", x64, 16384),     # FP64 register path ends at 8192
+        ("pcr", x32, 32768),     # FP32 register path ends at 16384
     ]
     for m, v, n in cases:
         with pytest.raises(trid.TridError) as e:

@@ -160,6 +160,24 @@ int solve_impl(trid_handle h, int method, const T* a, const T* b, const T* c, co
             g.stream = stream;
             return from_hip(trid::launch_pcr_thomas<T>(g));
         }
+        case trid::Family::Part: {
+            const int m = method == TRID_CR ? TRID_CR : TRID_CRPCR;
+            st = ensure_ws(h, trid::part_ws_bytes<T>(m, n, N));
+            if (st != TRID_OK) return st;
+            trid::PartArgs<T> g{};
+            g.a = a;
+            g.b = b;
+            g.c = c;
+            g.d = d;
+            g.x = x;
+            g.n = n;
+            g.N = N;
+            g.method = m;
+            g.rest = p.rest;
+            g.ws = h->ws;
+            g.stream = stream;
+            return from_hip(trid::launch_part<T>(g));
+        }
         default:
             return TRID_ENOTSUP;
     }

@@ -123,6 +123,30 @@ size_t pivot_ws_bytes(int64_t n, int64_t N, bool interleaved);
 template <typename T>
 hipError_t launch_pivot(const PivotArgs<T>& g);
 
+// CR / CR-PCR by separators for systems beyond the register kernels
+// (kern_part.hip): chunks of 1,024 rows, a separator system of
+// part_chunks(n) unknowns per system, the chunks solved again.
+template <typename T>
+struct PartArgs {
+    const T* a;
+    const T* b;
+    const T* c;
+    const T* d;
+    T* x;
+    int64_t n;
+    int64_t N;
+    int method;   // TRID_CR or TRID_CRPCR (the separator system's method)
+    int rest;     // PCR system size of the chunk solves: 2, 4, .., 512
+    void* ws;     // part_ws_bytes<T>(method, n, N) bytes, 256-byte aligned
+    hipStream_t stream;
+};
+
+int64_t part_chunks(int64_t n);
+template <typename T>
+size_t part_ws_bytes(int method, int64_t n, int64_t N);
+template <typename T>
+hipError_t launch_part(const PartArgs<T>& g);
+
 // FP32 solve + FP64 residual refinement (kern_refine.hip).
 struct RefineArgs {
     const double* a;

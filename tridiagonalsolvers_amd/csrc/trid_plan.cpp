@@ -69,6 +69,20 @@ int plan_reg(int64_t n_pad, int64_t N, int elem_size, const trid_opts& o, Plan* 
 
 // LDS-resident family: four arrays of n_pad elements (x overwrites d) plus
 // one pad element per `pad` elements must fit in one workgroup's LDS.
+// Separator form (kern_part.hip) for systems longer than the register
+// kernels hold: chunks of 1,024 rows, one wave each (16 unknowns per lane),
+// solved with switch size `rest` (at most half a chunk).
+int plan_part(int rest, Plan* p)
+{
+    if (rest > 512) return TRID_ENOTSUP;
+    p->family = Family::Part;
+    p->M = 16;
+    p->L = 64;
+    p->rest = rest;
+    p->io = kDefaultIo;
+    return TRID_OK;
+}
+
 int plan_lds(int64_t n_pad, int elem_size, int pad, Plan* p)
 {
     const int64_t padded = pad > 0 ? n_pad + n_pad / pad : n_pad;
@@ -110,6 +124,12 @@ int make_plan(int method, int64_t n, int64_t N, int elem_size, const trid_opts& 
             }
             int st = plan_reg(n_pad, N, elem_size, o, p);
             if (st == TRID_EINVAL) return st;
+            if (st == TRID_ENOTSUP) {
+                // longer than a workgroup's registers hold: CR / CR-PCR by
+                // separators (PCR has no such form)
+                if (method == TRID_PCR) return TRID_ENOTSUP;
+                return plan_part(rest < 0 ? 64 : rest, p);
+            }
             if (st == TRID_OK) {
                 if (rest < 0) {
                     // default switch size: one PCR equation per lane of a wave

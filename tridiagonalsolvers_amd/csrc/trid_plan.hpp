@@ -16,6 +16,7 @@ enum class Family : int {
     Reg = 0,       // kern_reg.hip: register-resident CR / CR-PCR / PCR
     Lds = 1,       // kern_lds.hip: LDS-resident CR / CR-PCR with a pad map
     PcrThomas = 2, // kern_pcrgauss.hip: PCR levels then Thomas per subsystem
+    Part = 3,      // kern_part.hip: CR / CR-PCR by separators, chunks of 1,024
 };
 
 struct Plan {
