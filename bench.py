@@ -922,7 +922,9 @@ def bench_gauss_sweep(args):
     (global PCR pass of depth k, then two-ended Gaussian elimination on the
     2^k interleaved subsystems), k = 2..8, plus k = 9, 10 for reference
     (not part of the sweep). One JSON line; value = the best depth's
-    unknowns/s in 2..8."""
+    unknowns/s in 2..8. The same batch through crpcr (the separator form at
+    this size) is reported beside it."""
+    import numpy as np
     import torch
 
     import tridiagonalsolvers_amd as T
@@ -955,6 +957,32 @@ def bench_gauss_sweep(args):
                      "moved_bytes_model": (8 + 9) * 4 * n * N,
                      "in_sweep": k <= 8})
     best = min((r for r in rows if r["in_sweep"]), key=lambda r: r["ms"])
+    # the same batch through crpcr, which takes the separator form at this
+    # size (chunks of 1,024 rows; reads the inputs twice, writes x once)
+    for _ in range(args.warmup):
+        h.solve("crpcr", a, b, c, d, x, n, N, stream=stream)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        h.solve("crpcr", a, b, c, d, x, n, N, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    sep_ms = e0.elapsed_time(e1) / args.steps
+    sep = {"method": "crpcr", "family": T.plan_query("crpcr", n, N, 4)["family"],
+           "ms": sep_ms, "unknowns_per_s": n * N / (sep_ms * 1e-3),
+           "alg_frac_of_peak": 5 * 4 * n * N / (sep_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+           "moved_bytes_model": 9 * 4 * n * N + (21 * 16 + 5 * 4) * (n // 1024) * N,
+           "moved_frac_of_peak": (9 * 4 * n * N) / (sep_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+           "speedup_vs_best_depth": best["ms"] / sep_ms}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        ah, bh, ch, dh = O.generate("dominant", 1, n, N, np.float32)
+        xref = O.thomas(ah, bh, ch, dh, n=n, threads=O.max_threads())
+        rel, _, fails = O.compare(x.cpu().numpy(), xref, n=n, rms_eps=1e-3)
+        sep["max_rel_err_vs_fp64_thomas"] = float(np.max(rel))
+        sep["reference_rms_failures"] = int(fails)
     out = {"metric": METRIC, "value": best["unknowns_per_s"], "unit": "unknowns/s", "n_gpus": 1,
            "steps": max(1, args.steps // 5), "warmup": args.warmup, "ms_per_step": best["ms"],
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
@@ -962,6 +990,7 @@ def bench_gauss_sweep(args):
            "config": {"workload": "pcrgauss f32 64 systems x 1048576 unknowns, depth 2..8",
                       "best_depth": best["depth"]},
            "sweep": rows,
+           "separator_form": sep,
            "roofline": {"bound": "hbm", "achieved": 5 * 4 * n * N / (best["ms"] * 1e-3) / 1e9,
                         "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "frac": 5 * 4 * n * N / (best["ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS,

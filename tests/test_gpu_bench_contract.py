@@ -59,3 +59,15 @@ def test_adi_and_pivoting_lines():
         assert k in j, k
     cb = j["cpu_baseline"]
     assert cb["gpu_bitwise_equal_to_oracle"] and cb["gpu_info_equal_to_oracle"]
+
+
+def test_gauss_sweep_line_reports_the_separator_form():
+    j = _json(_run(["--workload", "gauss-sweep", "--steps", "5", "--warmup", "1",
+                    "--cpu-seconds", "0.2"]))
+    for k in KEYS:
+        assert k in j, k
+    assert [r["depth"] for r in j["sweep"] if r["in_sweep"]] == list(range(2, 9))
+    sf = j["separator_form"]
+    assert sf["family"] == 3 and sf["ms"] > 0
+    assert sf["max_rel_err_vs_fp64_thomas"] <= 1e-4 and sf["reference_rms_failures"] == 0
+    assert j["cpu_baseline"]["gpu_max_rel_err_vs_fp64_thomas"] <= 1e-4
