@@ -47,60 +47,116 @@ __global__ __launch_bounds__(256) void adi_rhs_rows_kernel(const double* __restr
     }
 }
 
-// Dt[j][i] = u[i][j] + r * (u[i][j-1] - 2 u[i][j] + u[i][j+1]) for one 64 x 64
-// tile: rows i0.., columns j0-1 .. j0+64 are read (coalesced along j) into
-// LDS, the tile is written transposed (coalesced along i). Row pitch 67
-// doubles spreads the column reads tile[tx][c] over the banks.
-__global__ __launch_bounds__(256) void adi_rhs_cols_t_kernel(const double* __restrict__ U,
-                                                             double* __restrict__ Dt, int64_t n,
-                                                             double r)
+typedef double adi_d2 __attribute__((ext_vector_type(2)));
+
+// Tile (ty, tx) of block b of a one-dimensional grid of tiles x tiles blocks.
+// Blocks are dealt to the 8 XCDs round-robin; when the tile count is a
+// multiple of 8, XCD k takes the k-th eighth of the tiles in row order, so
+// neighbouring tiles (which share the lines of their halo columns and of
+// their rows) run together on one XCD's L2.
+__device__ __forceinline__ void adi_tile(int64_t b, int64_t tiles, int64_t& ty, int64_t& tx)
 {
-    __shared__ double tile[kTile][kTile + 3];
-    const int tx = threadIdx.x & (kTile - 1);
-    const int ty = threadIdx.x / kTile;  // 0..3
-    const int64_t i0 = (int64_t)blockIdx.y * kTile;
-    const int64_t j0 = (int64_t)blockIdx.x * kTile;
-    // the 64 columns of the tile as one aligned 512-byte row per wave; the
-    // two halo columns j0-1 and j0+64 by two lanes
-    for (int row = ty; row < kTile; row += 4) {
-        const int64_t i = i0 + row;
-        const int64_t j = j0 + tx;
-        tile[row][tx + 1] = (i < n && j < n) ? U[i * n + j] : 0.0;
-        if (tx < 2) {
-            const int64_t jh = tx == 0 ? j0 - 1 : j0 + kTile;
-            tile[row][tx == 0 ? 0 : kTile + 1] =
-                (i < n && jh >= 0 && jh < n) ? U[i * n + jh] : 0.0;
-        }
-    }
-    __syncthreads();
-    const int64_t i = i0 + tx;
-    for (int jj = ty; jj < kTile; jj += 4) {
-        const int64_t j = j0 + jj;
-        if (i < n && j < n) {
-            const double u = tile[tx][jj + 1];
-            Dt[j * n + i] = u + r * (tile[tx][jj] - 2.0 * u + tile[tx][jj + 2]);
-        }
+    const int64_t T = tiles * tiles;
+    const int64_t L = (T % 8 == 0) ? (b % 8) * (T / 8) + b / 8 : b;
+    ty = L / tiles;
+    tx = L - ty * tiles;
+}
+
+// Elements j, j + 1 of row i (zero outside the grid); 16-byte load when the
+// rows are 16-byte aligned (vec: n even) and both are inside.
+__device__ __forceinline__ void adi_load2(const double* __restrict__ U, int64_t n, int64_t i,
+                                          int64_t j, bool vec, double& v0, double& v1)
+{
+    v0 = 0.0;
+    v1 = 0.0;
+    if (i >= n) return;
+    if (vec && j + 1 < n) {
+        const adi_d2 q = *reinterpret_cast<const adi_d2*>(U + i * n + j);
+        v0 = q.x;
+        v1 = q.y;
+    } else {
+        if (j < n) v0 = U[i * n + j];
+        if (j + 1 < n) v1 = U[i * n + j + 1];
     }
 }
 
-// U[i][j] = Xt[j][i], 64 x 64 tiles through LDS (pitch 65 doubles).
-__global__ __launch_bounds__(256) void adi_transpose_kernel(const double* __restrict__ Xt,
-                                                            double* __restrict__ U, int64_t n)
+__device__ __forceinline__ void adi_store2(double* __restrict__ D, int64_t n, int64_t j,
+                                           int64_t i, bool vec, double v0, double v1)
 {
-    __shared__ double tile[kTile][kTile + 1];
-    const int tx = threadIdx.x & (kTile - 1);
-    const int ty = threadIdx.x / kTile;
-    const int64_t j0 = (int64_t)blockIdx.y * kTile;  // rows of Xt
-    const int64_t i0 = (int64_t)blockIdx.x * kTile;  // columns of Xt
-    for (int jj = ty; jj < kTile; jj += 4) {
-        const int64_t j = j0 + jj, i = i0 + tx;
-        if (j < n && i < n) tile[jj][tx] = Xt[j * n + i];
+    if (j >= n) return;
+    if (vec && i + 1 < n) {
+        adi_d2 q;
+        q.x = v0;
+        q.y = v1;
+        *reinterpret_cast<adi_d2*>(D + j * n + i) = q;
+    } else {
+        if (i < n) D[j * n + i] = v0;
+        if (i + 1 < n) D[j * n + i + 1] = v1;
+    }
+}
+
+// Dt[j][i] = u[i][j] + r * (u[i][j-1] - 2 u[i][j] + u[i][j+1]) for one 64 x 64
+// tile: rows i0.., columns j0-1 .. j0+64 are read into LDS (32 lanes per
+// row, two elements each: a wave reads two 512-byte rows per instruction; the
+// two halo columns by 128 single loads), the tile is written transposed the
+// same way. Row pitch 67 doubles.
+__global__ __launch_bounds__(256) void adi_rhs_cols_t_kernel(const double* __restrict__ U,
+                                                             double* __restrict__ Dt, int64_t n,
+                                                             double r, int vec)
+{
+    __shared__ double tile[kTile][kTile + 3];
+    const int t = threadIdx.x;
+    const int cx = (t & 31) * 2;  // column pair of the tile
+    const int ry = t / 32;        // 0..7
+    int64_t ty, tx;
+    adi_tile(blockIdx.x, (n + kTile - 1) / kTile, ty, tx);
+    const int64_t i0 = ty * kTile;
+    const int64_t j0 = tx * kTile;
+    for (int row = ry; row < kTile; row += 8) {
+        double v0, v1;
+        adi_load2(U, n, i0 + row, j0 + cx, vec != 0, v0, v1);
+        tile[row][cx + 1] = v0;
+        tile[row][cx + 2] = v1;
+    }
+    if (t < 2 * kTile) {
+        const int row = t >> 1;
+        const int64_t i = i0 + row;
+        const int64_t jh = (t & 1) ? j0 + kTile : j0 - 1;
+        tile[row][(t & 1) ? kTile + 1 : 0] = (i < n && jh >= 0 && jh < n) ? U[i * n + jh] : 0.0;
     }
     __syncthreads();
-    for (int ii = ty; ii < kTile; ii += 4) {
-        const int64_t i = i0 + ii, j = j0 + tx;
-        if (i < n && j < n) U[i * n + j] = tile[tx][ii];
+    for (int jj = ry; jj < kTile; jj += 8) {
+        const double u0 = tile[cx][jj + 1];
+        const double u1 = tile[cx + 1][jj + 1];
+        const double r0 = u0 + r * (tile[cx][jj] - 2.0 * u0 + tile[cx][jj + 2]);
+        const double r1 = u1 + r * (tile[cx + 1][jj] - 2.0 * u1 + tile[cx + 1][jj + 2]);
+        adi_store2(Dt, n, j0 + jj, i0 + cx, vec != 0, r0, r1);
     }
+}
+
+// U[i][j] = Xt[j][i], 64 x 64 tiles through LDS (pitch 65 doubles), the
+// same access shape as the stencil kernel.
+__global__ __launch_bounds__(256) void adi_transpose_kernel(const double* __restrict__ Xt,
+                                                            double* __restrict__ U, int64_t n,
+                                                            int vec)
+{
+    __shared__ double tile[kTile][kTile + 1];
+    const int t = threadIdx.x;
+    const int cx = (t & 31) * 2;
+    const int ry = t / 32;
+    int64_t ty, tx;
+    adi_tile(blockIdx.x, (n + kTile - 1) / kTile, ty, tx);
+    const int64_t j0 = ty * kTile;  // rows of Xt
+    const int64_t i0 = tx * kTile;  // columns of Xt
+    for (int jj = ry; jj < kTile; jj += 8) {
+        double v0, v1;
+        adi_load2(Xt, n, j0 + jj, i0 + cx, vec != 0, v0, v1);
+        tile[jj][cx] = v0;
+        tile[jj][cx + 1] = v1;
+    }
+    __syncthreads();
+    for (int ii = ry; ii < kTile; ii += 8)
+        adi_store2(U, n, i0 + ii, j0 + cx, vec != 0, tile[cx][ii], tile[cx + 1][ii]);
 }
 
 hipError_t launch_adi_steps(const AdiArgs& g, int steps)
@@ -110,7 +166,12 @@ hipError_t launch_adi_steps(const AdiArgs& g, int steps)
     int64_t blocks = (total + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     const int64_t tiles = (n + kTile - 1) / kTile;
-    if (tiles > 65535) return hipErrorInvalidConfiguration;
+    if (tiles * tiles > 0x7fffffffLL) return hipErrorInvalidConfiguration;
+    const int tvec = (n % 2 == 0 && (reinterpret_cast<uintptr_t>(g.u) & 15u) == 0 &&
+                      (reinterpret_cast<uintptr_t>(g.t) & 15u) == 0 &&
+                      (reinterpret_cast<uintptr_t>(g.t2) & 15u) == 0)
+                         ? 1
+                         : 0;
     hipError_t e;
     RegArgs<double> ra{};
     ra.a = ra.b = ra.c = nullptr;
@@ -131,7 +192,7 @@ hipError_t launch_adi_steps(const AdiArgs& g, int steps)
         ra.x = x;
         return ra.L <= kWave ? launch_reg_crpcr<double>(ra) : launch_reg_mw<double>(ra);
     };
-    const dim3 tgrid((unsigned)tiles, (unsigned)tiles);
+    const dim3 tgrid((unsigned)(tiles * tiles));
     for (int k = 0; k < steps; ++k) {
         // row sweep: rhs into T, solve T -> U (u*). After a step whose u' was
         // left transposed in T2, the same tiled kernel forms the row rhs from
@@ -141,18 +202,19 @@ hipError_t launch_adi_steps(const AdiArgs& g, int steps)
                                g.stream, g.u, g.t, n, g.r);
         else
             hipLaunchKernelGGL(adi_rhs_cols_t_kernel, tgrid, dim3(256), 0, g.stream, g.t2, g.t,
-                               n, g.r);
+                               n, g.r, tvec);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = solve(g.t, g.u)) != hipSuccess) return e;
         // column sweep: rhs transposed into T, solve T -> T2 (u' transposed)
         hipLaunchKernelGGL(adi_rhs_cols_t_kernel, tgrid, dim3(256), 0, g.stream, g.u, g.t, n,
-                           g.r);
+                           g.r, tvec);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = solve(g.t, g.t2)) != hipSuccess) return e;
     }
     // the grid itself after the last step: transpose T2 -> U
     if (steps > 0) {
-        hipLaunchKernelGGL(adi_transpose_kernel, tgrid, dim3(256), 0, g.stream, g.t2, g.u, n);
+        hipLaunchKernelGGL(adi_transpose_kernel, tgrid, dim3(256), 0, g.stream, g.t2, g.u, n,
+                           tvec);
         return hipGetLastError();
     }
     return hipSuccess;
