@@ -192,7 +192,8 @@ int trid_solve_refined_f64(trid_handle handle, int method,
  * rocSPARSE's gtsv_no_pivot_strided_batch; batchStride == systemSize is the
  * contiguous batch of trid_solve_*. No counterpart in the reference
  * (contiguous only). Methods TRID_CR, TRID_PCR, TRID_CRPCR and TRID_AUTO on
- * the register kernels (systemSize up to 16384 FP32 / 8192 FP64); other
+ * the register kernels (systemSize up to 16384 FP32 / 8192 FP64); CR and
+ * CR-PCR above that by separators (handle workspace, as trid_solve_*); other
  * methods or sizes return TRID_ENOTSUP. Elements between systems are neither
  * read nor written.
  */

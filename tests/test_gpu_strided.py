@@ -28,6 +28,9 @@ def test_strided_matches_oracle(oracle, trid, dtype, method):
     cases = [(1024, 300, 1100), (64, 1000, 80), (64, 333, 65), (1000, 50, 1003)]
     if method != "pcr":
         cases.append((8192, 40, 8200))
+        # past the register sizes (the separator form): 16-byte path, element
+        # path (odd stride), a partial last chunk
+        cases += [(40000, 3, 40960), (40000, 3, 40003), (17000, 4, 20000)]
     for n, N, bs in cases:
         a, b, c, d = oracle.generate("dominant", 50 + n, n, N, dtype)
         dev = [torch.from_numpy(_spread(v, n, N, bs, 7.0)).cuda() for v in (a, b, c, d)]

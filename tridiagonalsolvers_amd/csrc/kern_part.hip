@@ -29,8 +29,10 @@
 // HBM traffic per unknown: 4 T read (pass 1) + 4 T read, 1 T written
 // (pass 3), plus 21 equations and 5 separator values per 1,024 unknowns:
 // 9 T + 0.35 bytes, against the 5 T of a solve that fits a workgroup.
-// Rows past n are identity rows (x = 0), so any n works; loads and stores
-// are 16-byte vectors staged through LDS when n is a multiple of 16 bytes
+// Rows past n are identity rows (x = 0), so any n works. System s starts at
+// s * bs (bs = n: a contiguous batch; bs > n: a strided one, the elements
+// between systems are neither read nor written). Loads and stores are
+// 16-byte vectors staged through LDS when n and bs are multiples of 16 bytes
 // and the arrays are 16-byte aligned, element-wise otherwise.
 #include "reg_core.hpp"
 #include "reg_io.hpp"
@@ -79,14 +81,14 @@ __device__ __forceinline__ Eq<T> get_eq(const T* p)
 template <typename T>
 __device__ __forceinline__ void part_load(const T* __restrict__ A, const T* __restrict__ B,
                                           const T* __restrict__ C, const T* __restrict__ D,
-                                          int64_t n, int64_t K, int64_t chunk, int lane,
-                                          char* lds, bool vec, Eq<T> (&e)[kPM], int64_t& base,
-                                          int64_t& rows, int64_t& k)
+                                          int64_t n, int64_t bs, int64_t K, int64_t chunk,
+                                          int lane, char* lds, bool vec, Eq<T> (&e)[kPM],
+                                          int64_t& base, int64_t& rows, int64_t& k)
 {
     const int64_t sys = chunk / K;
     k = chunk - sys * K;
     const int64_t r0 = k * kChunk;
-    base = sys * n + r0;
+    base = sys * bs + r0;
     rows = n - r0 < kChunk ? n - r0 : kChunk;
     if (vec) {
         load_staged<T, kPM, true>(A, B, C, D, base, base + rows, lane, lds, e,
@@ -113,8 +115,8 @@ __global__ __launch_bounds__(256) void part_reduce_kernel(const T* __restrict__ 
                                                           const T* __restrict__ B,
                                                           const T* __restrict__ C,
                                                           const T* __restrict__ D, int64_t n,
-                                                          int64_t K, int64_t chunks, int vec,
-                                                          T* __restrict__ rec)
+                                                          int64_t bs, int64_t K, int64_t chunks,
+                                                          int vec, T* __restrict__ rec)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t chunk = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
@@ -123,7 +125,7 @@ __global__ __launch_bounds__(256) void part_reduce_kernel(const T* __restrict__ 
     char* lds = trid_smem + (threadIdx.x / kWave) * Stage<T, kPM>::kWaveBytes;
     Eq<T> e[kPM];
     int64_t base, rows, k;
-    part_load<T>(A, B, C, D, n, K, chunk, lane, lds, vec != 0, e, base, rows, k);
+    part_load<T>(A, B, C, D, n, bs, K, chunk, lane, lds, vec != 0, e, base, rows, k);
 
     // the record is gathered in the wave's staging area (free once the
     // chunk is loaded), then leaves as one contiguous block
@@ -198,9 +200,9 @@ __global__ __launch_bounds__(256) void part_solve_kernel(const T* __restrict__ A
                                                          const T* __restrict__ B,
                                                          const T* __restrict__ C,
                                                          const T* __restrict__ D,
-                                                         T* __restrict__ X, int64_t n, int64_t K,
-                                                         int64_t chunks, int vec, int rest,
-                                                         const T* __restrict__ xs)
+                                                         T* __restrict__ X, int64_t n, int64_t bs,
+                                                         int64_t K, int64_t chunks, int vec,
+                                                         int rest, const T* __restrict__ xs)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t chunk = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
@@ -213,7 +215,7 @@ __global__ __launch_bounds__(256) void part_solve_kernel(const T* __restrict__ A
     const T xr = lane == kWave - 1 ? xs[chunk] : T(0);
     Eq<T> e[kPM];
     int64_t base, rows, k;
-    part_load<T>(A, B, C, D, n, K, chunk, lane, lds, vec != 0, e, base, rows, k);
+    part_load<T>(A, B, C, D, n, bs, K, chunk, lane, lds, vec != 0, e, base, rows, k);
     // the left separator's value enters the first row's right-hand side;
     // the chunk's own separator row becomes x = its value
     if (lane == 0) {
@@ -286,8 +288,9 @@ hipError_t launch_part(const PartArgs<T>& g)
         ws += round256((size_t)chunks * sizeof(T));
     }
     constexpr int E = 16 / (int)sizeof(T);
-    const int vec = (g.n % E == 0 && aligned16(g.a) && aligned16(g.b) && aligned16(g.c) &&
-                     aligned16(g.d) && aligned16(g.x))
+    const int64_t bs = g.bs ? g.bs : g.n;
+    const int vec = (g.n % E == 0 && bs % E == 0 && aligned16(g.a) && aligned16(g.b) &&
+                     aligned16(g.c) && aligned16(g.d) && aligned16(g.x))
                         ? 1
                         : 0;
     // pass 1 gathers its record in the staging area on either load path
@@ -295,7 +298,7 @@ hipError_t launch_part(const PartArgs<T>& g)
     const size_t lds = vec ? lds1 : 0;
 
     hipLaunchKernelGGL((part_reduce_kernel<T>), dim3((unsigned)blocks), dim3(256), lds1, g.stream,
-                       g.a, g.b, g.c, g.d, g.n, K, chunks, vec, rec);
+                       g.a, g.b, g.c, g.d, g.n, bs, K, chunks, vec, rec);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((part_sep_kernel<T>), dim3((unsigned)((chunks + 255) / 256)), dim3(256),
@@ -329,6 +332,7 @@ hipError_t launch_part(const PartArgs<T>& g)
         q.d = sep[3];
         q.x = sep[4];
         q.n = K;
+        q.bs = 0;
         q.rest = p.rest;
         q.ws = ws;
         e = launch_part<T>(q);
@@ -340,7 +344,7 @@ hipError_t launch_part(const PartArgs<T>& g)
 #define TRID_PART_CASE(RR)                                                                     \
     case RR:                                                                                   \
         hipLaunchKernelGGL((part_solve_kernel<T, RR>), dim3((unsigned)blocks), dim3(256), lds, \
-                           g.stream, g.a, g.b, g.c, g.d, g.x, g.n, K, chunks, vec, g.rest,      \
+                           g.stream, g.a, g.b, g.c, g.d, g.x, g.n, bs, K, chunks, vec, g.rest,  \
                            sep[4]);                                                            \
         break;
     switch (R) {

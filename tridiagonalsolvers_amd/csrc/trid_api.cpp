@@ -20,7 +20,7 @@
 
 struct trid_context {
     int device;
-    void* ws;         // workspace of the two-pass pcrGauss form and interleaved solves
+    void* ws;         // workspace: two-pass pcrGauss, separator form, interleaved, pivoting
     size_t ws_bytes;
 };
 
@@ -183,7 +183,8 @@ int solve_impl(trid_handle h, int method, const T* a, const T* b, const T* c, co
     }
 }
 
-// Strided batch: system s at s * bs (bs >= n), the register family only.
+// Strided batch: system s at s * bs (bs >= n): the register kernels, and the
+// separator form for systems longer than they hold.
 template <typename T>
 int strided_impl(trid_context* h, int method, const T* a, const T* b, const T* c, const T* d,
                  T* x, int64_t n, int64_t N, int64_t bs, const trid_opts* opts,
@@ -202,9 +203,28 @@ int strided_impl(trid_context* h, int method, const T* a, const T* b, const T* c
     trid::Plan p;
     int st = trid::make_plan(method, n, N, (int)sizeof(T), o, &p);
     if (st != TRID_OK) return st;
-    if (p.family != trid::Family::Reg) return TRID_ENOTSUP;
+    if (p.family != trid::Family::Reg && p.family != trid::Family::Part) return TRID_ENOTSUP;
     st = check_device();
     if (st != TRID_OK) return st;
+    if (p.family == trid::Family::Part) {
+        const int m = method == TRID_CR ? TRID_CR : TRID_CRPCR;
+        st = ensure_ws(h, trid::part_ws_bytes<T>(m, n, N));
+        if (st != TRID_OK) return st;
+        trid::PartArgs<T> g{};
+        g.a = a;
+        g.b = b;
+        g.c = c;
+        g.d = d;
+        g.x = x;
+        g.n = n;
+        g.N = N;
+        g.bs = bs;
+        g.method = m;
+        g.rest = p.rest;
+        g.ws = h->ws;
+        g.stream = stream;
+        return from_hip(trid::launch_part<T>(g));
+    }
     trid::RegArgs<T> g{};
     g.a = a;
     g.b = b;

@@ -135,6 +135,7 @@ struct PartArgs {
     T* x;
     int64_t n;
     int64_t N;
+    int64_t bs;   // elements from one system's first row to the next's (0: n)
     int method;   // TRID_CR or TRID_CRPCR (the separator system's method)
     int rest;     // PCR system size of the chunk solves: 2, 4, .., 512
     void* ws;     // part_ws_bytes<T>(method, n, N) bytes, 256-byte aligned
