@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void part_reduce_kernel(const T* __restrict__ 
     for (int lv = 1; lv <= Log2<kPM>::value; ++lv) {
         const int SS = 1 << lv, H = SS >> 1;
         if (lane == kWave - 1) put_eq(rl + 4 * lv, e[kPM - 1 - H]);
-        const Eq<T> nb = lane_read_eq(e[H - 1], lane + 1, lane + 1 < kWave);
+        const Eq<T> nb = lane_step_eq<1>(e[H - 1], lane + 1 < kWave);
 #pragma unroll
         for (int j = 0; j < kPM / SS; ++j) {
             const int i = SS * j + SS - 1;

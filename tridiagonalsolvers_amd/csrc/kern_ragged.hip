@@ -335,8 +335,8 @@ __device__ void wave_item(const RaggedArgs& g, const RaggedItem& it, const Ragge
     Eq<float> e[M];
     float xf[M];
     for (int step = 0; step <= g.iters; ++step) {
-        const double xl = lane_read_or_zero(x[M - 1], lane - 1, ll >= 1);
-        const double xr = lane_read_or_zero(x[0], lane + 1, ll + 1 < L);
+        const double xl = lane_step_or_zero<-1>(x[M - 1], ll >= 1);
+        const double xr = lane_step_or_zero<1>(x[0], ll + 1 < L);
         ragged_rows<M>(g, x, xl, xr, base, n, j0, n, L, ll, lane, live, stage, e);
         wave_core<float, M, 1>(e, xf, lane, ll, L, bk.rest);
 #pragma unroll

@@ -107,8 +107,8 @@ __global__ __launch_bounds__(256) void refine_wave_kernel(const double* __restri
     Eq<float> e[M];
     float xf[M];
     for (int it = 0; it <= iters; ++it) {
-        const double xl = lane_read_or_zero(x[M - 1], lane - 1, ll >= 1);
-        const double xr = lane_read_or_zero(x[0], lane + 1, ll + 1 < L);
+        const double xl = lane_step_or_zero<-1>(x[M - 1], ll >= 1);
+        const double xr = lane_step_or_zero<1>(x[0], ll + 1 < L);
         build_rows<M>(A, B, C, D, x, xl, xr, base, j0, n, live, vec != 0, e);
         wave_core<float, M, R>(e, xf, lane, ll, L, rest);
 #pragma unroll

@@ -32,7 +32,7 @@ __device__ __forceinline__ void wave_core(Eq<T> (&e)[M], T (&xv)[M], int lane, i
         const int SS = 1 << k, H = SS >> 1;
         // The top equation of this lane needs the right lane's equation at
         // slot H-1, which that lane finished at level k-1.
-        const Eq<T> nb = lane_read_eq(e[H - 1], lane + 1, ll + 1 < L);
+        const Eq<T> nb = lane_step_eq<1>(e[H - 1], ll + 1 < L);
 #pragma unroll
         for (int j = 0; j < M / SS; ++j) {
             const int i = SS * j + SS - 1;
@@ -47,24 +47,37 @@ __device__ __forceinline__ void wave_core(Eq<T> (&e)[M], T (&xv)[M], int lane, i
 
     if constexpr (R == 1) {
         // ---- 2. cross-lane CR levels on slot M-1 while count > rest ------
+        // (the first level reads the neighbouring lanes by DPP, the later
+        // ones by ds_bpermute; the same for steps 3 and 5a)
         int hs = 1;
-        for (int count = L; count > rest; count >>= 1, hs <<= 1) {
+        int count = L;
+        auto cr_level = [&](const Eq<T>& lft, const Eq<T>& rgt) {
             const bool active = ((ll + 1) & (2 * hs - 1)) == 0;
-            const Eq<T> lft = lane_read_eq(e[M - 1], lane - hs, ll - hs >= 0);
-            const Eq<T> rgt = lane_read_eq(e[M - 1], lane + hs, ll + hs < L);
             const Eq<T> r = reduce_eq(e[M - 1], lft, rgt);
             if (active) e[M - 1] = r;
-        }
+            count >>= 1;
+            hs <<= 1;
+        };
+        if (count > rest)
+            cr_level(lane_step_eq<-1>(e[M - 1], ll >= 1), lane_step_eq<1>(e[M - 1], ll + 1 < L));
+        while (count > rest)
+            cr_level(lane_read_eq(e[M - 1], lane - hs, ll - hs >= 0),
+                     lane_read_eq(e[M - 1], lane + hs, ll + hs < L));
         // hs = L / rest: spacing of the lanes that hold the rest system
         const int LS = hs;
         const bool pact = ((ll + 1) & (LS - 1)) == 0;
         // ---- 3. PCR: log2(rest/2) levels, lane distance LS .. L/4 ----------
-        for (int dist = LS; dist < (L >> 1); dist <<= 1) {
-            const Eq<T> lft = lane_read_eq(e[M - 1], lane - dist, ll - dist >= 0);
-            const Eq<T> rgt = lane_read_eq(e[M - 1], lane + dist, ll + dist < L);
+        int dist = LS;
+        auto pcr_level = [&](const Eq<T>& lft, const Eq<T>& rgt) {
             const Eq<T> r = reduce_eq(e[M - 1], lft, rgt);
             if (pact) e[M - 1] = r;
-        }
+            dist <<= 1;
+        };
+        if (dist == 1 && dist < (L >> 1))
+            pcr_level(lane_step_eq<-1>(e[M - 1], ll >= 1), lane_step_eq<1>(e[M - 1], ll + 1 < L));
+        while (dist < (L >> 1))
+            pcr_level(lane_read_eq(e[M - 1], lane - dist, ll - dist >= 0),
+                      lane_read_eq(e[M - 1], lane + dist, ll + dist < L));
         // ---- 4. 2x2 solves at lane distance L/2 --------------------------
         const int half = L >> 1;
         const bool lower = ll < half;
@@ -77,13 +90,17 @@ __device__ __forceinline__ void wave_core(Eq<T> (&e)[M], T (&xv)[M], int lane, i
             xv[M - 1] = lower ? x1 : x2;
         }
         // ---- 5a. cross-lane CR back-substitution -------------------------
-        for (int h = LS >> 1; h >= 1; h >>= 1) {
-            const T xl = lane_read_or_zero(xv[M - 1], lane - h, ll - h >= 0);
-            const T xr = lane_read_or_zero(xv[M - 1], lane + h, ll + h < L);
+        auto bs_level = [&](int h, T xl, T xr) {
             const bool solve = ((ll + 1) & (2 * h - 1)) == h;
             const T v = backsub(e[M - 1], xl, xr);
             if (solve) xv[M - 1] = v;
-        }
+        };
+        for (int h = LS >> 1; h >= 2; h >>= 1)
+            bs_level(h, lane_read_or_zero(xv[M - 1], lane - h, ll - h >= 0),
+                     lane_read_or_zero(xv[M - 1], lane + h, ll + h < L));
+        if (LS >= 2)
+            bs_level(1, lane_step_or_zero<-1>(xv[M - 1], ll >= 1),
+                     lane_step_or_zero<1>(xv[M - 1], ll + 1 < L));
     } else {
         // rest = R * L: lane ll holds PCR equations q = ll*R + r at slots
         // S*r + S - 1, r = 0..R-1.
@@ -95,8 +112,8 @@ __device__ __forceinline__ void wave_core(Eq<T> (&e)[M], T (&xv)[M], int lane, i
 #pragma unroll
             for (int q = 0; q < R; ++q) {
                 if (q < dl) {
-                    fromL[q] = lane_read_eq(e[S * (R - dl + q) + S - 1], lane - 1, ll >= 1);
-                    fromR[q] = lane_read_eq(e[S * q + S - 1], lane + 1, ll + 1 < L);
+                    fromL[q] = lane_step_eq<-1>(e[S * (R - dl + q) + S - 1], ll >= 1);
+                    fromR[q] = lane_step_eq<1>(e[S * q + S - 1], ll + 1 < L);
                 }
             }
             Eq<T> nw[R];
@@ -112,7 +129,18 @@ __device__ __forceinline__ void wave_core(Eq<T> (&e)[M], T (&xv)[M], int lane, i
             for (int r = 0; r < R; ++r) e[S * r + S - 1] = nw[r];
         }
         // ---- 3b. PCR levels with distance >= R: lane distance 1 .. L/4 ---
-        for (int ld = 1; ld < (L >> 1); ld <<= 1) {
+        if (1 < (L >> 1)) {  // lane distance 1: DPP
+            Eq<T> nw[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const Eq<T> lft = lane_step_eq<-1>(e[S * r + S - 1], ll >= 1);
+                const Eq<T> rgt = lane_step_eq<1>(e[S * r + S - 1], ll + 1 < L);
+                nw[r] = reduce_eq(e[S * r + S - 1], lft, rgt);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) e[S * r + S - 1] = nw[r];
+        }
+        for (int ld = 2; ld < (L >> 1); ld <<= 1) {
             Eq<T> nw[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
@@ -139,7 +167,7 @@ __device__ __forceinline__ void wave_core(Eq<T> (&e)[M], T (&xv)[M], int lane, i
     }
 
     // ---- 5b. in-lane CR back-substitution --------------------------------
-    const T xprev = lane_read_or_zero(xv[M - 1], lane - 1, ll >= 1);
+    const T xprev = lane_step_or_zero<-1>(xv[M - 1], ll >= 1);
 #pragma unroll
     for (int k = KIN; k >= 1; --k) {
         const int SS = 1 << k, H = SS >> 1;
@@ -175,7 +203,7 @@ __device__ __forceinline__ void wg_core(Eq<T> (&e)[M], T (&xv)[M], int w, int W,
 #pragma unroll
     for (int k = 1; k <= LOGM; ++k) {
         const int SS = 1 << k, H = SS >> 1;
-        const Eq<T> nb = lane_read_eq(e[H - 1], lane + 1, lane + 1 < kWave);
+        const Eq<T> nb = lane_step_eq<1>(e[H - 1], lane + 1 < kWave);
 #pragma unroll
         for (int j = 0; j < M / SS; ++j) {
             const int i = SS * j + SS - 1;

@@ -118,6 +118,51 @@ __device__ __forceinline__ T lane_read_or_zero(T v, int src, bool valid)
     return valid ? r : T(0);
 }
 
+// Value of lane (lane + D), D = +1 or -1, by a DPP wave shift (wave_shl:1 /
+// wave_shr:1): a VALU move that reads the neighbouring lane's register, with
+// no trip through the LDS crossbar that ds_bpermute takes. The lane shifted in
+// at the end of the wave reads 0; callers mask the lanes whose neighbour lies
+// outside their system, as they do for ds_bpermute reads. D is a template
+// argument so that the choice of instruction is made at compile time.
+template <int D>
+__device__ __forceinline__ int lane_step_i32(int v)
+{
+    static_assert(D == 1 || D == -1, "DPP wave shift by one lane");
+    return __builtin_amdgcn_update_dpp(0, v, D == 1 ? 0x130 : 0x138, 0xf, 0xf, false);
+}
+template <int D>
+__device__ __forceinline__ float lane_step(float v)
+{
+    return __int_as_float(lane_step_i32<D>(__float_as_int(v)));
+}
+template <int D>
+__device__ __forceinline__ double lane_step(double v)
+{
+    const int lo = lane_step_i32<D>(__double2loint(v));
+    const int hi = lane_step_i32<D>(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+// Equation of lane (lane + D); the identity row where `valid` is false.
+template <int D, typename T>
+__device__ __forceinline__ Eq<T> lane_step_eq(const Eq<T>& e, bool valid)
+{
+    Eq<T> r;
+    r.a = lane_step<D>(e.a);
+    r.b = lane_step<D>(e.b);
+    r.c = lane_step<D>(e.c);
+    r.d = lane_step<D>(e.d);
+    if (!valid) r = identity_eq<T>();
+    return r;
+}
+
+template <int D, typename T>
+__device__ __forceinline__ T lane_step_or_zero(T v, bool valid)
+{
+    const T r = lane_step<D>(v);
+    return valid ? r : T(0);
+}
+
 // Value of `v` in lane `src` (wave-uniform index), broadcast to all lanes.
 __device__ __forceinline__ float lane_bcast(float v, int src)
 {
