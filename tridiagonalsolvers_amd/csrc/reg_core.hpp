@@ -81,7 +81,10 @@ __device__ __forceinline__ void wave_core(Eq<T> (&e)[M], T (&xv)[M], int lane, i
         // ---- 4. 2x2 solves at lane distance L/2 --------------------------
         const int half = L >> 1;
         const bool lower = ll < half;
-        const Eq<T> partner = lane_read_eq(e[M - 1], lower ? lane + half : lane - half, true);
+        Eq<T> partner;
+        if (L == 64) partner = lane_xor_eq<32>(e[M - 1], lane);
+        else if (L == 32) partner = lane_xor_eq<16>(e[M - 1], lane);
+        else partner = lane_read_eq(e[M - 1], lower ? lane + half : lane - half, true);
         {
             const Eq<T> lo = lower ? e[M - 1] : partner;
             const Eq<T> hi = lower ? partner : e[M - 1];
@@ -130,40 +133,46 @@ __device__ __forceinline__ void wave_core(Eq<T> (&e)[M], T (&xv)[M], int lane, i
         }
         // ---- 3b. PCR levels with distance >= R: lane distance 1 .. L/4 ---
         if (1 < (L >> 1)) {  // lane distance 1: DPP
-            Eq<T> nw[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const Eq<T> lft = lane_step_eq<-1>(e[S * r + S - 1], ll >= 1);
                 const Eq<T> rgt = lane_step_eq<1>(e[S * r + S - 1], ll + 1 < L);
-                nw[r] = reduce_eq(e[S * r + S - 1], lft, rgt);
+                e[S * r + S - 1] = reduce_eq(e[S * r + S - 1], lft, rgt);
             }
-#pragma unroll
-            for (int r = 0; r < R; ++r) e[S * r + S - 1] = nw[r];
         }
+        // (equation r of a lane meets only equation r of other lanes here, so
+        // each is updated in place once its neighbours are read)
         for (int ld = 2; ld < (L >> 1); ld <<= 1) {
-            Eq<T> nw[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const Eq<T> lft = lane_read_eq(e[S * r + S - 1], lane - ld, ll - ld >= 0);
                 const Eq<T> rgt = lane_read_eq(e[S * r + S - 1], lane + ld, ll + ld < L);
-                nw[r] = reduce_eq(e[S * r + S - 1], lft, rgt);
+                e[S * r + S - 1] = reduce_eq(e[S * r + S - 1], lft, rgt);
             }
-#pragma unroll
-            for (int r = 0; r < R; ++r) e[S * r + S - 1] = nw[r];
         }
         // ---- 4. 2x2 solves at lane distance L/2 --------------------------
         const int half = L >> 1;
         const bool lower = ll < half;
+        auto solve_pairs = [&](auto partner_of) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const Eq<T> own = e[S * r + S - 1];
-            const Eq<T> partner = lane_read_eq(own, lower ? lane + half : lane - half, true);
-            const Eq<T> lo = lower ? own : partner;
-            const Eq<T> hi = lower ? partner : own;
-            T x1, x2;
-            solve2(lo, hi, x1, x2);
-            xv[S * r + S - 1] = lower ? x1 : x2;
-        }
+            for (int r = 0; r < R; ++r) {
+                const Eq<T> own = e[S * r + S - 1];
+                const Eq<T> partner = partner_of(own);
+                const Eq<T> lo = lower ? own : partner;
+                const Eq<T> hi = lower ? partner : own;
+                T x1, x2;
+                solve2(lo, hi, x1, x2);
+                xv[S * r + S - 1] = lower ? x1 : x2;
+            }
+        };
+        if (L == 64)
+            solve_pairs([&](const Eq<T>& q) { return lane_xor_eq<32>(q, lane); });
+        else if (L == 32)
+            solve_pairs([&](const Eq<T>& q) { return lane_xor_eq<16>(q, lane); });
+        else
+            solve_pairs([&](const Eq<T>& q) {
+                return lane_read_eq(q, lower ? lane + half : lane - half, true);
+            });
     }
 
     // ---- 5b. in-lane CR back-substitution --------------------------------

@@ -163,6 +163,41 @@ __device__ __forceinline__ T lane_step_or_zero(T v, bool valid)
     return valid ? r : T(0);
 }
 
+// Value of lane (lane ^ X), X = 32 or 16, by gfx950's v_permlane32_swap /
+// v_permlane16_swap: the swap of a register with itself leaves, in its two
+// results, each half (row pair) of the wave next to its partner half, and the
+// lane picks the one that is not its own. A VALU operation, no LDS crossbar.
+template <int X>
+__device__ __forceinline__ int lane_xor_i32(int v, int lane)
+{
+    static_assert(X == 32 || X == 16, "v_permlane32_swap / v_permlane16_swap");
+    if constexpr (X == 32) {
+        const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return (lane & 32) ? r[0] : r[1];
+    } else {
+        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return (lane & 16) ? r[0] : r[1];
+    }
+}
+template <int X>
+__device__ __forceinline__ float lane_xor(float v, int lane)
+{
+    return __int_as_float(lane_xor_i32<X>(__float_as_int(v), lane));
+}
+template <int X>
+__device__ __forceinline__ double lane_xor(double v, int lane)
+{
+    const int lo = lane_xor_i32<X>(__double2loint(v), lane);
+    const int hi = lane_xor_i32<X>(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+template <int X, typename T>
+__device__ __forceinline__ Eq<T> lane_xor_eq(const Eq<T>& e, int lane)
+{
+    return Eq<T>{lane_xor<X>(e.a, lane), lane_xor<X>(e.b, lane), lane_xor<X>(e.c, lane),
+                 lane_xor<X>(e.d, lane)};
+}
+
 // Value of `v` in lane `src` (wave-uniform index), broadcast to all lanes.
 __device__ __forceinline__ float lane_bcast(float v, int src)
 {
