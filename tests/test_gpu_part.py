@@ -106,3 +106,15 @@ def test_64x1048576_full_size(oracle, trid):
     a, b, c, d = oracle.generate("dominant", 9, n, N, np.float32)
     worst = check(oracle, x.cpu().numpy(), a, b, c, d, n, np.float32)
     print(f"64x1048576 f32: max rel err {worst:.3e}", flush=True)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_host_dropins_past_the_register_sizes(oracle, trid, dtype):
+    """The reference-named host entry points (crpcr_app.cu:5, cr_app.cu:4)
+    take the separator form for long systems, with a workspace of their own."""
+    n, N = 40000, 3
+    a, b, c, d = oracle.generate("dominant", 44, n, N, dtype)
+    for name, args in (("crpcr", (64,)), ("cr", ())):
+        x = np.full_like(a, np.nan)
+        getattr(trid, name)(a, b, c, d, x, n, N, *args)
+        check(oracle, x, a, b, c, d, n, dtype)
