@@ -32,12 +32,19 @@ namespace {
 
 constexpr int kIlM = 16;  // unknowns per lane: n_pad = 1024
 
-// S systems (waves) per workgroup: 16 in FP32 (1024 threads, 109 VGPRs), 8 in
-// FP64 (512 threads: the FP64 core needs about 210 VGPRs). Either way a row of
-// the workgroup's systems is 64 bytes.
+// S systems (waves) per workgroup (TRID_IL_SYSTEMS_F32 in FP32, 8 in FP64:
+// the FP64 core needs about 210 VGPRs). A row of the workgroup's systems is
+// S * sizeof(T) bytes, a part of a 128-byte line shared by P workgroups.
+// FP32 at 16 (1,024 threads, one workgroup per CU) takes 444-463 us on
+// 65,536 x 1,024; at 8 (512 threads, two workgroups per CU, 32-byte row
+// parts, P = 4) 564 us: the narrower parts cost more than the overlap of
+// two workgroups gains.
+#ifndef TRID_IL_SYSTEMS_F32
+#define TRID_IL_SYSTEMS_F32 16
+#endif
 template <typename T>
 struct IlGeom {
-    static constexpr int S = sizeof(T) == 4 ? 16 : 8;
+    static constexpr int S = sizeof(T) == 4 ? TRID_IL_SYSTEMS_F32 : 8;
     static constexpr int EV = 16 / (int)sizeof(T);           // elements per vector
     static constexpr int VPR = S / EV;                        // vectors per row
     static constexpr int RC = S * kWave / VPR;                // rows per chunk
@@ -46,18 +53,21 @@ struct IlGeom {
     static constexpr int GP = kIlM + EV;                      // padded lane group
     static constexpr int SP = LC * GP + EV;                   // system pitch
     static constexpr int AP = S * SP;                         // array pitch
-    static_assert(LC * NC == kWave && S * (int)sizeof(T) == 64, "chunk geometry");
+    static constexpr int P = 128 / (S * (int)sizeof(T));      // workgroups per 128-byte line
+    static_assert(LC * NC == kWave && P * S * (int)sizeof(T) == 128, "chunk geometry");
 };
 
-// Workgroup -> group of S systems. A 64-byte row is half a 128-byte line; the
-// other half belongs to the neighbouring group. Workgroups are dealt to the 8
-// XCDs round-robin, so b and b + 8 run at about the same time on one XCD:
-// they take the two halves of the same lines, and the second read of a line
-// hits that XCD's L2. (The grid is a multiple of 16 so the map is one-to-one.)
+// Workgroup -> group of S systems. A row of the group is 1/P of a 128-byte
+// line; the other parts belong to the neighbouring groups. Workgroups are
+// dealt to the 8 XCDs round-robin, so b, b + 8, .., b + 8 (P-1) run at about
+// the same time on one XCD: they take the P parts of the same lines, and
+// every read of a line after the first hits that XCD's L2. (The grid is a
+// multiple of 8 P so the map is one-to-one.)
+template <int P>
 __device__ __forceinline__ int64_t il_group(int64_t b)
 {
     const int64_t x = b % 8, k = b / 8;
-    return 2 * ((k / 2) * 8 + x) + (k % 2);
+    return P * ((k / P) * 8 + x) + (k % P);
 }
 
 template <typename T>
@@ -106,7 +116,7 @@ __global__ __launch_bounds__(IlGeom<T>::S * kWave) void il_reg_kernel(
     const int t = threadIdx.x;
     const int w = t / kWave;
     const int lane = t & (kWave - 1);
-    const int64_t s0 = il_group(blockIdx.x) * G::S;
+    const int64_t s0 = il_group<G::P>(blockIdx.x) * G::S;
     if (s0 >= N) return;  // padding workgroup: the whole block leaves
     const T* P[4] = {A, B, C, D};
 
@@ -203,7 +213,8 @@ hipError_t launch_interleaved_reg(const InterleavedArgs<T>& g, bool* taken)
     *taken = false;
     if (g.n <= kWave * kIlM / 2 || g.n > kWave * kIlM || g.N <= 0) return hipSuccess;
     const int64_t groups = (g.N + G::S - 1) / G::S;
-    const int64_t grid = (groups + 15) / 16 * 16;  // il_group is one-to-one on these
+    constexpr int64_t kMul = 8 * G::P;  // il_group is one-to-one on multiples of this
+    const int64_t grid = (groups + kMul - 1) / kMul * kMul;
     if (grid > 0x7fffffffLL) return hipSuccess;
     const int vec = (g.stride % G::EV == 0 && il_aligned(g.a) && il_aligned(g.b) &&
                      il_aligned(g.c) && il_aligned(g.d) && il_aligned(g.x))
