@@ -194,7 +194,9 @@ def bench_interleaved(args):
            "kernel_ms": {"avg": avg, "min": min(kms)},
            "roofline": {"bound": "hbm", "achieved": alg / (avg * 1e-3) / 1e9,
                         "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                        "frac": alg / (avg * 1e-3) / 1e9 / PEAK_HBM_GBS, "traffic": None,
+                        "frac": alg / (avg * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                        "traffic": traffic_from_profiles(
+                            f"interleaved {args.dtype} {N} systems x {n} unknowns"),
                         "note": ("512 < n <= 1024: register CR-PCR on systems gathered "
                                  "through LDS, 5 x T bytes per unknown moved"
                                  if 512 < n <= 1024 else
@@ -445,7 +447,8 @@ def bench_adi(args):
            "roofline": {"bound": "hbm", "achieved": per[n]["hbm_gbs_64B_per_point"],
                         "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "frac": per[n]["hbm_gbs_64B_per_point"] / PEAK_HBM_GBS,
-                        "traffic": None,
+                        "traffic": traffic_from_profiles(
+                            "adi f64 4096^2 resident grid, Peaceman-Rachford step"),
                         "note": "64 B per point and step: 4 passes (row rhs, row solve, "
                                 "column rhs + transpose, column solve) of 16 B each; one "
                                 "transpose back per call is not credited"}}
