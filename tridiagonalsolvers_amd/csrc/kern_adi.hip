@@ -31,13 +31,33 @@ namespace {
 constexpr int kTile = 64;
 }
 
+typedef double adi_d2 __attribute__((ext_vector_type(2)));
+
 // D = u + r * (u[i-1][j] - 2 u[i][j] + u[i+1][j]), zero outside the grid.
+// Two consecutive points per thread with 16-byte accesses when n is even
+// (vec; a pair then never straddles two rows), one otherwise.
 __global__ __launch_bounds__(256) void adi_rhs_rows_kernel(const double* __restrict__ U,
                                                            double* __restrict__ D, int64_t n,
-                                                           double r)
+                                                           double r, int vec)
 {
     const int64_t total = n * n;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    if (vec) {
+        for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; 2 * p < total;
+             p += stride) {
+            const int64_t g = 2 * p;
+            const int64_t i = g / n;
+            const adi_d2 u = *reinterpret_cast<const adi_d2*>(U + g);
+            const adi_d2 up = i > 0 ? *reinterpret_cast<const adi_d2*>(U + g - n) : adi_d2{0, 0};
+            const adi_d2 dn =
+                i < n - 1 ? *reinterpret_cast<const adi_d2*>(U + g + n) : adi_d2{0, 0};
+            adi_d2 o;
+            o.x = u.x + r * (up.x - 2.0 * u.x + dn.x);
+            o.y = u.y + r * (up.y - 2.0 * u.y + dn.y);
+            *reinterpret_cast<adi_d2*>(D + g) = o;
+        }
+        return;
+    }
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
         const int64_t i = g / n;
         const double u = U[g];
@@ -46,8 +66,6 @@ __global__ __launch_bounds__(256) void adi_rhs_rows_kernel(const double* __restr
         D[g] = u + r * (up - 2.0 * u + dn);
     }
 }
-
-typedef double adi_d2 __attribute__((ext_vector_type(2)));
 
 // Tile (ty, tx) of block b of a one-dimensional grid of tiles x tiles blocks.
 // Blocks are dealt to the 8 XCDs round-robin; when the tile count is a
@@ -199,7 +217,7 @@ hipError_t launch_adi_steps(const AdiArgs& g, int steps)
         // T2 (transposing it back on the way), saving a transpose and a pass.
         if (k == 0)
             hipLaunchKernelGGL(adi_rhs_rows_kernel, dim3((unsigned)blocks), dim3(256), 0,
-                               g.stream, g.u, g.t, n, g.r);
+                               g.stream, g.u, g.t, n, g.r, tvec);
         else
             hipLaunchKernelGGL(adi_rhs_cols_t_kernel, tgrid, dim3(256), 0, g.stream, g.t2, g.t,
                                n, g.r, tvec);
