@@ -353,7 +353,9 @@ int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* 
 typedef struct trid_adi_s* trid_adi;
 
 /* u_device: an existing n*n device grid to step in place, or NULL to
- * allocate one (trid_adi_grid returns it). 2 <= n <= 16384. */
+ * allocate one (trid_adi_grid returns it). 2 <= n <= 8192 (the FP64
+ * register kernels' range): TRID_ENOTSUP for 8192 < n <= 16384, TRID_EINVAL
+ * beyond. */
 int trid_adi_create(trid_adi* adi, int64_t n, double* u_device);
 int trid_adi_destroy(trid_adi adi);
 double* trid_adi_grid(trid_adi adi);
