@@ -118,3 +118,31 @@ def test_host_dropins_past_the_register_sizes(oracle, trid, dtype):
         x = np.full_like(a, np.nan)
         getattr(trid, name)(a, b, c, d, x, n, N, *args)
         check(oracle, x, a, b, c, d, n, dtype)
+
+
+def test_separator_form_replays_in_a_hip_graph(oracle, trid):
+    """Once a first call has sized the handle's workspace, the separator
+    form's launches (no allocation, no memset) can be captured: every replay
+    equals the eager result bit for bit, with eager work in between."""
+    import torch
+    n, N = 50000, 4
+    a, b, c, d = oracle.generate("dominant", 61, n, N, np.float32)
+    dev = [torch.from_numpy(v).cuda() for v in (a, b, c, d)]
+    h = trid.Handle()
+    x_eager = torch.empty_like(dev[0])
+    h.solve("crpcr", *dev, x_eager, n, N)            # sizes the workspace
+    torch.cuda.synchronize()
+    x = torch.full_like(dev[0], float("nan"))
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        h.solve("crpcr", *dev, x, n, N, stream=s)
+    for rep in range(3):
+        x.fill_(float("nan"))
+        junk = torch.randn(1 << 20, device="cuda").sum()   # eager work between replays
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(x, x_eager), rep
+        assert bool(torch.isfinite(junk))
+    check(oracle, x.cpu().numpy(), a, b, c, d, n, np.float32)
+    h.close()
