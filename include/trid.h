@@ -347,15 +347,15 @@ int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* 
  * Dirichlet boundary) stays on the device; each step with parameter r solves
  * (I - r dxx) u* = (I + r dyy) u   (n row systems tridiag(-r, 1+2r, -r))
  * (I - r dyy) u' = (I + r dxx) u*  (n column systems, stride n)
- * with the constant-coefficient register CR-PCR kernel for the rows, and for
- * the columns on the transposed grid (tiled transposes through LDS).
+ * with the LU factors of tridiag(-r, 1+2r, -r), computed once per call and
+ * resident on the device, applied to the rows and, on the transposed grid
+ * (tiled transposes through LDS), to the columns.
  */
 typedef struct trid_adi_s* trid_adi;
 
 /* u_device: an existing n*n device grid to step in place, or NULL to
- * allocate one (trid_adi_grid returns it). 2 <= n <= 8192 (the FP64
- * register kernels' range): TRID_ENOTSUP for 8192 < n <= 16384, TRID_EINVAL
- * beyond. */
+ * allocate one (trid_adi_grid returns it). 2 <= n <= 16384 (TRID_EINVAL
+ * otherwise). */
 int trid_adi_create(trid_adi* adi, int64_t n, double* u_device);
 int trid_adi_destroy(trid_adi adi);
 double* trid_adi_grid(trid_adi adi);

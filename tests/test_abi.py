@@ -212,11 +212,10 @@ def test_log_uniform_sizes_are_seeded_and_in_range():
 
 
 def test_adi_grid_sizes_are_checked_without_a_device(trid):
-    # the FP64 register sweep ends at 8192: larger grids are refused before
-    # anything is allocated
+    # grids of 2 .. 16384 points a side; others are refused before anything
+    # is allocated
     import ctypes
     from tridiagonalsolvers_amd._lib import lib
     h = ctypes.c_void_p()
-    assert lib().trid_adi_create(ctypes.byref(h), 10000, None) == 4      # TRID_ENOTSUP
-    assert lib().trid_adi_create(ctypes.byref(h), 20000, None) == 1      # TRID_EINVAL
+    assert lib().trid_adi_create(ctypes.byref(h), 16385, None) == 1      # TRID_EINVAL
     assert lib().trid_adi_create(ctypes.byref(h), 1, None) == 1

@@ -1,6 +1,8 @@
 """GPU tests of the resident ADI grid (Peaceman-Rachford steps of the 2D heat
 equation): sine modes against their exact amplification factor, and random
-grids against the oracle's FP64 ADI step (tolerance 1e-12 relative)."""
+grids against the oracle's FP64 ADI step (tolerance 1e-12 relative), at
+sizes from 37 to 16384 points a side (one wave of the solve per 1,024 rows;
+odd sizes take the element-wise path)."""
 import numpy as np
 import pytest
 
@@ -20,11 +22,18 @@ def _run(n, u0, rs):
     return out
 
 
-@pytest.mark.parametrize("n", [64, 100, 1024, 2048])
+@pytest.mark.parametrize("n", [64, 100, 1024, 2048, 10000, 16384])
 def test_adi_sine_modes(oracle, trid, n):
-    i = np.arange(1, n + 1)
+    i = np.arange(1, n + 1, dtype=np.int64)
+
+    def mode(k):
+        # sin(k pi i / (n+1)) with k i reduced modulo the period 2 (n+1) in
+        # integers first: at n = 16384, k = n/2 the unreduced argument is
+        # ~2.6e4 and its rounding alone (~3e-12) would exceed the bound
+        return np.sin(np.pi * ((k * i) % (2 * (n + 1))) / (n + 1))
+
     for p, q, r in ((1, 1, 0.1), (5, 3, 1.7), (n // 2, 7, 10.0)):
-        u0 = np.outer(np.sin(p * np.pi * i / (n + 1)), np.sin(q * np.pi * i / (n + 1)))
+        u0 = np.outer(mode(p), mode(q))
         u1 = _run(n, u0, [r])
         g = oracle.adi_mode_factor(n, p, q, r)
         err = np.max(np.abs(u1 - g * u0))
@@ -32,7 +41,7 @@ def test_adi_sine_modes(oracle, trid, n):
         assert err <= 1e-12
 
 
-@pytest.mark.parametrize("n", [37, 256, 1024])
+@pytest.mark.parametrize("n", [37, 256, 1024, 4097])
 def test_adi_random_grid_matches_oracle(oracle, trid, n):
     rng = np.random.default_rng(n)
     u0 = rng.uniform(-1, 1, (n, n))

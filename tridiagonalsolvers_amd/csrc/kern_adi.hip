@@ -7,8 +7,10 @@
 //   (I - r dxx) u* = (I + r dyy) u        row sweep:    n systems, contiguous
 //   (I - r dyy) u' = (I + r dxx) u*       column sweep: n systems, stride n
 //
-// Both sweeps run on the register CR-PCR kernels in their constant-
-// coefficient mode (only d is read, x written: 16 bytes per unknown in FP64).
+// Both sweeps solve systems with the same operator, so its LU factors are
+// resident for the whole call (adi_factor_kernel, three values per row) and
+// every system is solved by the two substitutions of the factored operator
+// (adi_lu_kernel: d read, x written, 16 bytes per unknown in FP64).
 // * row sweep: the right-hand side is one elementwise pass (adi_rhs_rows),
 //   then the row solve;
 // * column sweep: the right-hand side is formed and transposed in one tiled
@@ -22,6 +24,7 @@
 //   stencil along the transposed grid's rows is the stencil along u's
 //   columns); only the last step transposes u' back into u: 4 passes a step
 //   instead of 5.
+#include "reg_io.hpp"
 #include "trid_device.hpp"
 #include "trid_launch.hpp"
 
@@ -29,6 +32,174 @@ namespace trid {
 
 namespace {
 constexpr int kTile = 64;
+constexpr int kAdiM = 16;                  // unknowns per lane in the solve
+constexpr int kAdiRows = kWave * kAdiM;    // rows per wave
+}
+
+int64_t adi_npad(int64_t n) { return (n + kAdiRows - 1) / kAdiRows * kAdiRows; }
+
+// LU factors of tridiag(-r, 1+2r, -r) of order n, padded to npad rows. With
+// the pivots u_0 = 1 + 2r, u_i = 1 + 2r - r^2 / u_{i-1} and q_i = 1 / u_i,
+// the two substitutions are
+//   y_i = d_i + r q_{i-1} y_{i-1}     (nothing before the first row)
+//   x_i = q_i (y_i + r x_{i+1})       (nothing after the last row),
+// so the table holds q only. With l1 > l2 the roots of l^2 - (1+2r) l + r^2
+// (l1 l2 = r^2) and p = l2 / l1 < 1, the pivots have the closed form
+// u_i = l1 (1 - p^(i+2)) / (1 - p^(i+1)) (induction on the recurrence),
+// evaluated here per row with expm1 so that every row is independent.
+// Rows past n: q = 0. Then x = 0 there whatever y is, so x_{n-1} =
+// q_{n-1} y_{n-1} as it must be, and the kernel needs no mask (with q = 0
+// taken before the system's first row as well).
+__global__ __launch_bounds__(256) void adi_factor_kernel(double* __restrict__ q, int64_t n,
+                                                         int64_t npad, double r)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npad) return;
+    const double l1 = 0.5 * (1.0 + 2.0 * r + sqrt(1.0 + 4.0 * r));
+    const double lg = 2.0 * log(r / l1);  // log(l2 / l1)
+    q[i] = i < n ? 1.0 / (l1 * (expm1((double)(i + 2) * lg) / expm1((double)(i + 1) * lg))) : 0.0;
+}
+
+// One system (a row of the grid in D, n unknowns) per workgroup of W waves,
+// lane l of wave w holding rows 16 (64 w + l) .. +15; d and the factors q come
+// in through the wave-coalesced staged loads. Both substitutions are linear
+// recurrences: each lane composes its 16 rows into one affine map (y_out =
+// A y_in + B forward, x_first = G x_in + H backward), a Kogge-Stone scan
+// composes the maps across the lanes (DPP at distance 1, ds_bpermute
+// beyond), the wave maps cross the waves through LDS, and each lane then
+// runs its rows from its own input. About 200 FP64 operations per lane and
+// no division, against about 500 for the CR-PCR solve of the same system.
+template <int MAXT>
+__global__ __launch_bounds__(MAXT) void adi_lu_kernel(const double* __restrict__ D,
+                                                      double* __restrict__ X, int64_t n,
+                                                      int64_t npad, const double* __restrict__ Q,
+                                                      double r, int vec)
+{
+    const int W = blockDim.x / kWave;
+    const int w = threadIdx.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t base = (int64_t)blockIdx.x * n;
+    const int64_t row0 = (int64_t)(w * kWave + lane) * kAdiM;
+    const int64_t wave_base = base + (int64_t)w * kAdiRows;
+    extern __shared__ __align__(16) char trid_smem[];
+    char* lds = trid_smem + w * Stage<double, kAdiM>::kWaveBytes;
+    __shared__ double wmap[2][16][2];  // per wave: forward (A, B), backward (G, H)
+
+    // q of the row before the lane's first (0 for the system's first row)
+    const double q_prev_wave = (lane == 0 && row0 > 0) ? Q[row0 - 1] : 0.0;
+    double y[kAdiM], q[kAdiM];
+    {
+        Eq<double> e[kAdiM];
+        load_staged<double, kAdiM, false>(Q, Q, Q, Q, (int64_t)w * kAdiRows, npad, lane, lds, e,
+                                          ConstCoef<double>{true, 0.0, 0.0, 0.0});
+#pragma unroll
+        for (int k = 0; k < kAdiM; ++k) q[k] = e[k].d;
+    }
+    if (vec) {
+        Eq<double> e[kAdiM];
+        load_staged<double, kAdiM, true>(D, D, D, D, wave_base, base + n, lane, lds, e,
+                                         ConstCoef<double>{true, 0.0, 0.0, 0.0});
+#pragma unroll
+        for (int k = 0; k < kAdiM; ++k) y[k] = e[k].d;
+    } else {
+#pragma unroll
+        for (int k = 0; k < kAdiM; ++k) y[k] = row0 + k < n ? D[base + row0 + k] : 0.0;
+    }
+    // (the DPP move runs on every lane: inside a branch that lane 0 skips,
+    // lane 1 would read the inactive lane 0 and get 0)
+    const double q_left = lane_step<-1>(q[kAdiM - 1]);
+    const double q_prev = lane == 0 ? q_prev_wave : q_left;
+
+    // ---- forward: y_i = d_i + r q_{i-1} y_{i-1} ------------------------------
+    {
+        double A = 1.0, B = 0.0;
+#pragma unroll
+        for (int k = 0; k < kAdiM; ++k) {
+            const double f = r * (k == 0 ? q_prev : q[k > 0 ? k - 1 : 0]);
+            B = fma(f, B, y[k]);
+            A *= f;
+        }
+        {  // inclusive scan over the lanes: lane l maps the wave's input to the end of its rows
+            const double Ap = lane_step<-1>(A), Bp = lane_step<-1>(B);
+            if (lane >= 1) {
+                B = fma(A, Bp, B);
+                A *= Ap;
+            }
+        }
+#pragma unroll
+        for (int sh = 2; sh < kWave; sh <<= 1) {
+            const double Ap = lane_read(A, (lane - sh) & (kWave - 1));
+            const double Bp = lane_read(B, (lane - sh) & (kWave - 1));
+            if (lane >= sh) {
+                B = fma(A, Bp, B);
+                A *= Ap;
+            }
+        }
+        if (lane == kWave - 1) {
+            wmap[0][w][0] = A;
+            wmap[0][w][1] = B;
+        }
+        __syncthreads();
+        double yin = 0.0;
+        for (int v = 0; v < w; ++v) yin = fma(wmap[0][v][0], yin, wmap[0][v][1]);
+        const double Ap = lane_step<-1>(A), Bp = lane_step<-1>(B);
+        double t = lane == 0 ? yin : fma(Ap, yin, Bp);
+        double qp = q_prev;
+#pragma unroll
+        for (int k = 0; k < kAdiM; ++k) {
+            t = fma(r, qp * t, y[k]);
+            y[k] = t;
+            qp = q[k];
+        }
+    }
+
+    // ---- backward: x_i = q_i (y_i + r x_{i+1}) -------------------------------
+    {
+        double G = 1.0, H = 0.0;
+#pragma unroll
+        for (int k = kAdiM - 1; k >= 0; --k) {
+            H = q[k] * fma(r, H, y[k]);
+            G *= r * q[k];
+        }
+        {  // suffix scan: lane l maps the wave's right input to its first row
+            const double Gn = lane_step<1>(G), Hn = lane_step<1>(H);
+            if (lane + 1 < kWave) {
+                H = fma(G, Hn, H);
+                G *= Gn;
+            }
+        }
+#pragma unroll
+        for (int sh = 2; sh < kWave; sh <<= 1) {
+            const double Gn = lane_read(G, (lane + sh) & (kWave - 1));
+            const double Hn = lane_read(H, (lane + sh) & (kWave - 1));
+            if (lane + sh < kWave) {
+                H = fma(G, Hn, H);
+                G *= Gn;
+            }
+        }
+        if (lane == 0) {
+            wmap[1][w][0] = G;
+            wmap[1][w][1] = H;
+        }
+        __syncthreads();
+        double xin = 0.0;
+        for (int v = W - 1; v > w; --v) xin = fma(wmap[1][v][0], xin, wmap[1][v][1]);
+        const double Gn = lane_step<1>(G), Hn = lane_step<1>(H);
+        double x = lane == kWave - 1 ? xin : fma(Gn, xin, Hn);
+#pragma unroll
+        for (int k = kAdiM - 1; k >= 0; --k) {
+            x = q[k] * fma(r, x, y[k]);
+            y[k] = x;
+        }
+    }
+
+    if (vec) {
+        store_staged<double, kAdiM, true>(X, wave_base, base + n, lane, lds, y);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kAdiM; ++k)
+            if (row0 + k < n) X[base + row0 + k] = y[k];
+    }
 }
 
 typedef double adi_d2 __attribute__((ext_vector_type(2)));
@@ -191,24 +362,30 @@ hipError_t launch_adi_steps(const AdiArgs& g, int steps)
                          ? 1
                          : 0;
     hipError_t e;
-    RegArgs<double> ra{};
-    ra.a = ra.b = ra.c = nullptr;
-    ra.n = n;
-    ra.N = n;
-    ra.M = g.M;
-    ra.L = g.L;
-    ra.rest = g.rest;
-    ra.vec = g.vec;
-    ra.io = 3;
-    ra.stream = g.stream;
-    ra.cc = 1;
-    ra.a0 = -g.r;
-    ra.b0 = 1.0 + 2.0 * g.r;
-    ra.c0 = -g.r;
+    // the operator's factors, once for the call
+    const int64_t npad = adi_npad(n);
+    hipLaunchKernelGGL(adi_factor_kernel, dim3((unsigned)((npad + 255) / 256)), dim3(256), 0,
+                       g.stream, g.fac, n, npad, g.r);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const int W = (int)(npad / kAdiRows);
+    const size_t lu_lds = (size_t)W * Stage<double, kAdiM>::kWaveBytes;
+    if (W > 16) return hipErrorInvalidConfiguration;
+    // up to 8 waves: 512-thread bound (registers for 2-3 workgroups per CU);
+    // 9..16 waves: the 1024-thread bound
+    const bool wide = W > 8;
+    if (wide && lu_lds > 64 * 1024) {
+        e = hipFuncSetAttribute((const void*)adi_lu_kernel<1024>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lu_lds);
+        if (e != hipSuccess) return e;
+    }
     auto solve = [&](const double* d, double* x) -> hipError_t {
-        ra.d = d;
-        ra.x = x;
-        return ra.L <= kWave ? launch_reg_crpcr<double>(ra) : launch_reg_mw<double>(ra);
+        if (wide)
+            hipLaunchKernelGGL((adi_lu_kernel<1024>), dim3((unsigned)n), dim3(W * kWave), lu_lds,
+                               g.stream, d, x, n, npad, g.fac, g.r, tvec);
+        else
+            hipLaunchKernelGGL((adi_lu_kernel<512>), dim3((unsigned)n), dim3(W * kWave), lu_lds,
+                               g.stream, d, x, n, npad, g.fac, g.r, tvec);
+        return hipGetLastError();
     };
     const dim3 tgrid((unsigned)(tiles * tiles));
     for (int k = 0; k < steps; ++k) {

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256) void reg_crpcr_kernel(const T* __restrict__ A,
                                                         const T* __restrict__ D,
                                                         T* __restrict__ X, int64_t n,
                                                         int64_t N, int L, int logL, int rest,
-                                                        int vec, ConstCoef<T> cc, int64_t bs)
+                                                        int vec, int64_t bs)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int ll = lane & (L - 1);
@@ -64,13 +64,13 @@ __global__ __launch_bounds__(256) void reg_crpcr_kernel(const T* __restrict__ A,
     Eq<T> e[M];
     if constexpr (M % E == 0 && IO >= 2) {
         if (vec) {
-            load_staged<T, M, IO == 3>(A, B, C, D, wave_base, total, lane, lds, e, cc, seg_log2,
-                                       bs);
+            load_staged<T, M, IO == 3>(A, B, C, D, wave_base, total, lane, lds, e,
+                                       kNoConstCoef<T>, seg_log2, bs);
             if (!live) set_identity<T, M>(e);
         }
     } else if constexpr (M % E == 0) {
         if (vec) {
-            if (live) load_direct<T, M, IO == 0>(A, B, C, D, base + j0, e, cc);
+            if (live) load_direct<T, M, IO == 0>(A, B, C, D, base + j0, e, kNoConstCoef<T>);
             else set_identity<T, M>(e);
         }
     }
@@ -79,8 +79,7 @@ __global__ __launch_bounds__(256) void reg_crpcr_kernel(const T* __restrict__ A,
         for (int s = 0; s < M; ++s) {
             const int64_t j = j0 + s;
             if (live && j < n)
-                e[s] = cc.on ? Eq<T>{cc.a, cc.b, cc.c, D[base + j]}
-                             : Eq<T>{A[base + j], B[base + j], C[base + j], D[base + j]};
+                e[s] = Eq<T>{A[base + j], B[base + j], C[base + j], D[base + j]};
             else e[s] = identity_eq<T>();
         }
     }
@@ -124,7 +123,7 @@ static hipError_t launch_reg_io(const RegArgs<T>& g, int R, dim3 grid, dim3 bloc
         if constexpr (RR <= M) {                                                             \
             hipLaunchKernelGGL((reg_crpcr_kernel<T, M, RR, IO>), grid, block, lds, g.stream, \
                                g.a, g.b, g.c, g.d, g.x, g.n, g.N, g.L, logL, g.rest, g.vec,    \
-                               ConstCoef<T>{g.cc != 0, g.a0, g.b0, g.c0}, g.bs ? g.bs : g.n);   \
+                               g.bs ? g.bs : g.n);                                          \
             return hipGetLastError();                                                        \
         }                                                                                    \
         break;

@@ -32,7 +32,7 @@ template <typename T, int M, int IO>
 __device__ __forceinline__ void reg_mw_body(const T* __restrict__ A, const T* __restrict__ B,
                                             const T* __restrict__ C, const T* __restrict__ D,
                                             T* __restrict__ X, int64_t n, int rest, int vec,
-                                            int64_t N, ConstCoef<T> cc, int64_t bs)
+                                            int64_t N, int64_t bs)
 {
     constexpr int E = 16 / (int)sizeof(T);
     const int W = blockDim.x / kWave;
@@ -54,12 +54,12 @@ __device__ __forceinline__ void reg_mw_body(const T* __restrict__ A, const T* __
     if constexpr (M % E == 0 && IO >= 2) {
         if (vec) {
             load_staged<T, M, IO == 3>(A, B, C, D, base + (int64_t)w * kWave * M, total, lane,
-                                       lds_stage, e, cc);
+                                       lds_stage, e, kNoConstCoef<T>);
             loaded = true;
         }
     } else if constexpr (M % E == 0) {
         if (vec) {
-            load_direct<T, M, IO == 0>(A, B, C, D, base + j0, e, cc);
+            load_direct<T, M, IO == 0>(A, B, C, D, base + j0, e, kNoConstCoef<T>);
             loaded = true;
         }
     }
@@ -68,8 +68,7 @@ __device__ __forceinline__ void reg_mw_body(const T* __restrict__ A, const T* __
         for (int s = 0; s < M; ++s) {
             const int64_t j = j0 + s;
             if (j < n)
-                e[s] = cc.on ? Eq<T>{cc.a, cc.b, cc.c, D[base + j]}
-                             : Eq<T>{A[base + j], B[base + j], C[base + j], D[base + j]};
+                e[s] = Eq<T>{A[base + j], B[base + j], C[base + j], D[base + j]};
             else e[s] = identity_eq<T>();
         }
     }
@@ -109,10 +108,9 @@ __global__ __launch_bounds__(1024) void reg_mw_kernel(const T* __restrict__ A,
                                                       const T* __restrict__ C,
                                                       const T* __restrict__ D,
                                                       T* __restrict__ X, int64_t n, int rest,
-                                                      int vec, int64_t N, ConstCoef<T> cc,
-                                                      int64_t bs)
+                                                      int vec, int64_t N, int64_t bs)
 {
-    reg_mw_body<T, M, IO>(A, B, C, D, X, n, rest, vec, N, cc, bs);
+    reg_mw_body<T, M, IO>(A, B, C, D, X, n, rest, vec, N, bs);
 }
 
 // ... or up to 8 waves with twice the registers: FP64 at 16 unknowns per lane.
@@ -122,10 +120,9 @@ __global__ __launch_bounds__(512) void reg_mw512_kernel(const T* __restrict__ A,
                                                         const T* __restrict__ C,
                                                         const T* __restrict__ D,
                                                         T* __restrict__ X, int64_t n, int rest,
-                                                        int vec, int64_t N, ConstCoef<T> cc,
-                                                        int64_t bs)
+                                                        int vec, int64_t N, int64_t bs)
 {
-    reg_mw_body<T, M, IO>(A, B, C, D, X, n, rest, vec, N, cc, bs);
+    reg_mw_body<T, M, IO>(A, B, C, D, X, n, rest, vec, N, bs);
 }
 
 template <typename T, int M>
@@ -153,14 +150,13 @@ static hipError_t launch_mw_io(const RegArgs<T>& g)
             hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    const ConstCoef<T> cc{g.cc != 0, g.a0, g.b0, g.c0};
     if constexpr (wide)
         hipLaunchKernelGGL((reg_mw512_kernel<T, M, IO>), dim3((unsigned)g.N), dim3(kWave * W),
-                           lds, g.stream, g.a, g.b, g.c, g.d, g.x, g.n, g.rest, g.vec, g.N, cc,
+                           lds, g.stream, g.a, g.b, g.c, g.d, g.x, g.n, g.rest, g.vec, g.N,
                            g.bs ? g.bs : g.n);
     else
         hipLaunchKernelGGL((reg_mw_kernel<T, M, IO>), dim3((unsigned)g.N), dim3(kWave * W), lds,
-                           g.stream, g.a, g.b, g.c, g.d, g.x, g.n, g.rest, g.vec, g.N, cc,
+                           g.stream, g.a, g.b, g.c, g.d, g.x, g.n, g.rest, g.vec, g.N,
                            g.bs ? g.bs : g.n);
     return hipGetLastError();
 }

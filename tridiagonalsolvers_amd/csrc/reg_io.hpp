@@ -41,6 +41,8 @@ struct ConstCoef {
     bool on;
     T a, b, c;
 };
+template <typename T>
+constexpr ConstCoef<T> kNoConstCoef{false, T(0), T(0), T(0)};
 
 template <typename T, int M, bool NT>
 static __device__ __forceinline__ void load_direct(const T* __restrict__ A, const T* __restrict__ B,

@@ -123,7 +123,9 @@ __device__ __forceinline__ T lane_read_or_zero(T v, int src, bool valid)
 // no trip through the LDS crossbar that ds_bpermute takes. The lane shifted in
 // at the end of the wave reads 0; callers mask the lanes whose neighbour lies
 // outside their system, as they do for ds_bpermute reads. D is a template
-// argument so that the choice of instruction is made at compile time.
+// argument so that the choice of instruction is made at compile time. Call
+// it where every lane of the wave is active (not in a branch that some lanes
+// skip): a lane whose source lane is inactive reads 0.
 template <int D>
 __device__ __forceinline__ int lane_step_i32(int v)
 {

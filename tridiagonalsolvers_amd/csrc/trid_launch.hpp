@@ -27,8 +27,6 @@ struct RegArgs {
     int io;          // dense-path I/O mode: 0 per-lane non-temporal, 1 per-lane,
                      // 2 LDS-staged coalesced, 3 LDS-staged non-temporal
     hipStream_t stream;
-    int cc = 0;      // constant coefficients a0, b0, c0 on every row (d only)
-    T a0 = 0, b0 = 1, c0 = 0;
     int64_t bs = 0;  // elements from one system's first row to the next's (0: n);
                      // with bs != n the wave-staged I/O needs L >= 64
 };
@@ -231,15 +229,17 @@ hipError_t launch_generate(int kind, uint64_t seed, T* a, T* b, T* c, T* d, int6
 
 // Peaceman-Rachford ADI steps on an n x n FP64 grid (kern_adi.hip).
 struct AdiArgs {
-    double* u;      // grid, row-major, n*n (input and output)
+    double* u;      // grid, row-major, n*n (input and output), 2 <= n <= 16384
     double* t;      // scratch grid, n*n
     double* t2;     // second scratch grid, n*n
+    double* fac;    // the operator's LU factors (pivot reciprocals), adi_npad(n) doubles
     int64_t n;
     double r;
-    int M, L, rest, vec;  // register-kernel plan of the row sweep (n systems of n)
     hipStream_t stream;
 };
 
+// Rows of the factor table (n rounded up to a wave's 1,024 rows).
+int64_t adi_npad(int64_t n);
 // `steps` Peaceman-Rachford steps (the grid in U is current when they end).
 hipError_t launch_adi_steps(const AdiArgs& g, int steps);
 

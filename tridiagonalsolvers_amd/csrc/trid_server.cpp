@@ -79,10 +79,9 @@ struct Req {
 
 struct Adi {
     int64_t n = 0;
-    char* dev = nullptr;  // u | t | t2 (grids), inside the pool
+    char* dev = nullptr;  // u | t | t2 (grids) | LU factors, inside the pool
     size_t pool_off = 0;
     size_t bytes = 0;
-    trid::Plan plan{};
     int refs = 0;         // queued or running requests
     bool running = false;
 };
@@ -301,12 +300,9 @@ int trid_server_s::launch(Slot& s)
         g.u = reinterpret_cast<double*>(a.dev);
         g.t = reinterpret_cast<double*>(a.dev + grid);
         g.t2 = reinterpret_cast<double*>(a.dev + 2 * grid);
+        g.fac = reinterpret_cast<double*>(a.dev + align256(3 * grid));
         g.n = a.n;
         g.r = q->r.adi_r;
-        g.M = a.plan.M;
-        g.L = a.plan.L;
-        g.rest = a.plan.rest;
-        g.vec = a.plan.dense;
         g.stream = s.stream;
         if (trid::launch_adi_steps(g, q->r.adi_steps) != hipSuccess) return TRID_EHIP;
         return hipEventRecord(s.done, s.stream) == hipSuccess ? TRID_OK : TRID_EHIP;
@@ -764,12 +760,8 @@ int trid_server_create_adi(trid_server s, int64_t n, const double* u0, int32_t* 
     if (!s || !u0 || !id || n < 2 || n > trid::kRaggedMaxSize) return TRID_EINVAL;
     Adi a;
     a.n = n;
-    trid_opts o{};
-    int st = trid::make_plan(TRID_CRPCR, n, n, 8, o, &a.plan);
-    if (st != TRID_OK) return st;
-    if (a.plan.family != trid::Family::Reg) return TRID_ENOTSUP;
     const size_t grid = (size_t)n * n * sizeof(double);
-    a.bytes = align256(3 * grid);
+    a.bytes = align256(3 * grid) + align256((size_t)trid::adi_npad(n) * sizeof(double));
     std::lock_guard<std::mutex> lk(s->mu);
     if (a.bytes > s->budget_free()) return TRID_ENOMEM;
     size_t off = 0;
