@@ -66,14 +66,19 @@ def test_interleaved_in_place(oracle, trid, dtype):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_interleaved_register_form(oracle, trid, dtype):
-    """512 < n <= 1024 runs one wave per system on the register CR-PCR core
-    (the systems gathered through LDS): partial groups of systems, strides
-    that break 16-byte vectors, an odd number of rows, unaligned arrays, x = d
-    in place, and the whole 65,536 x 1,024 batch."""
+    """256 < n <= 1024 (FP32) / 128 < n <= 1024 (FP64) runs one wave per
+    system on the register CR-PCR core (16, 8 or 4 unknowns per lane; the
+    systems gathered through LDS), shorter FP32 systems the lane-per-system
+    elimination; both are checked here: partial
+    groups of systems, strides that break 16-byte vectors, an odd number of
+    rows, unaligned arrays, x = d in place, and the whole 65,536 x 1,024
+    batch."""
     import torch
     h = trid.Handle()
     cases = ((1024, 17, 20, 0), (513, 100, 101, 0), (1000, 3000, 3100, 0), (1024, 4096, 4096, 1),
-             (777, 250, 256, 0), (1024, 65536, 65536, 0))
+             (777, 250, 256, 0), (1024, 65536, 65536, 0),
+             (512, 300, 300, 0), (300, 1000, 1003, 0), (257, 64, 64, 1), (256, 4096, 4096, 0),
+             (200, 17, 20, 0), (129, 100, 101, 0))
     for n, N, stride, offset in cases:
         a, b, c, d = oracle.generate("dominant", 40 + n, n, N, dtype)
         dev = []

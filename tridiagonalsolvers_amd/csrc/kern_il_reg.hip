@@ -7,19 +7,19 @@
 // factors in memory: 9 elements of traffic per unknown. Here a workgroup of
 // S waves takes S consecutive systems (S = 16 FP32, 8 FP64) and gives each
 // wave one system in the layout of the register core (lane l holds unknowns
-// [16 l, 16 l + 16)), so the solve moves only the 5 elements per unknown it
-// must (read a, b, c, d; write x). The gather is the transposition: a row of
-// the S systems is 64 contiguous bytes, read as 16-byte vectors, one per
-// thread and array per chunk of 256 (FP32) / 128 (FP64) rows; the chunk is scattered
-// into LDS by system and lane group, and the 16 (8) lanes of every wave that
-// own those rows read their 16 consecutive rows back as 16-byte vectors. The
-// solution leaves the same way. LDS layout per array: [system][lane group]
-// [16 rows + 16 bytes of padding], plus 16 bytes between systems: the per-lane
-// vector reads fall on distinct banks and the scattering writes of a
-// half-wave conflict at most 2-way.
+// [MI l, MI l + MI), MI = 16, 8 or 4 unknowns per lane for n up to 1024, 512
+// or 256), so the solve moves only the 5 elements per unknown it must (read
+// a, b, c, d; write x). The gather is the transposition: a row of the S
+// systems is 64 contiguous bytes, read as 16-byte vectors, one per thread and
+// array per chunk of rows; the chunk is scattered into LDS by system and lane
+// group, and the lanes of every wave that own those rows read their MI
+// consecutive rows back as 16-byte vectors. The solution leaves the same way.
+// LDS layout per array: [system][lane group][MI rows + 16 bytes of padding],
+// plus 16 bytes between systems: the per-lane vector reads fall on distinct
+// banks and the scattering writes of a half-wave conflict at most 2-way.
 //
 // Rows past n are identity rows and systems past N are never stored, so any
-// 512 < n <= 1024 and any N work; a[first] and c[last] are ignored. x may be
+// n in the range and any N work; a[first] and c[last] are ignored. x may be
 // d: a workgroup reads all of its systems' rows before it writes any x.
 #include "reg_core.hpp"
 #include "reg_io.hpp"
@@ -29,8 +29,6 @@
 namespace trid {
 
 namespace {
-
-constexpr int kIlM = 16;  // unknowns per lane: n_pad = 1024
 
 // S systems (waves) per workgroup (TRID_IL_SYSTEMS_F32 in FP32, 8 in FP64:
 // the FP64 core needs about 210 VGPRs). A row of the workgroup's systems is
@@ -42,15 +40,15 @@ constexpr int kIlM = 16;  // unknowns per lane: n_pad = 1024
 #ifndef TRID_IL_SYSTEMS_F32
 #define TRID_IL_SYSTEMS_F32 16
 #endif
-template <typename T>
+template <typename T, int MI>
 struct IlGeom {
     static constexpr int S = sizeof(T) == 4 ? TRID_IL_SYSTEMS_F32 : 8;
     static constexpr int EV = 16 / (int)sizeof(T);           // elements per vector
     static constexpr int VPR = S / EV;                        // vectors per row
     static constexpr int RC = S * kWave / VPR;                // rows per chunk
-    static constexpr int LC = RC / kIlM;                      // lanes per chunk
+    static constexpr int LC = RC / MI;                        // lanes per chunk
     static constexpr int NC = kWave / LC;                     // chunks per system
-    static constexpr int GP = kIlM + EV;                      // padded lane group
+    static constexpr int GP = MI + EV;                        // padded lane group
     static constexpr int SP = LC * GP + EV;                   // system pitch
     static constexpr int AP = S * SP;                         // array pitch
     static constexpr int P = 128 / (S * (int)sizeof(T));      // workgroups per 128-byte line
@@ -70,21 +68,21 @@ __device__ __forceinline__ int64_t il_group(int64_t b)
     return P * ((k / P) * 8 + x) + (k % P);
 }
 
-template <typename T>
+template <typename T, int MI>
 __device__ __forceinline__ int il_addr(int sys, int row_in_chunk)
 {
-    using G = IlGeom<T>;
-    return sys * G::SP + (row_in_chunk / kIlM) * G::GP + (row_in_chunk % kIlM);
+    using G = IlGeom<T, MI>;
+    return sys * G::SP + (row_in_chunk / MI) * G::GP + (row_in_chunk % MI);
 }
 
 // One vector of array `arr` for chunk `c`: row c*RC + t/VPR, systems
 // s0 + (t%VPR)*EV .. +EV-1. Identity values past n; nothing read past N.
-template <typename T>
+template <typename T, int MI>
 __device__ __forceinline__ void il_fetch(const T* __restrict__ P, int arr, int c, int t,
                                          int64_t n, int64_t N, int64_t stride, int64_t s0,
-                                         bool vec, T (&v)[IlGeom<T>::EV])
+                                         bool vec, T (&v)[IlGeom<T, MI>::EV])
 {
-    using G = IlGeom<T>;
+    using G = IlGeom<T, MI>;
     const int64_t row = (int64_t)c * G::RC + t / G::VPR;
     const int64_t s = s0 + (t % G::VPR) * G::EV;
     const T fill = arr == 1 ? T(1) : T(0);
@@ -106,12 +104,12 @@ __device__ __forceinline__ void il_fetch(const T* __restrict__ P, int arr, int c
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(IlGeom<T>::S * kWave) void il_reg_kernel(
+template <typename T, int MI>
+__global__ __launch_bounds__((IlGeom<T, MI>::S * kWave)) void il_reg_kernel(
     const T* __restrict__ A, const T* __restrict__ B, const T* __restrict__ C,
     const T* __restrict__ D, T* X, int64_t n, int64_t N, int64_t stride, int rest, int vec)
 {
-    using G = IlGeom<T>;
+    using G = IlGeom<T, MI>;
     __shared__ __align__(16) T tile[4 * G::AP];
     const int t = threadIdx.x;
     const int w = t / kWave;
@@ -128,9 +126,9 @@ __global__ __launch_bounds__(IlGeom<T>::S * kWave) void il_reg_kernel(
     for (int c = 0; c < NB && c < G::NC; ++c)
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            il_fetch<T>(P[k], k, c, t, n, N, stride, s0, vec != 0, pre[c][k]);
+            il_fetch<T, MI>(P[k], k, c, t, n, N, stride, s0, vec != 0, pre[c][k]);
 
-    Eq<T> e[kIlM];
+    Eq<T> e[MI];
 #pragma unroll
     for (int c = 0; c < G::NC; ++c) {
         __syncthreads();  // the previous chunk has been read back
@@ -141,39 +139,39 @@ __global__ __launch_bounds__(IlGeom<T>::S * kWave) void il_reg_kernel(
             for (int k = 0; k < 4; ++k)
 #pragma unroll
                 for (int q = 0; q < G::EV; ++q)
-                    tile[k * G::AP + il_addr<T>(sb + q, r)] = pre[c % NB][k][q];
+                    tile[k * G::AP + il_addr<T, MI>(sb + q, r)] = pre[c % NB][k][q];
         }
         __syncthreads();
         if (c + NB < G::NC) {
 #pragma unroll
             for (int k = 0; k < 4; ++k)
-                il_fetch<T>(P[k], k, c + NB, t, n, N, stride, s0, vec != 0, pre[c % NB][k]);
+                il_fetch<T, MI>(P[k], k, c + NB, t, n, N, stride, s0, vec != 0, pre[c % NB][k]);
         }
         if (lane / G::LC == c) {
             const int base = w * G::SP + (lane % G::LC) * G::GP;
-            T v[4][kIlM];
+            T v[4][MI];
 #pragma unroll
             for (int k = 0; k < 4; ++k)
 #pragma unroll
-                for (int q = 0; q < kIlM / G::EV; ++q) {
+                for (int q = 0; q < MI / G::EV; ++q) {
                     const u32x4 x4 =
                         *reinterpret_cast<const u32x4*>(&tile[k * G::AP + base + q * G::EV]);
                     __builtin_memcpy(&v[k][q * G::EV], &x4, 16);
                 }
 #pragma unroll
-            for (int i = 0; i < kIlM; ++i) e[i] = Eq<T>{v[0][i], v[1][i], v[2][i], v[3][i]};
+            for (int i = 0; i < MI; ++i) e[i] = Eq<T>{v[0][i], v[1][i], v[2][i], v[3][i]};
         }
     }
 
-    T xv[kIlM];
-    wave_core<T, kIlM, 1>(e, xv, lane, lane, kWave, rest);
+    T xv[MI];
+    wave_core<T, MI, 1>(e, xv, lane, lane, kWave, rest);
 
     for (int c = 0; c < G::NC; ++c) {
         __syncthreads();
         if (lane / G::LC == c) {
             const int base = w * G::SP + (lane % G::LC) * G::GP;
 #pragma unroll
-            for (int q = 0; q < kIlM / G::EV; ++q) {
+            for (int q = 0; q < MI / G::EV; ++q) {
                 u32x4 x4;
                 __builtin_memcpy(&x4, &xv[q * G::EV], 16);
                 *reinterpret_cast<u32x4*>(&tile[base + q * G::EV]) = x4;
@@ -187,7 +185,7 @@ __global__ __launch_bounds__(IlGeom<T>::S * kWave) void il_reg_kernel(
         if (row < n) {
             T v[G::EV];
 #pragma unroll
-            for (int q = 0; q < G::EV; ++q) v[q] = tile[il_addr<T>(sb + q, r)];
+            for (int q = 0; q < G::EV; ++q) v[q] = tile[il_addr<T, MI>(sb + q, r)];
             const int64_t off = row * stride + s;
             if (vec && s + G::EV <= N) {
                 u32x4 x4;
@@ -206,24 +204,40 @@ bool il_aligned(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) =
 
 }  // namespace
 
-template <typename T>
-hipError_t launch_interleaved_reg(const InterleavedArgs<T>& g, bool* taken)
+template <typename T, int MI>
+hipError_t launch_il_m(const InterleavedArgs<T>& g)
 {
-    using G = IlGeom<T>;
-    *taken = false;
-    if (g.n <= kWave * kIlM / 2 || g.n > kWave * kIlM || g.N <= 0) return hipSuccess;
+    using G = IlGeom<T, MI>;
     const int64_t groups = (g.N + G::S - 1) / G::S;
     constexpr int64_t kMul = 8 * G::P;  // il_group is one-to-one on multiples of this
     const int64_t grid = (groups + kMul - 1) / kMul * kMul;
-    if (grid > 0x7fffffffLL) return hipSuccess;
+    if (grid > 0x7fffffffLL) return hipErrorInvalidConfiguration;
     const int vec = (g.stride % G::EV == 0 && il_aligned(g.a) && il_aligned(g.b) &&
                      il_aligned(g.c) && il_aligned(g.d) && il_aligned(g.x))
                         ? 1
                         : 0;
-    *taken = true;
-    hipLaunchKernelGGL((il_reg_kernel<T>), dim3((unsigned)grid), dim3(G::S * kWave), 0,
+    hipLaunchKernelGGL((il_reg_kernel<T, MI>), dim3((unsigned)grid), dim3(G::S * kWave), 0,
                        g.stream, g.a, g.b, g.c, g.d, g.x, g.n, g.N, g.stride, kWave, vec);
     return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_interleaved_reg(const InterleavedArgs<T>& g, bool* taken)
+{
+    *taken = false;
+    // one wave per system: 16, 8 or 4 unknowns per lane for n up to 1024, 512
+    // or 256; shorter systems stay with the lane-per-system elimination. The
+    // bounds are measured (2^26 unknowns, against that elimination): FP32
+    // n = 512: 452 / 492 us, n = 256 at 4 per lane: 532 / 460 us (its padded
+    // LDS rows leave one workgroup per CU), so FP32 starts above 256; FP64
+    // n = 512: 647 / 945 us, n = 200: 622 / 984 us, n = 129: 1012 / 1069 us.
+    const int64_t lo = sizeof(T) == 4 ? kWave * 4 : kWave * 2;
+    if (g.n <= lo || g.n > kWave * 16 || g.N <= 0) return hipSuccess;
+    if ((g.N + 15) / 16 > 0x7fffffffLL / 64) return hipSuccess;
+    *taken = true;
+    if (g.n > kWave * 8) return launch_il_m<T, 16>(g);
+    if (g.n > kWave * 4) return launch_il_m<T, 8>(g);
+    return launch_il_m<T, 4>(g);
 }
 
 template hipError_t launch_interleaved_reg<float>(const InterleavedArgs<float>&, bool*);
