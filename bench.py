@@ -184,6 +184,9 @@ def bench_interleaved(args):
     kms = [e0.elapsed_time(e1) for e0, e1 in evs]
     avg = sum(kms) / len(kms)
     alg = 5 * n * N * esize
+    # smallest system size (exclusive) of the library's register form for
+    # interleaved batches: 256 for FP32, 128 for FP64
+    il_lo = 256 if args.dtype == "f32" else 128
     out = {"metric": METRIC, "value": n * N * args.steps / wall, "unit": "unknowns/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
@@ -197,9 +200,9 @@ def bench_interleaved(args):
                         "frac": alg / (avg * 1e-3) / 1e9 / PEAK_HBM_GBS,
                         "traffic": traffic_from_profiles(
                             f"interleaved {args.dtype} {N} systems x {n} unknowns"),
-                        "note": ("512 < n <= 1024: register CR-PCR on systems gathered "
+                        "note": (f"{il_lo} < n <= 1024: register CR-PCR on systems gathered "
                                  "through LDS, 5 x T bytes per unknown moved"
-                                 if 512 < n <= 1024 else
+                                 if il_lo < n <= 1024 else
                                  "the two-ended elimination moves 9 x T bytes per unknown "
                                  "(reads a, b, c, d; writes and re-reads c', d'; writes x)")}}
     if not args.no_vendor:
