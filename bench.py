@@ -1017,8 +1017,8 @@ def bench_gauss_sweep(args):
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: multi-GPU runs are not part of this build "
-              "(independent systems shard by system range with no exchange).",
+        print("bench.py measures one GPU only; independent systems shard by system range "
+              "with no exchange, so several GPUs each run this one-GPU workload.",
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "ragged":

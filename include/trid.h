@@ -37,7 +37,7 @@ typedef enum trid_status {
     TRID_EINVAL = 1,  /* bad size, method parameter or null pointer */
     TRID_ENOMEM = 2,  /* device or host allocation failed */
     TRID_EHIP = 3,    /* a HIP runtime call or kernel launch failed */
-    TRID_ENOTSUP = 4, /* valid request that this build cannot run */
+    TRID_ENOTSUP = 4, /* valid request that the library cannot run */
     TRID_ENODEV = 5,  /* no usable gfx950 device */
     TRID_EAGAIN = 6   /* not finished / still in use: try again later */
 } trid_status;

@@ -485,7 +485,7 @@ const char* trid_status_string(int status)
         case TRID_EINVAL: return "TRID_EINVAL: invalid argument";
         case TRID_ENOMEM: return "TRID_ENOMEM: out of memory";
         case TRID_EHIP: return "TRID_EHIP: HIP runtime error";
-        case TRID_ENOTSUP: return "TRID_ENOTSUP: not supported by this build";
+        case TRID_ENOTSUP: return "TRID_ENOTSUP: not supported by this library";
         case TRID_ENODEV: return "TRID_ENODEV: no gfx950 device";
         case TRID_EAGAIN: return "TRID_EAGAIN: not finished, try again";
         default: return "unknown trid status";
