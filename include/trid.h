@@ -209,14 +209,15 @@ int trid_solve_strided_f64(trid_handle handle, int method, const double* a, cons
 /*
  * Interleaved batch: row i of system j at i * batchStride + j (batchStride >=
  * numSystems), the layout of rocSPARSE's gtsv_interleaved_batch. No
- * counterpart in the reference (system-major only). Systems of 513..1024
- * unknowns are gathered through LDS, a few per workgroup, and solved by the
- * register CR-PCR kernel (one wave each); other sizes by Gaussian elimination
- * from both ends at once (two lanes per system, every access coalesced
- * across systems). The a of row 0 and the c of the last row are ignored; x
- * may be d (in place) but no other input. Uses the handle's workspace
- * (systemSize * batchStride elements), so calls on one handle must not
- * overlap on different streams.
+ * counterpart in the reference (system-major only). Systems of 257..1024
+ * (FP32) or 129..1024 (FP64) unknowns are gathered through LDS, a few per
+ * workgroup, and solved by the register CR-PCR kernel (one wave each), with
+ * no workspace; other sizes by Gaussian elimination from both ends at once
+ * (two lanes per system, every access coalesced across systems), which keeps
+ * its factors in the handle's workspace (systemSize * batchStride elements),
+ * so such calls on one handle must not overlap on different streams. The a
+ * of row 0 and the c of the last row are ignored; x may be d (in place) but
+ * no other input.
  */
 int trid_solve_interleaved_f32(trid_handle handle, const float* a, const float* b,
                                const float* c, const float* d, float* x, int64_t systemSize,
@@ -450,10 +451,16 @@ int trid_server_create_adi(trid_server server, int64_t n, const double* u0, int3
 int trid_server_read_adi(trid_server server, int32_t adi, double* u);
 int trid_server_release_adi(trid_server server, int32_t adi);
 /* TRID_ENOMEM if the request can never fit the budget or the batch limits.
- * The caller keeps every array alive until trid_server_wait reports it. */
+ * A batch occupies one contiguous block of the budget, and resident
+ * operators and ADI grids are placed from the top of it; a request that fits
+ * no free block while no solve batch is in flight (so that no retirement can
+ * free memory) is not kept waiting: it completes as failed with status
+ * TRID_ENOMEM. The caller keeps every array alive until trid_server_wait
+ * reports it. */
 int trid_server_submit(trid_server server, const trid_request* request, uint64_t* ticket);
 /* timeout_us < 0 waits without limit; TRID_EAGAIN on timeout. After a final
- * state is reported the ticket is retired. */
+ * state is reported the ticket is retired (later waits return TRID_EINVAL);
+ * threads already waiting on the same ticket all receive the result. */
 int trid_server_wait(trid_server server, uint64_t ticket, int64_t timeout_us,
                      trid_request_result* result);
 /* Page-locked host memory for request arrays (a, b, c, d, x): arrays that lie

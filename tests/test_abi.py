@@ -219,3 +219,65 @@ def test_adi_grid_sizes_are_checked_without_a_device(trid):
     h = ctypes.c_void_p()
     assert lib().trid_adi_create(ctypes.byref(h), 16385, None) == 1      # TRID_EINVAL
     assert lib().trid_adi_create(ctypes.byref(h), 1, None) == 1
+
+
+def test_device_tensors_are_checked_before_the_c_abi(trid):
+    """Torch tensors handed to the device API are checked on the host: a
+    tensor in host memory, a short one, a non-contiguous one or mixed
+    dtypes raise before any kernel could see a host pointer or read past
+    an array's end."""
+    import torch
+    import tridiagonalsolvers_amd.ragged as R
+    h = trid.Handle()
+    cpu = [torch.zeros(64, dtype=torch.float32) for _ in range(5)]
+    with pytest.raises(ValueError, match="not on a GPU"):
+        h.solve("crpcr", *cpu, 64, 1)
+    with pytest.raises(ValueError, match="not on a GPU"):
+        h.solve_interleaved(*cpu, 64, 1)
+    with pytest.raises(ValueError, match="not on a GPU"):
+        trid.generate(0, 1, *cpu[:4], 64, 1)
+    with pytest.raises(TypeError):
+        h.solve("crpcr", *cpu[:4], torch.zeros(64, dtype=torch.float64), 64, 1)
+    with pytest.raises(TypeError):
+        h.solve("crpcr", *(torch.zeros(64, dtype=torch.float16) for _ in range(5)), 64, 1)
+    with pytest.raises(ValueError, match="not on a GPU"):
+        h.solve_refined("crpcr", *(torch.zeros(64, dtype=torch.float64) for _ in range(5)),
+                        64, 1)
+    with pytest.raises(TypeError):
+        h.solve_refined("crpcr", *cpu, 64, 1)             # FP64 only
+    # the ragged entry points use the same checks (a plan itself needs a device)
+    with pytest.raises(ValueError, match="not on a GPU"):
+        R.residual(*(torch.zeros(56, dtype=torch.float64) for _ in range(5)),
+                   torch.tensor([0, 16, 56]), 2, torch.zeros(2, dtype=torch.float64))
+    # the checks that need no device: lengths and layout, with a stand-in
+    # object that reports itself as a GPU tensor
+
+    class FakeGpu:
+        def __init__(self, n, dtype=torch.float32, contiguous=True):
+            self.dtype, self._n, self._c, self.is_cuda = dtype, n, contiguous, True
+
+        def numel(self):
+            return self._n
+
+        def is_contiguous(self):
+            return self._c
+
+        def data_ptr(self):
+            return 256
+
+    with pytest.raises(ValueError, match="63 elements, 64 needed"):
+        h.solve("crpcr", *(FakeGpu(64) for _ in range(4)), FakeGpu(63), 64, 1)
+    with pytest.raises(ValueError, match="not contiguous"):
+        h.solve("crpcr", *(FakeGpu(64) for _ in range(4)), FakeGpu(64, contiguous=False), 64, 1)
+    # strided: (N - 1) * stride + n elements; interleaved: (n - 1) * stride + N
+    with pytest.raises(ValueError, match="needed"):
+        h.solve_strided("crpcr", *(FakeGpu(3 * 80 + 63) for _ in range(5)), 64, 4, 80)
+    with pytest.raises(ValueError, match="needed"):
+        h.solve_interleaved(*(FakeGpu(63 * 10 + 7) for _ in range(5)), 64, 8, 10)
+    with pytest.raises(ValueError, match="info"):
+        h.solve_pivoting(*(FakeGpu(640) for _ in range(5)), 64, 10,
+                         info=FakeGpu(9, dtype=torch.int32))
+    with pytest.raises(TypeError, match="info"):
+        h.solve_pivoting(*(FakeGpu(640) for _ in range(5)), 64, 10,
+                         info=FakeGpu(10, dtype=torch.int64))
+    h.close()

@@ -188,3 +188,66 @@ def test_host_buffers_fake_executor(S):
     with pytest.raises(ValueError):
         srv.host_free(a)
     srv.close()
+
+
+def test_request_larger_than_every_free_block_fails_instead_of_waiting(S):
+    # 64 MiB budget; two resident operators of 15.75 MiB each, placed from
+    # the top of the pool; releasing the upper one leaves 48.25 MiB free in
+    # two blocks (32.5 MiB below the lower operator, 15.75 MiB above it). A
+    # 62 x 16384 request (about 38.8 MiB as a batch) passes the budget check
+    # at submission but fits no block; with nothing in flight it fails with
+    # TRID_ENOMEM rather than staying queued, and close() returns.
+    import time
+    srv = S.Server(fake=True, hbm_budget_bytes=64 << 20, slots=2, batch_window_us=100)
+    sizes = np.full(42, 16384, dtype=np.int64)            # 3 x 688,128 x 8 B = 15.75 MiB
+    n = int(sizes.sum())
+    ops = [srv.register_operator(sizes, np.ones(n), 4 * np.ones(n), np.ones(n))
+           for _ in range(2)]
+    assert srv.release_operator(ops[0]) == 0
+    m = 62 * 16384
+    offs = np.arange(0, m + 1, 16384, dtype=np.int64)
+    t = srv.submit(np.ones(m), np.zeros(m), offs, np.ones(m), 4 * np.ones(m), np.ones(m))
+    r = srv.wait(t, timeout_us=2_000_000)
+    assert r is not None, "request stayed queued with nothing running"
+    assert r["state"] == "failed" and r["status"] == 2   # TRID_ENOMEM
+    # a request that fits the larger block is still served
+    k = 30 * 16384
+    offs = np.arange(0, k + 1, 16384, dtype=np.int64)
+    t = srv.submit(np.ones(k), np.zeros(k), offs, np.ones(k), 4 * np.ones(k), np.ones(k))
+    assert srv.wait(t, timeout_us=2_000_000)["state"] == "done"
+    t0 = time.monotonic()
+    srv.close()
+    assert time.monotonic() - t0 < 5.0
+
+
+def test_close_with_requests_still_queued(S):
+    # close() finishes queued and running requests, then returns
+    import time
+    srv = S.Server(fake=True, slots=1, fake_batch_us=2000, max_batch_systems=1,
+                   batch_window_us=0)
+    rng = np.random.default_rng(11)
+    for _ in range(30):
+        offs, (a, b, c, d), x = _req(rng, most=2)
+        srv.submit(d, x, offsets=offs, a=a, b=b, c=c)
+    t0 = time.monotonic()
+    srv.close()
+    assert time.monotonic() - t0 < 5.0
+
+
+def test_two_threads_wait_on_one_ticket(S):
+    srv = S.Server(fake=True, slots=1, fake_batch_us=50000)
+    offs, (a, b, c, d), x = _req(np.random.default_rng(2), most=3)
+    t = srv.submit(d, x, offsets=offs, a=a, b=b, c=c)
+    out = [None, None]
+
+    def waiter(i):
+        out[i] = srv.wait(t)
+
+    ths = [threading.Thread(target=waiter, args=(i,)) for i in range(2)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    assert all(r is not None and r["state"] == "done" for r in out)
+    assert out[0]["batch"] == out[1]["batch"]
+    srv.close()

@@ -221,25 +221,33 @@ hipError_t launch_il_m(const InterleavedArgs<T>& g)
     return hipGetLastError();
 }
 
+// One wave per system: 16, 8 or 4 unknowns per lane for n up to 1024, 512
+// or 256; shorter systems stay with the lane-per-system elimination. The
+// bounds are measured (2^26 unknowns, against that elimination): FP32
+// n = 512: 452 / 492 us, n = 256 at 4 per lane: 532 / 460 us (its padded
+// LDS rows leave one workgroup per CU), so FP32 starts above 256; FP64
+// n = 512: 647 / 945 us, n = 200: 622 / 984 us, n = 129: 1012 / 1069 us.
+template <typename T>
+bool interleaved_reg_takes(int64_t n, int64_t N)
+{
+    const int64_t lo = sizeof(T) == 4 ? kWave * 4 : kWave * 2;
+    if (n <= lo || n > kWave * 16 || N <= 0) return false;
+    return (N + 15) / 16 <= 0x7fffffffLL / 64;
+}
+
 template <typename T>
 hipError_t launch_interleaved_reg(const InterleavedArgs<T>& g, bool* taken)
 {
     *taken = false;
-    // one wave per system: 16, 8 or 4 unknowns per lane for n up to 1024, 512
-    // or 256; shorter systems stay with the lane-per-system elimination. The
-    // bounds are measured (2^26 unknowns, against that elimination): FP32
-    // n = 512: 452 / 492 us, n = 256 at 4 per lane: 532 / 460 us (its padded
-    // LDS rows leave one workgroup per CU), so FP32 starts above 256; FP64
-    // n = 512: 647 / 945 us, n = 200: 622 / 984 us, n = 129: 1012 / 1069 us.
-    const int64_t lo = sizeof(T) == 4 ? kWave * 4 : kWave * 2;
-    if (g.n <= lo || g.n > kWave * 16 || g.N <= 0) return hipSuccess;
-    if ((g.N + 15) / 16 > 0x7fffffffLL / 64) return hipSuccess;
+    if (!interleaved_reg_takes<T>(g.n, g.N)) return hipSuccess;
     *taken = true;
     if (g.n > kWave * 8) return launch_il_m<T, 16>(g);
     if (g.n > kWave * 4) return launch_il_m<T, 8>(g);
     return launch_il_m<T, 4>(g);
 }
 
+template bool interleaved_reg_takes<float>(int64_t, int64_t);
+template bool interleaved_reg_takes<double>(int64_t, int64_t);
 template hipError_t launch_interleaved_reg<float>(const InterleavedArgs<float>&, bool*);
 template hipError_t launch_interleaved_reg<double>(const InterleavedArgs<double>&, bool*);
 

@@ -89,7 +89,8 @@ struct InterleavedArgs {
     int64_t n;       // rows per system
     int64_t N;       // systems, N <= stride
     int64_t stride;  // elements between consecutive rows of one system
-    T* ws;           // workspace of n * stride elements (c' factors)
+    T* ws;           // workspace of n * stride elements (c' factors); unused
+                     // (may be null) when interleaved_reg_takes<T>(n, N)
     hipStream_t stream;
 };
 
@@ -99,6 +100,9 @@ hipError_t launch_interleaved(const InterleavedArgs<T>& g);
 // false when the request is outside its range and nothing was launched.
 template <typename T>
 hipError_t launch_interleaved_reg(const InterleavedArgs<T>& g, bool* taken);
+// True when the register-core form serves (n, N); it needs no workspace.
+template <typename T>
+bool interleaved_reg_takes(int64_t n, int64_t N);
 
 // Partial-pivoting solve, LAPACK ?gtsv semantics (kern_gtsv.hip).
 template <typename T>

@@ -75,6 +75,8 @@ struct Req {
     int64_t batch_systems = 0;
     int64_t pos = 0;  // element offset inside its batch
     bool x_direct = false;  // result copied straight into a client host buffer
+    int waiters = 0;        // threads inside trid_server_wait on this ticket
+    bool retired = false;   // final state reported; no longer in the ticket map
 };
 
 struct Adi {
@@ -118,6 +120,29 @@ struct Pool {
             }
         }
         return false;
+    }
+    // From the top end of the highest free block that holds `bytes`:
+    // resident objects (operators, ADI grids) pile up from the top of the
+    // pool, so the space batches come and go in stays one block at the bottom.
+    bool alloc_high(size_t bytes, size_t* off)
+    {
+        for (auto it = free_.rbegin(); it != free_.rend(); ++it) {
+            if (it->second >= bytes) {
+                const size_t start = it->first;
+                const size_t size = it->second;
+                *off = start + size - bytes;
+                free_.erase(start);
+                if (size > bytes) free_[start] = size - bytes;
+                return true;
+            }
+        }
+        return false;
+    }
+    size_t largest() const
+    {
+        size_t m = 0;
+        for (const auto& kv : free_) m = std::max(m, kv.second);
+        return m;
     }
     void release(size_t off, size_t bytes)
     {
@@ -276,14 +301,20 @@ struct trid_server_s {
         else if (state == kCancelled) st.cancelled++;
     }
 
-    int launch(Slot& s);
-    void retire(Slot& s, bool ok);
+    int launch(Slot& s, std::unique_lock<std::mutex>& lk);
+    void retire(Slot& s, bool ok, std::unique_lock<std::mutex>& lk);
     void run();
 };
 
 // ---------------------------------------------------------------------------
 
-int trid_server_s::launch(Slot& s)
+// Called with the server mutex held (lk). A solve batch is planned under the
+// lock (sizes, work queue, which arrays go by DMA, device addresses of the
+// operators it uses); the lock is released for the host staging copies and
+// the HIP calls, which touch only the slot (owned by the scheduler thread)
+// and that snapshot, so submit / wait / stats from clients never wait for a
+// batch's copies. The lock is held again on return.
+int trid_server_s::launch(Slot& s, std::unique_lock<std::mutex>& lk)
 {
     if (s.reqs.size() == 1 && s.reqs[0]->r.adi >= 0) {
         Req* q = s.reqs[0];
@@ -346,9 +377,21 @@ int trid_server_s::launch(Slot& s)
         const void* src;
         size_t bytes;
     };
+    struct OpCopy {
+        int64_t pos;
+        const double* abc;  // [a | b | c] of the operator, `stride` apart
+        int64_t stride;
+        size_t bytes;
+    };
+    struct XCopy {
+        void* dst;
+        int64_t pos;
+        size_t bytes;
+    };
     std::vector<CopyJob> jobs;
     std::vector<Dma> dmas;
-    bool staged_any = false;
+    std::vector<OpCopy> opcopies;
+    std::vector<XCopy> xcopies;
     for (Req* q : s.reqs) {
         const size_t nb = (size_t)q->unknowns * sizeof(double);
         const double* src[4] = {q->r.a, q->r.b, q->r.c, q->r.d};
@@ -359,17 +402,29 @@ int trid_server_s::launch(Slot& s)
             } else {
                 jobs.push_back(CopyJob{stg[a] + q->pos, src[a], nb});
                 dmas.push_back(Dma{a, q->pos, stg[a] + q->pos, nb});
-                staged_any = true;
             }
         }
+        if (q->r.op >= 0) {
+            const Op& op = ops[q->r.op];  // held by the request's reference
+            opcopies.push_back(OpCopy{q->pos, op.d_abc, op.unknowns, nb});
+        }
+        // results: straight into client page-locked buffers, else via staging
+        q->x_direct = in_host_buf(q->r.x, nb);
+        xcopies.push_back(XCopy{q->x_direct ? static_cast<void*>(q->r.x)
+                                            : static_cast<void*>(P + 4 * L.arr +
+                                                                 q->pos * sizeof(double)),
+                                q->pos, nb});
     }
+    const int64_t work = (int64_t)hp.n_wg_items + (hp.n_wave_items + 15) / 16;
+    const int grid_now = work < grid ? (int)std::max<int64_t>(1, work) : grid;
+    char* D = d_pool + s.dev_off;
+
+    lk.unlock();
     parallel_copy(jobs);
-    (void)staged_any;
     std::memcpy(pdesc, hp.sys.data(), sizeof(trid::RaggedSys) * N);
     std::memcpy(pdesc + L.sys, hp.items.data(), sizeof(trid::RaggedItem) * hp.items.size());
     std::memcpy(pdesc + L.sys + L.itm, hp.buckets.data(),
                 sizeof(trid::RaggedBucket) * hp.buckets.size());
-    char* D = d_pool + s.dev_off;
     hipError_t e = hipSuccess;
     for (const Dma& m : dmas) {
         if (e != hipSuccess) break;
@@ -379,54 +434,45 @@ int trid_server_s::launch(Slot& s)
     if (e == hipSuccess)
         e = hipMemcpyAsync(D + 5 * L.arr, pdesc, L.sys + L.itm + L.bkt,
                            hipMemcpyHostToDevice, s.stream);
-    for (Req* q : s.reqs) {
-        if (e != hipSuccess) break;
-        if (q->r.op < 0) continue;
-        const Op& op = ops[q->r.op];
-        const size_t nb = (size_t)q->unknowns * sizeof(double);
+    for (const OpCopy& c : opcopies)
         for (int a = 0; a < 3 && e == hipSuccess; ++a)
-            e = hipMemcpyAsync(D + a * L.arr + q->pos * sizeof(double),
-                               op.d_abc + (size_t)a * op.unknowns, nb, hipMemcpyDeviceToDevice,
-                               s.stream);
+            e = hipMemcpyAsync(D + a * L.arr + c.pos * sizeof(double), c.abc + a * c.stride,
+                               c.bytes, hipMemcpyDeviceToDevice, s.stream);
+    if (e == hipSuccess) {
+        trid::RaggedArgs g{};
+        g.a = reinterpret_cast<const double*>(D);
+        g.b = reinterpret_cast<const double*>(D + L.arr);
+        g.c = reinterpret_cast<const double*>(D + 2 * L.arr);
+        g.d = reinterpret_cast<const double*>(D + 3 * L.arr);
+        g.x = reinterpret_cast<double*>(D + 4 * L.arr);
+        char* desc = D + 5 * L.arr;
+        g.sys = reinterpret_cast<const trid::RaggedSys*>(desc);
+        g.items = reinterpret_cast<const trid::RaggedItem*>(desc + L.sys);
+        g.buckets = reinterpret_cast<const trid::RaggedBucket*>(desc + L.sys + L.itm);
+        g.counters = reinterpret_cast<int*>(desc + L.sys + L.itm + L.bkt);
+        g.n_wg_items = hp.n_wg_items;
+        g.n_wave_items = hp.n_wave_items;
+        g.n_buckets = (int)hp.buckets.size();
+        g.iters = o.iters;
+        g.wide = 0;
+        for (const auto& b : hp.buckets)
+            if (b.per_lane == 16) g.wide = 1;
+        e = trid::launch_ragged(g, grid_now, s.stream);
     }
-    if (e != hipSuccess) return TRID_EHIP;
-    trid::RaggedArgs g{};
-    g.a = reinterpret_cast<const double*>(D);
-    g.b = reinterpret_cast<const double*>(D + L.arr);
-    g.c = reinterpret_cast<const double*>(D + 2 * L.arr);
-    g.d = reinterpret_cast<const double*>(D + 3 * L.arr);
-    g.x = reinterpret_cast<double*>(D + 4 * L.arr);
-    char* desc = D + 5 * L.arr;
-    g.sys = reinterpret_cast<const trid::RaggedSys*>(desc);
-    g.items = reinterpret_cast<const trid::RaggedItem*>(desc + L.sys);
-    g.buckets = reinterpret_cast<const trid::RaggedBucket*>(desc + L.sys + L.itm);
-    g.counters = reinterpret_cast<int*>(desc + L.sys + L.itm + L.bkt);
-    g.n_wg_items = hp.n_wg_items;
-    g.n_wave_items = hp.n_wave_items;
-    g.n_buckets = (int)hp.buckets.size();
-    g.iters = o.iters;
-    g.wide = 0;
-    for (const auto& b : hp.buckets)
-        if (b.per_lane == 16) g.wide = 1;
-    int grid_now = grid;
-    const int64_t work = (int64_t)hp.n_wg_items + (hp.n_wave_items + 15) / 16;
-    if (work < grid_now) grid_now = (int)std::max<int64_t>(1, work);
-    e = trid::launch_ragged(g, grid_now, s.stream);
-    // results: straight into client page-locked buffers, else via staging
-    for (Req* q : s.reqs) {
+    for (const XCopy& c : xcopies) {
         if (e != hipSuccess) break;
-        const size_t nb = (size_t)q->unknowns * sizeof(double);
-        q->x_direct = in_host_buf(q->r.x, nb);
-        void* dst = q->x_direct ? static_cast<void*>(q->r.x)
-                                : static_cast<void*>(P + 4 * L.arr + q->pos * sizeof(double));
-        e = hipMemcpyAsync(dst, D + 4 * L.arr + q->pos * sizeof(double), nb,
+        e = hipMemcpyAsync(c.dst, D + 4 * L.arr + c.pos * sizeof(double), c.bytes,
                            hipMemcpyDeviceToHost, s.stream);
     }
     if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
+    lk.lock();
     return e == hipSuccess ? TRID_OK : TRID_EHIP;
 }
 
-void trid_server_s::retire(Slot& s, bool ok)
+// Called with the server mutex held (lk); released while the results are
+// copied out of the slot's staging block (the slot's requests are running,
+// so no client call changes them meanwhile).
+void trid_server_s::retire(Slot& s, bool ok, std::unique_lock<std::mutex>& lk)
 {
     const bool adi_req = s.reqs.size() == 1 && s.reqs[0]->r.adi >= 0;
     if (ok && !fake && !adi_req) {
@@ -437,7 +483,9 @@ void trid_server_s::retire(Slot& s, bool ok)
             if (!q->x_direct)
                 jobs.push_back(
                     CopyJob{q->r.x, px + q->pos, (size_t)q->unknowns * sizeof(double)});
+        lk.unlock();
         parallel_copy(jobs);
+        lk.lock();
     }
     for (Req* q : s.reqs) finish(q, ok ? kDone : kFailed, ok ? TRID_OK : TRID_EHIP);
     if (s.dev_bytes) pool.release(s.dev_off, s.dev_bytes);
@@ -457,11 +505,11 @@ void trid_server_s::run()
         for (Slot& s : slots) {
             if (!s.busy) continue;
             if (fake) {
-                if (Clock::now() >= s.fake_done) retire(s, !s.failed);
+                if (Clock::now() >= s.fake_done) retire(s, !s.failed, lk);
             } else {
                 const hipError_t q = hipEventQuery(s.done);
-                if (q == hipSuccess) retire(s, true);
-                else if (q != hipErrorNotReady) retire(s, false);
+                if (q == hipSuccess) retire(s, true, lk);
+                else if (q != hipErrorNotReady) retire(s, false, lk);
             }
             any_busy = any_busy || s.busy;
         }
@@ -502,7 +550,7 @@ void trid_server_s::run()
                 adi_q->t_launch = now;
                 adi_q->batch = s.batch;
                 adi_q->batch_systems = 2 * adis[adi_q->r.adi].n;
-                const int rc = launch(s);
+                const int rc = launch(s, lk);
                 if (rc != TRID_OK) {
                     finish(adi_q, kFailed, rc);
                     s.reqs.clear();
@@ -527,16 +575,19 @@ void trid_server_s::run()
             if (device_busy && age < o.batch_window_us && waiting < o.max_batch_unknowns &&
                 !stopping)
                 continue;
-            // FIFO selection of solve requests within limits and the memory budget
+            // FIFO selection of solve requests within the batch limits and
+            // the largest free block of the pool (a batch is one block)
             int64_t total = 0, N = 0;
             std::vector<Req*> pick;
-            const size_t free_b = budget_free();
+            const size_t free_b = std::min(budget_free(), pool.largest());
+            Req* head = nullptr;  // the oldest solve request
             for (auto it = pending.begin(); it != pending.end();) {
                 Req* q = *it;
                 if (q->r.adi >= 0) {  // waits for its grid (taken above when free)
                     ++it;
                     continue;
                 }
+                if (!head) head = q;
                 const int64_t t2 = total + q->unknowns;
                 const int64_t n2 = N + (int64_t)q->sizes.size();
                 const bool fits = t2 <= o.max_batch_unknowns && n2 <= o.max_batch_systems &&
@@ -552,7 +603,22 @@ void trid_server_s::run()
                 if (us_between(q->t_submit, now) > o.batch_window_us) break;
                 ++it;
             }
-            if (pick.empty()) break;
+            if (pick.empty()) {
+                // Nothing fits. With no solve batch in flight no retirement
+                // will free memory (the free blocks are the gaps between
+                // resident operators and grids), so the oldest request fails
+                // with TRID_ENOMEM instead of waiting for ever.
+                const bool batch_in_flight = std::any_of(
+                    slots.begin(), slots.end(),
+                    [](const Slot& x) { return x.busy && x.dev_bytes > 0; });
+                if (!batch_in_flight && head) {
+                    pending.erase(std::find(pending.begin(), pending.end(), head));
+                    finish(head, kFailed, TRID_ENOMEM);
+                    cv_done.notify_all();
+                    continue;
+                }
+                break;
+            }
             const BatchLayout L(total, N);
             size_t off = 0;
             if (!pool.alloc(L.bytes, &off)) {
@@ -578,7 +644,7 @@ void trid_server_s::run()
                 q->batch = s.batch;
                 q->batch_systems = N;
             }
-            const int rc = launch(s);
+            const int rc = launch(s, lk);
             if (rc != TRID_OK) {
                 for (Req* q : s.reqs) finish(q, kFailed, rc);
                 pool.release(s.dev_off, s.dev_bytes);
@@ -723,7 +789,7 @@ int trid_server_register_operator(trid_server s, const int64_t* sizes, int64_t n
     std::lock_guard<std::mutex> lk(s->mu);
     if (o.bytes > s->budget_free()) return TRID_ENOMEM;
     size_t off = 0;
-    if (!s->pool.alloc(o.bytes, &off)) return TRID_ENOMEM;
+    if (!s->pool.alloc_high(o.bytes, &off)) return TRID_ENOMEM;
     o.pool_off = off;
     if (!s->fake) {
         o.d_abc = reinterpret_cast<double*>(s->d_pool + off);
@@ -765,7 +831,7 @@ int trid_server_create_adi(trid_server s, int64_t n, const double* u0, int32_t* 
     std::lock_guard<std::mutex> lk(s->mu);
     if (a.bytes > s->budget_free()) return TRID_ENOMEM;
     size_t off = 0;
-    if (!s->pool.alloc(a.bytes, &off)) return TRID_ENOMEM;
+    if (!s->pool.alloc_high(a.bytes, &off)) return TRID_ENOMEM;
     a.pool_off = off;
     if (!s->fake) {
         a.dev = s->d_pool + off;
@@ -903,9 +969,13 @@ int trid_server_wait(trid_server s, uint64_t ticket, int64_t timeout_us, trid_re
     if (it == s->reqs.end()) return TRID_EINVAL;
     Req* q = it->second;
     auto final_ = [q] { return q->state >= kDone; };
+    // several threads may wait on one ticket: the last one out frees it
+    q->waiters++;
+    bool done = true;
     if (timeout_us < 0) s->cv_done.wait(lk, final_);
-    else if (!s->cv_done.wait_for(lk, std::chrono::microseconds(timeout_us), final_))
-        return TRID_EAGAIN;
+    else done = s->cv_done.wait_for(lk, std::chrono::microseconds(timeout_us), final_);
+    q->waiters--;
+    if (!done) return TRID_EAGAIN;
     if (res) {
         res->status = q->status;
         res->state = q->state;
@@ -915,8 +985,11 @@ int trid_server_wait(trid_server s, uint64_t ticket, int64_t timeout_us, trid_re
         res->batch = q->batch;
         res->batch_systems = q->batch_systems;
     }
-    s->reqs.erase(it);
-    delete q;
+    if (!q->retired) {
+        q->retired = true;
+        s->reqs.erase(ticket);
+    }
+    if (q->waiters == 0) delete q;
     return TRID_OK;
 }
 

@@ -13,7 +13,7 @@ import os
 import numpy as np
 
 from ._lib import BucketCfg, RaggedInfo, check, lib
-from .solvers import _ptr, _stream_ptr
+from .solvers import _dev_ptrs, _ptr, _stream_ptr
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_TABLE = os.path.join(HERE, "tune", "ragged_f64_refine.json")
@@ -85,8 +85,9 @@ class RaggedPlan:
         if iters is None:
             iters = 0 if fp64 else 2
         flags = (0 if graph else NO_GRAPH) | (FP64 if fp64 else 0)
-        st = lib().trid_ragged_solve_f64(self._h, *(ctypes.c_void_p(_ptr(v)) for v in (a, b, c, d, x)),
-                                         int(iters), flags, ctypes.c_void_p(_stream_ptr(stream)))
+        ptrs = _dev_ptrs((a, b, c, d, x), int(self.offsets[-1]), "f64", "ragged solve")
+        st = lib().trid_ragged_solve_f64(self._h, *ptrs, int(iters), flags,
+                                         ctypes.c_void_p(_stream_ptr(stream)))
         check(st, "trid_ragged_solve_f64")
 
     def close(self):
@@ -103,7 +104,11 @@ class RaggedPlan:
 
 def residual(a, b, c, d, x, d_offsets, num_systems: int, out, stream=None) -> None:
     """Per-system relative residual in FP64 on the device (trid_residual_f64)."""
-    st = lib().trid_residual_f64(*(ctypes.c_void_p(_ptr(v)) for v in (a, b, c, d, x)),
-                                 ctypes.c_void_p(_ptr(d_offsets)), int(num_systems),
-                                 ctypes.c_void_p(_ptr(out)), ctypes.c_void_p(_stream_ptr(stream)))
+    offs = _dev_ptrs((d_offsets,), int(num_systems) + 1, "i64", "residual offsets")[0]
+    res = _dev_ptrs((out,), int(num_systems), "f64", "residual out")[0]
+    # array lengths come from the offsets on the device; dtype and placement
+    # are checked here
+    ptrs = _dev_ptrs((a, b, c, d, x), 0, "f64", "residual")
+    st = lib().trid_residual_f64(*ptrs, offs, int(num_systems), res,
+                                 ctypes.c_void_p(_stream_ptr(stream)))
     check(st, "trid_residual_f64")

@@ -9,7 +9,9 @@ for numpy arrays (float32 or float64 picks the _f32 or _f64 entry point).
 ``TridError`` instead of exiting the process.
 
 ``Handle.solve`` is the stream-ordered device API (``trid_solve_*``) for
-device pointers, e.g. ``torch.Tensor.data_ptr()`` of tensors on ``cuda:0``.
+device arrays: torch tensors on a GPU (checked for device, dtype,
+contiguity and length before any kernel runs) or integer device addresses
+(FP32 unless ``dtype="f64"`` is given).
 """
 from __future__ import annotations
 
@@ -131,6 +133,56 @@ def _ptr(v) -> int:
     raise TypeError("device arrays must be torch tensors or integer addresses")
 
 
+_TORCH_DTYPES = {"torch.float32": "f32", "torch.float64": "f64", "torch.int32": "i32",
+                 "torch.int64": "i64"}
+
+
+def _dtype_of(v) -> str | None:
+    """'f32' / 'f64' / 'i32' / 'i64' for a torch tensor, None for a raw address."""
+    if isinstance(v, int) or v is None:
+        return None
+    name = str(getattr(v, "dtype", ""))
+    if name not in _TORCH_DTYPES:
+        raise TypeError(f"unsupported device array dtype {name or type(v).__name__}")
+    return _TORCH_DTYPES[name]
+
+
+def _infer_dtype(a, dtype: str | None) -> str:
+    if dtype is not None:
+        if dtype not in ("f32", "f64"):
+            raise ValueError("dtype must be 'f32' or 'f64'")
+        return dtype
+    t = _dtype_of(a)
+    if t is None:            # integer device address: FP32 unless dtype says otherwise
+        return "f32"
+    if t not in ("f32", "f64"):
+        raise TypeError("device arrays must be float32 or float64")
+    return t
+
+
+def _dev_ptrs(arrs, need: int, dtype: str, what: str = "array"):
+    """Device addresses of `arrs` after checking what a torch tensor can tell:
+    on a GPU, of `dtype`, contiguous and holding at least `need` elements.
+    Integer addresses pass unchecked (the caller vouches for them). Raises
+    before the C ABI is called, so no kernel sees a host pointer or reads past
+    an array's end."""
+    tensors = [v for v in arrs if _dtype_of(v) is not None]
+    for v in tensors:
+        if _dtype_of(v) != dtype:
+            raise TypeError(f"{what}: tensor dtype {v.dtype} does not match {dtype}")
+    out = []
+    for v in arrs:
+        if _dtype_of(v) is not None:
+            if not getattr(v, "is_cuda", False):
+                raise ValueError(f"{what}: tensor is not on a GPU device")
+            if not v.is_contiguous():
+                raise ValueError(f"{what}: tensor is not contiguous")
+            if v.numel() < need:
+                raise ValueError(f"{what}: tensor holds {v.numel()} elements, {need} needed")
+        out.append(ctypes.c_void_p(_ptr(v)))
+    return out
+
+
 def _stream_ptr(stream) -> int | None:
     if stream is None:
         return None
@@ -160,13 +212,12 @@ class Handle:
 
     def solve(self, method, a, b, c, d, x, systemSize: int, numSystems: int,
               opts: Opts | None = None, stream=None, dtype: str | None = None) -> None:
-        if dtype is None:
-            dtype = "f64" if str(getattr(a, "dtype", "")) == "torch.float64" else "f32"
+        dtype = _infer_dtype(a, dtype)
+        ptrs = _dev_ptrs((a, b, c, d, x), int(systemSize) * int(numSystems), dtype, "solve")
         f = getattr(lib(), f"trid_solve_{dtype}")
         o = opts if opts is not None else Opts()
-        st = f(self._h, method_id(method), *(ctypes.c_void_p(_ptr(v)) for v in (a, b, c, d, x)),
-               int(systemSize), int(numSystems), ctypes.byref(o),
-               ctypes.c_void_p(_stream_ptr(stream)))
+        st = f(self._h, method_id(method), *ptrs, int(systemSize), int(numSystems),
+               ctypes.byref(o), ctypes.c_void_p(_stream_ptr(stream)))
         check(st, "trid_solve")
 
     def solve_strided(self, method, a, b, c, d, x, systemSize: int, numSystems: int,
@@ -174,13 +225,13 @@ class Handle:
                       dtype: str | None = None) -> None:
         """Strided batch: system s at s * batchStride (>= systemSize), the
         layout of rocSPARSE's gtsv_no_pivot_strided_batch; register methods."""
-        if dtype is None:
-            dtype = "f64" if str(getattr(a, "dtype", "")) == "torch.float64" else "f32"
+        dtype = _infer_dtype(a, dtype)
+        need = (int(numSystems) - 1) * int(batchStride) + int(systemSize) if numSystems > 0 else 0
+        ptrs = _dev_ptrs((a, b, c, d, x), need, dtype, "solve_strided")
         f = getattr(lib(), f"trid_solve_strided_{dtype}")
         o = opts if opts is not None else Opts()
-        st = f(self._h, method_id(method), *(ctypes.c_void_p(_ptr(v)) for v in (a, b, c, d, x)),
-               int(systemSize), int(numSystems), int(batchStride), ctypes.byref(o),
-               ctypes.c_void_p(_stream_ptr(stream)))
+        st = f(self._h, method_id(method), *ptrs, int(systemSize), int(numSystems),
+               int(batchStride), ctypes.byref(o), ctypes.c_void_p(_stream_ptr(stream)))
         check(st, "trid_solve_strided")
 
     def solve_interleaved(self, a, b, c, d, x, systemSize: int, numSystems: int,
@@ -188,12 +239,12 @@ class Handle:
                           dtype: str | None = None) -> None:
         """Interleaved batch: row i of system j at i * batchStride + j
         (default batchStride = numSystems); x may be d."""
-        if dtype is None:
-            dtype = "f64" if str(getattr(a, "dtype", "")) == "torch.float64" else "f32"
-        f = getattr(lib(), f"trid_solve_interleaved_{dtype}")
+        dtype = _infer_dtype(a, dtype)
         stride = numSystems if batchStride is None else batchStride
-        st = f(self._h, *(ctypes.c_void_p(_ptr(v)) for v in (a, b, c, d, x)),
-               int(systemSize), int(numSystems), int(stride),
+        need = (int(systemSize) - 1) * int(stride) + int(numSystems) if numSystems > 0 else 0
+        ptrs = _dev_ptrs((a, b, c, d, x), need, dtype, "solve_interleaved")
+        f = getattr(lib(), f"trid_solve_interleaved_{dtype}")
+        st = f(self._h, *ptrs, int(systemSize), int(numSystems), int(stride),
                ctypes.c_void_p(_stream_ptr(stream)))
         check(st, "trid_solve_interleaved")
 
@@ -204,10 +255,15 @@ class Handle:
         batchStride, interleaved (row i of system j at i * batchStride + j).
         info: optional device int32 array of numSystems (0, or the 1-based
         row of the first zero pivot); x may be d."""
-        if dtype is None:
-            dtype = "f64" if str(getattr(a, "dtype", "")) == "torch.float64" else "f32"
-        ptrs = [ctypes.c_void_p(_ptr(v)) for v in (a, b, c, d, x)]
-        info_p = ctypes.c_void_p(_ptr(info) if info is not None else None)
+        dtype = _infer_dtype(a, dtype)
+        if batchStride is None:
+            need = int(systemSize) * int(numSystems)
+        else:
+            need = (int(systemSize) - 1) * int(batchStride) + int(numSystems) if numSystems > 0 else 0
+        ptrs = _dev_ptrs((a, b, c, d, x), need, dtype, "solve_pivoting")
+        info_p = ctypes.c_void_p(None)
+        if info is not None:
+            info_p = _dev_ptrs((info,), int(numSystems), "i32", "solve_pivoting info")[0]
         s = ctypes.c_void_p(_stream_ptr(stream))
         if batchStride is None:
             st = getattr(lib(), f"trid_solve_pivoting_{dtype}")(
@@ -221,21 +277,22 @@ class Handle:
                       iters: int = 2, opts: Opts | None = None, stream=None) -> None:
         """FP64 arrays; FP32 solves plus `iters` FP64 residual corrections."""
         o = opts if opts is not None else Opts()
+        ptrs = _dev_ptrs((a, b, c, d, x), int(systemSize) * int(numSystems), "f64",
+                         "solve_refined")
         st = lib().trid_solve_refined_f64(
-            self._h, method_id(method), *(ctypes.c_void_p(_ptr(v)) for v in (a, b, c, d, x)),
-            int(systemSize), int(numSystems), int(iters), ctypes.byref(o),
-            ctypes.c_void_p(_stream_ptr(stream)))
+            self._h, method_id(method), *ptrs, int(systemSize), int(numSystems), int(iters),
+            ctypes.byref(o), ctypes.c_void_p(_stream_ptr(stream)))
         check(st, "trid_solve_refined_f64")
 
 
 def generate(kind: int, seed: int, a, b, c, d, systemSize: int, numSystems: int,
              stream=None, dtype: str | None = None) -> None:
     """Seeded synthetic batch written on the device (trid_generate_*)."""
-    if dtype is None:
-        dtype = "f64" if str(getattr(a, "dtype", "")) == "torch.float64" else "f32"
+    dtype = _infer_dtype(a, dtype)
+    ptrs = _dev_ptrs((a, b, c, d), int(systemSize) * int(numSystems), dtype, "generate")
     f = getattr(lib(), f"trid_generate_{dtype}")
-    st = f(int(kind), ctypes.c_uint64(seed), *(ctypes.c_void_p(_ptr(v)) for v in (a, b, c, d)),
-           int(systemSize), int(numSystems), ctypes.c_void_p(_stream_ptr(stream)))
+    st = f(int(kind), ctypes.c_uint64(seed), *ptrs, int(systemSize), int(numSystems),
+           ctypes.c_void_p(_stream_ptr(stream)))
     check(st, "trid_generate")
 
 
