@@ -4,8 +4,11 @@ One step = one solve of the whole batch (inputs already resident in HBM,
 written by the device generator, seed 1, diagonally dominant kind). Prints
 ONE JSON line with unknowns solved per second, the HBM roofline of the solve
 kernel (algorithmic bytes 5 * n * 4 per system: read a, b, c, d, write x)
-and a CPU baseline (the FP32 Thomas recurrence of the oracle, OpenMP over all
-host cores) timed in the same run.
+and a CPU baseline (the FP64 and FP32 Thomas recurrences of the oracle,
+OpenMP over all host cores) timed in the same run, then the served half of
+the metric: the request server under Poisson arrivals with resident ADI
+grids (p50/p99 latency, served unknowns/s against plain per-size launches,
+max relative residual) in a `served` object.
 
     python bench.py [--steps K] [--warmup W] [--method crpcr] [--n 1024]
                     [--systems 65536] [--rest 64] [--dtype f32]
@@ -38,6 +41,10 @@ def parse():
                         "that needs row interchanges; adi: Peaceman-Rachford steps of a "
                         "resident 2048^2 and 4096^2 FP64 grid")
     p.add_argument("--requests", type=int, default=64)
+    p.add_argument("--served-requests", type=int, default=24,
+                   help="uniform workload: requests of the served leg added to the line "
+                        "(the request server with Poisson arrivals and resident ADI "
+                        "grids); 0 = no served leg")
     p.add_argument("--ragged-mode", default="refine", choices=["refine", "fp64"],
                    help="ragged: FP32 solve + 2 FP64 refinements (the mixed-precision "
                         "workload), or one FP64 solve per system (both are timed; this "
@@ -410,8 +417,9 @@ def bench_adi(args):
     column solve -- 4 passes of 16 bytes per point, u' staying transposed
     between steps; one transpose back per call). The 4096^2 grid is the
     reported workload. Unless
-    --no-cpu, one step on 2048^2 is checked against the oracle's FP64 ADI
-    step and the oracle step is timed on the host as the CPU baseline."""
+    --no-cpu, one step on a random grid of each size is checked against the
+    oracle's FP64 ADI step (max relative error <= 1e-12, else the run fails)
+    and the oracle step on 2048^2 is timed on the host as the CPU baseline."""
     import numpy as np
     import torch
 
@@ -458,12 +466,20 @@ def bench_adi(args):
     if not args.no_cpu:
         from oracle import oracle as O
 
+        # one step on each grid size against the oracle's FP64 step
+        errs = {}
+        for m in (2048, 4096):
+            u0 = np.random.default_rng(m + 5).random(m * m)
+            dev = torch.from_numpy(u0).cuda()
+            g = AdiGrid(m, dev)
+            g.step(1.5, 1, stream=stream)
+            torch.cuda.synchronize()
+            ref = O.adi_step_np(u0.reshape(m, m), 1.5)
+            got = dev.cpu().numpy().reshape(m, m)
+            errs[m] = float(np.abs(got - ref).max() / np.abs(ref).max())
+            g.close()
         m = 2048
         u0 = np.random.default_rng(5).random(m * m)
-        dev = torch.from_numpy(u0).cuda()
-        g = AdiGrid(m, dev)
-        g.step(1.5, 1, stream=stream)
-        torch.cuda.synchronize()
         reps, t1 = 0, time.perf_counter()
         while True:
             ref = O.adi_step_np(u0.reshape(m, m), 1.5)
@@ -471,15 +487,15 @@ def bench_adi(args):
             el = time.perf_counter() - t1
             if el >= args.cpu_seconds:
                 break
-        got = dev.cpu().numpy().reshape(m, m)
-        err = float(np.abs(got - ref).max() / np.abs(ref).max())
-        g.close()
         out["cpu_baseline"] = {
             "value": reps * m * m / el, "unit": "grid points/s", "cores": O.max_threads(),
             "kind": "port",
             "sample": f"oracle FP64 ADI step on 2048^2 x {reps}, {el:.1f} s (Thomas per "
                       f"sweep, OpenMP over rows)",
-            "gpu_max_rel_err_vs_oracle_step": err}
+            "gpu_max_rel_err_vs_oracle_step": max(errs.values()),
+            "gpu_max_rel_err_vs_oracle_step_by_grid": {str(k): v for k, v in errs.items()}}
+        if max(errs.values()) > 1e-12:
+            raise SystemExit(f"ADI step differs from the oracle step: {errs}")
     print(json.dumps(out), flush=True)
 
 
@@ -548,10 +564,13 @@ def stream_ceiling(a, b, c, d, x, stream, reps=20):
 
 
 def cpu_baseline(args, x_gpu_np, n, N):
-    """FP32 Thomas (same recurrence as tridiagonal_gold.h:26-44) with OpenMP
-    over all host cores on the same inputs, repeated for about
-    --cpu-seconds; also reports the GPU result's max relative error against
-    the FP64 Thomas oracle on those inputs."""
+    """The oracle's Thomas (same recurrence as tridiagonal_gold.h:26-44) with
+    OpenMP over all host cores on the same inputs: the FP64 recurrence (the
+    oracle itself, on the FP32-rounded inputs of an FP32 run) is the reported
+    value, and for FP32 runs the FP32 recurrence is timed beside it for a
+    like-for-like figure; each is repeated for about half of --cpu-seconds.
+    Also reports the GPU result's max relative error against the FP64
+    oracle on those inputs."""
     import numpy as np
 
     from oracle import oracle as O
@@ -559,28 +578,37 @@ def cpu_baseline(args, x_gpu_np, n, N):
     dt = np.float32 if args.dtype == "f32" else np.float64
     a, b, c, d = O.generate(getattr(args, "kind", "dominant"), 1, n, N, dt)
     threads = O.max_threads()
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        if dt == np.float32:
-            O.thomas_f32(a, b, c, d, n, threads)
-        else:
-            O.thomas(a, b, c, d, n=n, threads=threads)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= args.cpu_seconds:
-            break
+
+    def timed(fn, seconds):
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            fn()
+            reps += 1
+            el = time.perf_counter() - t0
+            if el >= seconds:
+                return reps, el
+
+    share = args.cpu_seconds / (2.0 if dt == np.float32 else 1.0)
+    reps64, el64 = timed(lambda: O.thomas(a, b, c, d, n=n, threads=threads), share)
     xref = O.thomas(a, b, c, d, n=n, threads=threads)
     rel, rms, fails = O.compare(x_gpu_np, xref, n=n, rms_eps=1e-3)
-    return {
-        "value": reps * n * N / el,
+    out = {
+        "value": reps64 * n * N / el64,
         "unit": "unknowns/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle Thomas ({args.dtype} arithmetic), full {N}x{n} batch "
-                   f"x {reps} passes, {el:.1f} s, OpenMP {threads} threads"),
+        "sample": (f"oracle Thomas in FP64 arithmetic ({args.dtype} inputs), full {N}x{n} "
+                   f"batch x {reps64} passes, {el64:.1f} s, OpenMP {threads} threads"),
         "gpu_max_rel_err_vs_fp64_thomas": float(np.max(rel)),
         "gpu_reference_rms_failures": int(fails),
     }
+    if dt == np.float32:
+        reps32, el32 = timed(lambda: O.thomas_f32(a, b, c, d, n, threads), share)
+        out["t_precision"] = {
+            "value": reps32 * n * N / el32, "unit": "unknowns/s", "cores": threads,
+            "sample": (f"oracle Thomas in FP32 arithmetic, full {N}x{n} batch x {reps32} "
+                       f"passes, {el32:.1f} s, OpenMP {threads} threads")}
+    return out
 
 
 def ragged_cpu_baseline(args, host, offs, x_gpu):
@@ -739,15 +767,17 @@ def _percentile(v, q):
     return float(np.percentile(np.asarray(v, dtype=np.float64), q)) if len(v) else None
 
 
-def bench_serve(args):
-    """C5: the request server under Poisson arrivals of requests of 1..4,096
-    systems with sizes log-uniform in [32, 16,384] (FP64 host data, FP32 solve
-    + 2 FP64 refinements), interleaved with ADI steps (r ~ U[0.1, 10]) on
-    resident 2048^2 and 4096^2 grids. Reports served throughput at saturation
-    (all requests submitted at once) against plain per-size FP64 launches of
-    the same requests, and p50/p99 latency at 70% of the saturated rate. The
-    request arrays live in the server's page-locked memory (the DMA path);
-    the saturated rate with pageable client arrays is reported beside it."""
+def serve_measure(n_requests: int, pageable: bool = True, poisson_frac: float = 0.7):
+    """C5 on the request server: `n_requests` requests of 1..4,096 systems
+    with sizes log-uniform in [32, 16,384] (FP64 host data, FP32 solve + 2
+    FP64 refinements), slices of one seeded pool. Measures the served rate at
+    saturation (all requests submitted at once, arrays in the server's
+    page-locked memory, and with `pageable` also from ordinary memory), the
+    same requests as plain per-size FP64 launches back to back, and p50/p99
+    latency of Poisson arrivals at `poisson_frac` of the saturated request
+    rate, interleaved with ADI steps (r ~ U[0.1, 10]) on resident 2048^2 and
+    4096^2 grids. The relative residual of every served request is computed
+    on the device. Returns a dict of the figures."""
     import threading
 
     import numpy as np
@@ -766,7 +796,7 @@ def bench_serve(args):
     host_pageable = [v.cpu().numpy() for v in pool]
     host = host_pageable
     reqs = []
-    while len(reqs) < args.requests:
+    while len(reqs) < n_requests:
         k = int(rng.integers(1, 4097))
         sizes = R.log_uniform_sizes(k, 32, 16384, seed=int(rng.integers(1 << 30)))
         tot = int(sizes.sum())
@@ -782,8 +812,7 @@ def bench_serve(args):
 
     srv = S.Server(hbm_budget_bytes=16 << 30, slots=3, max_batch_unknowns=max_req,
                    batch_window_us=300)
-    xs_pageable = [np.empty(r[2]) for r in reqs]
-    xs = xs_pageable
+    xs = [np.empty(r[2]) for r in reqs]
     # warm-up (kernel load, first-touch of staging)
     offs0, arr0 = req_arrays(0)
     srv.wait(srv.submit(arr0[3], xs[0], offs0, *arr0[:3]))
@@ -802,7 +831,7 @@ def bench_serve(args):
 
     # client arrays in ordinary (pageable) memory: staged through pinned
     # buffers by the server
-    sat_page_s = saturate()
+    sat_page_s = saturate() if pageable else None
     # client arrays in the server's page-locked memory: DMA with no staging
     host = [srv.host_array(pool_n) for _ in range(4)]
     for dst, src in zip(host, host_pageable):
@@ -816,15 +845,16 @@ def bench_serve(args):
     sat_s = saturate()
     served_rate = total_unknowns / sat_s
 
-    # accuracy of the served results: device residual of every 4th request
+    # accuracy of the served results: device residual of every request
     worst_res = 0.0
-    for i in range(0, len(reqs), 4):
+    for i in range(len(reqs)):
         offs, start, tot = reqs[i]
         d_offs = torch.from_numpy(offs).to(dev)
         x = torch.from_numpy(xs[i]).to(dev)
         rr = torch.empty(len(offs) - 1, dtype=torch.float64, device=dev)
         R.residual(*(v[start:start + tot] for v in pool), x, d_offs, len(offs) - 1, rr)
         worst_res = max(worst_res, float(rr.max()))
+    del pool
 
     # --- plain per-size FP64 launches of the same requests, back to back ----------
     h = T.Handle()
@@ -851,15 +881,16 @@ def bench_serve(args):
     torch.cuda.synchronize()
     plain_s = time.perf_counter() - t0
     plain_rate = total_unknowns / plain_s
+    h.close()
 
-    # --- Poisson arrivals at 70% of the saturated request rate, with ADI ----------
-    lam = 0.7 * len(reqs) / sat_s
+    # --- Poisson arrivals at a fraction of the saturated request rate, with ADI -
+    lam = poisson_frac * len(reqs) / sat_s
     gaps = rng.exponential(1.0 / lam, size=len(reqs))
     grids = {}
     for n in (2048, 4096):
         grids[n] = srv.create_adi(np.zeros(n * n) + 1.0)
     adi_period = 4.0 / lam
-    lat, adi_lat = [], {2048: [], 4096: []}
+    adi_lat = {2048: [], 4096: []}
     stop = threading.Event()
 
     def adi_client():
@@ -889,36 +920,78 @@ def bench_serve(args):
     for gid in grids.values():
         srv.release_adi(gid)
     srv.close()
-    h.close()
+
+    def pct(v, q):
+        return _percentile(v, q) if v else None
+
+    return {
+        "requests": len(reqs), "unknowns": total_unknowns, "slots": 3,
+        "budget_bytes": 16 << 30,
+        "sat_s": sat_s, "served_rate": served_rate, "sat_page_s": sat_page_s,
+        "max_rel_residual": worst_res, "plain_s": plain_s, "plain_rate": plain_rate,
+        "lam": lam, "poisson_s": poisson_s,
+        "latency_ms_p50": pct(lat, 50), "latency_ms_p99": pct(lat, 99),
+        "adi_2048_step_ms_p50": pct([v / 1e3 for v in adi_lat[2048]], 50),
+        "adi_2048_step_ms_p99": pct([v / 1e3 for v in adi_lat[2048]], 99),
+        "adi_4096_step_ms_p50": pct([v / 1e3 for v in adi_lat[4096]], 50),
+        "adi_4096_step_ms_p99": pct([v / 1e3 for v in adi_lat[4096]], 99),
+        "adi_steps": sum(len(v) for v in adi_lat.values()),
+        "server_stats": st,
+    }
+
+
+SERVE_WORKLOAD = ("served: requests of 1..4096 systems, sizes log-uniform [32, 16384], "
+                  "FP32 solve + 2 FP64 refinements; ADI 2048^2 and 4096^2 resident")
+
+
+def served_summary(m: dict) -> dict:
+    """The compact served object of the default benchmark line."""
+    return {
+        "workload": SERVE_WORKLOAD, "requests": m["requests"], "unknowns": m["unknowns"],
+        "unknowns_per_s": m["served_rate"],
+        "plain_per_size_unknowns_per_s": m["plain_rate"],
+        "speedup_vs_plain_per_size": m["plain_s"] / m["sat_s"],
+        "poisson_rate_req_per_s": m["lam"],
+        "latency_ms_p50": m["latency_ms_p50"], "latency_ms_p99": m["latency_ms_p99"],
+        "adi_2048_step_ms_p50": m["adi_2048_step_ms_p50"],
+        "adi_2048_step_ms_p99": m["adi_2048_step_ms_p99"],
+        "adi_4096_step_ms_p50": m["adi_4096_step_ms_p50"],
+        "adi_4096_step_ms_p99": m["adi_4096_step_ms_p99"],
+        "max_rel_residual": m["max_rel_residual"],
+        "note": "host FP64 arrays in the server's page-locked memory, PCIe copies included; "
+                "plain = the same requests as per-size FP64 CR-PCR launches (FP64 accuracy)",
+    }
+
+
+def bench_serve(args):
+    """C5: the request server (serve_measure) with --requests requests; one
+    JSON line, value = the served unknowns/s at saturation."""
+    m = serve_measure(args.requests)
     out = {
-        "metric": METRIC, "value": served_rate, "unit": "unknowns/s", "n_gpus": 1,
-        "steps": len(reqs), "warmup": 1, "ms_per_step": sat_s * 1e3 / len(reqs),
+        "metric": METRIC, "value": m["served_rate"], "unit": "unknowns/s", "n_gpus": 1,
+        "steps": m["requests"], "warmup": 1, "ms_per_step": m["sat_s"] * 1e3 / m["requests"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded device generator, FP64 host copies; request sizes seeded)",
-        "config": {"workload": f"served: {len(reqs)} requests of 1..4096 systems, sizes "
-                               f"log-uniform [32, 16384], FP32 solve + 2 FP64 refinements; "
-                               f"ADI 2048^2 and 4096^2 resident",
-                   "unknowns": total_unknowns, "slots": 3, "budget_bytes": 16 << 30},
-        "served_saturated": {"unknowns_per_s": served_rate, "seconds": sat_s,
-                             "max_rel_residual_sampled": worst_res,
+        "config": {"workload": f"{SERVE_WORKLOAD} ({m['requests']} requests)",
+                   "unknowns": m["unknowns"], "slots": m["slots"],
+                   "budget_bytes": m["budget_bytes"]},
+        "served_saturated": {"unknowns_per_s": m["served_rate"], "seconds": m["sat_s"],
+                             "max_rel_residual": m["max_rel_residual"],
                              "client_memory": "server page-locked buffers (DMA, no staging)"},
-        "served_saturated_pageable": {"unknowns_per_s": total_unknowns / sat_page_s,
-                                      "seconds": sat_page_s,
+        "served_saturated_pageable": {"unknowns_per_s": m["unknowns"] / m["sat_page_s"],
+                                      "seconds": m["sat_page_s"],
                                       "client_memory": "pageable (staged by the server)"},
-        "plain_per_size": {"unknowns_per_s": plain_rate, "seconds": plain_s},
-        "speedup_vs_plain_per_size": plain_s / sat_s,
-        "poisson": {"rate_req_per_s": lam, "seconds": poisson_s,
-                    "latency_ms_p50": _percentile(lat, 50), "latency_ms_p99": _percentile(lat, 99),
-                    "adi_2048_step_ms_p50": _percentile(adi_lat[2048], 50) / 1e3
-                    if adi_lat[2048] else None,
-                    "adi_2048_step_ms_p99": _percentile(adi_lat[2048], 99) / 1e3
-                    if adi_lat[2048] else None,
-                    "adi_4096_step_ms_p50": _percentile(adi_lat[4096], 50) / 1e3
-                    if adi_lat[4096] else None,
-                    "adi_4096_step_ms_p99": _percentile(adi_lat[4096], 99) / 1e3
-                    if adi_lat[4096] else None,
-                    "adi_steps": sum(len(v) for v in adi_lat.values())},
-        "server_stats": st,
+        "plain_per_size": {"unknowns_per_s": m["plain_rate"], "seconds": m["plain_s"]},
+        "speedup_vs_plain_per_size": m["plain_s"] / m["sat_s"],
+        "poisson": {"rate_req_per_s": m["lam"], "seconds": m["poisson_s"],
+                    "latency_ms_p50": m["latency_ms_p50"],
+                    "latency_ms_p99": m["latency_ms_p99"],
+                    "adi_2048_step_ms_p50": m["adi_2048_step_ms_p50"],
+                    "adi_2048_step_ms_p99": m["adi_2048_step_ms_p99"],
+                    "adi_4096_step_ms_p50": m["adi_4096_step_ms_p50"],
+                    "adi_4096_step_ms_p99": m["adi_4096_step_ms_p99"],
+                    "adi_steps": m["adi_steps"]},
+        "server_stats": m["server_stats"],
     }
     print(json.dumps(out), flush=True)
 
@@ -1138,6 +1211,11 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, x.cpu().numpy(), n, N)
     h.close()
+    if args.served_requests > 0 and bs == n:
+        # the served half of the metric, on the same GPU in the same run
+        del a, b, c, d, x
+        torch.cuda.empty_cache()
+        out["served"] = served_summary(serve_measure(args.served_requests, pageable=False))
     print(json.dumps(out), flush=True)
 
 

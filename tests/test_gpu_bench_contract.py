@@ -30,7 +30,7 @@ def _json(r):
 
 def test_uniform_line_has_roofline_and_cpu_baseline():
     j = _json(_run(["--n", "1024", "--systems", "2048", "--steps", "3", "--warmup", "1",
-                    "--cpu-seconds", "0.2"]))
+                    "--cpu-seconds", "0.2", "--served-requests", "4"]))
     for k in KEYS:
         assert k in j, k
     assert j["n_gpus"] == 1 and j["unit"] == "unknowns/s" and j["value"] > 0
@@ -40,7 +40,13 @@ def test_uniform_line_has_roofline_and_cpu_baseline():
     cb = j["cpu_baseline"]
     assert cb["kind"] in ("port", "reference") and cb["cores"] >= 1 and cb["value"] > 0
     assert cb["gpu_max_rel_err_vs_fp64_thomas"] <= 1e-4
+    assert "FP64" in cb["sample"] and cb["t_precision"]["value"] > 0   # FP64 oracle + FP32
     assert "workload" in j["config"]
+    sv = j["served"]                   # the served half of the metric
+    assert sv["requests"] == 4 and sv["unknowns_per_s"] > 0
+    assert sv["plain_per_size_unknowns_per_s"] > 0
+    assert sv["latency_ms_p50"] > 0 and sv["latency_ms_p99"] >= sv["latency_ms_p50"]
+    assert sv["max_rel_residual"] <= 1e-12
     sc = j["stream_ceiling"]           # plain stream of the same bytes
     assert sc["us"] > 0 and sc["bytes"] == 5 * 1024 * 2048 * 4
 
@@ -52,7 +58,9 @@ def test_adi_and_pivoting_lines():
     for k in KEYS:
         assert k in j, k
     assert j["dtype"] == "f64" and j["value"] > 0 and j["roofline"]["frac"] > 0
-    assert j["cpu_baseline"]["gpu_max_rel_err_vs_oracle_step"] <= 1e-11
+    assert j["cpu_baseline"]["gpu_max_rel_err_vs_oracle_step"] <= 1e-12
+    by_grid = j["cpu_baseline"]["gpu_max_rel_err_vs_oracle_step_by_grid"]
+    assert set(by_grid) == {"2048", "4096"} and max(by_grid.values()) <= 1e-12
     j = _json(_run(["--workload", "pivoting", "--n", "512", "--systems", "4096",
                     "--steps", "2", "--warmup", "1", "--cpu-seconds", "0.2"]))
     for k in KEYS:

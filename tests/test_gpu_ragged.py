@@ -17,6 +17,12 @@ EDGE_SIZES_FP64 = [n for n in EDGE_SIZES if n <= 8192]
 MODES = ["refine", "fp64"]
 
 
+def _table(mode):
+    """The shipped autotune table of the mode (tune/ragged_f64_refine.json or
+    tune/ragged_f64_fp64.json)."""
+    return "default-fp64" if mode == "fp64" else "default"
+
+
 def _solve(plan, dev, x, mode, **kw):
     if mode == "fp64":
         plan.solve(*dev, x, fp64=True, **kw)
@@ -57,7 +63,7 @@ def test_ragged_matches_oracle(oracle, trid, rg, kind, mode):
     sizes = np.concatenate([edge, rg.log_uniform_sizes(1500, 16, 8192, seed=5)])
     offs, host, dev = _batch(oracle, rg, sizes, kind=kind)
     x = torch.full_like(dev[0], float("nan"))
-    plan = rg.RaggedPlan(offs)
+    plan = rg.RaggedPlan(offs, table=_table(mode))
     info = plan.info()
     print(info, flush=True)
     assert info["num_systems"] == len(sizes)
@@ -93,7 +99,7 @@ def test_ragged_alone_equals_in_the_mix(oracle, trid, rg, mode):
     sizes = np.concatenate([edge, rg.log_uniform_sizes(300, 2, 8192, seed=9)])
     offs, host, dev = _batch(oracle, rg, sizes, seed=4)
     x = torch.full_like(dev[0], float("nan"))
-    plan = rg.RaggedPlan(offs)
+    plan = rg.RaggedPlan(offs, table=_table(mode))
     _solve(plan, dev, x, mode, graph=False)
     torch.cuda.synchronize()
     xm = x.cpu().numpy()
@@ -102,7 +108,7 @@ def test_ragged_alone_equals_in_the_mix(oracle, trid, rg, mode):
         o, e = int(offs[s]), int(offs[s + 1])
         one = [v[o:e].contiguous() for v in dev]
         xs = torch.full_like(one[0], float("nan"))
-        p1 = rg.RaggedPlan(np.array([0, e - o], dtype=np.int64))
+        p1 = rg.RaggedPlan(np.array([0, e - o], dtype=np.int64), table=_table(mode))
         _solve(p1, one, xs, mode, graph=False)
         torch.cuda.synchronize()
         p1.close()
@@ -171,3 +177,29 @@ def test_every_tuner_candidate_is_accurate(oracle, trid, rg, mode):
             torch.cuda.synchronize()
             plan.close()
             _check_ragged(oracle, offs, host, x.cpu().numpy(), 1e-11)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_ragged_100k_full_size(oracle, trid, rg, mode):
+    """The configured ragged workload at its full size: 100,000 systems with
+    sizes log-uniform in [16, 8192] (the benchmark's batch), each mode with
+    its shipped table, every system against FP64 Thomas and its residual."""
+    import torch
+    sizes = np.sort(rg.log_uniform_sizes(100000, 16, 8192, seed=1))
+    offs = rg.offsets_of(sizes)
+    total = int(offs[-1])
+    print(f"100k ragged {mode}: {total} unknowns", flush=True)
+    dev = [torch.empty(total, dtype=torch.float64, device="cuda") for _ in range(4)]
+    trid.generate(0, 2024, *dev, total, 1)   # a[first], c[last] of each system are ignored
+    x = torch.full_like(dev[0], float("nan"))
+    plan = rg.RaggedPlan(offs, table=_table(mode))
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        _solve(plan, dev, x, mode, stream=s)
+    s.synchronize()
+    plan.close()
+    host = [v.cpu().numpy() for v in dev]
+    rel, res = _check_ragged(oracle, offs, host, x.cpu().numpy(), 1e-11)
+    print(f"100k ragged {mode}: max rel err {rel:.2e}, max rel residual {res:.2e}", flush=True)
+    del dev, x
+    torch.cuda.empty_cache()
