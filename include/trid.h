@@ -174,6 +174,29 @@ int trid_solve_f64(trid_handle handle, int method,
                    const trid_opts* opts, void* stream);
 
 /*
+ * Solve without device allocations. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). It is
+ * trid_solve_f32 (elemSize 4, float arrays) or trid_solve_f64 (elemSize 8,
+ * double arrays) without a handle: same methods, options, kernels and
+ * results. The workspace some plans need (cr / crpcr by separators past the
+ * register sizes, the two-pass pcrGauss form) is one fixed device array of
+ * TRID_NOALLOC_SCRATCH_BYTES per device, part of the library and held until
+ * the process ends; a request that needs more returns TRID_ENOMEM before
+ * anything is enqueued (64 MiB holds the separator records of 64 systems of
+ * 2^20 unknowns in FP32 or FP64, and the two-pass pcrGauss workspace of 2^22
+ * FP32 / 2^21 FP64 unknowns). The call allocates nothing; its first call on
+ * a device looks up that device's array address. Every call on one device
+ * shares the array and finds the previous call's contents (no kernel here
+ * needs it cleared): calls that overlap on one device (different streams or
+ * threads) race on it and give wrong results, so order them on one stream.
+ */
+#define TRID_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int trid_solve_noalloc(int method, int elemSize, const void* a, const void* b, const void* c,
+                       const void* d, void* x, int64_t systemSize, int64_t numSystems,
+                       const trid_opts* opts, void* stream);
+
+/*
  * FP32 solve with FP64 iterative refinement. No counterpart in the reference
  * (its element type is fixed at build time, common.h:5-12). a,b,c,d,x are
  * FP64 device arrays; the solves run in FP32 with `method` (TRID_CR,

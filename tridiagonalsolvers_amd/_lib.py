@@ -27,7 +27,7 @@ EXPORTS = (
     "trid_pcrgauss_f32", "trid_pcrgauss_f64",
     "trid_create", "trid_destroy", "trid_solve_f32", "trid_solve_f64",
     "trid_generate_f32", "trid_generate_f64", "trid_plan_query",
-    "trid_solve_refined_f64", "trid_residual_f64",
+    "trid_solve_refined_f64", "trid_residual_f64", "trid_solve_noalloc",
     "trid_solve_interleaved_f32", "trid_solve_interleaved_f64",
     "trid_solve_strided_f32", "trid_solve_strided_f64",
     "trid_solve_pivoting_f32", "trid_solve_pivoting_f64",
@@ -177,6 +177,8 @@ def lib() -> ctypes.CDLL:
         v = getattr(L, f"trid_solve_strided_{t}")
         v.argtypes = [P, i32, P, P, P, P, P, i64, i64, i64, ctypes.POINTER(Opts), P]
         v.restype = i32
+    L.trid_solve_noalloc.argtypes = [i32, i32, P, P, P, P, P, i64, i64, ctypes.POINTER(Opts), P]
+    L.trid_solve_noalloc.restype = i32
     L.trid_solve_refined_f64.argtypes = [P, i32, P, P, P, P, P, i64, i64, i32,
                                          ctypes.POINTER(Opts), P]
     L.trid_solve_refined_f64.restype = i32

@@ -22,6 +22,7 @@ struct trid_context {
     int device;
     void* ws;         // workspace: two-pass pcrGauss, separator form, interleaved, pivoting
     size_t ws_bytes;
+    bool fixed;       // ws is the library's fixed scratch array: never grown or freed
 };
 
 namespace {
@@ -34,6 +35,7 @@ int ensure_ws(trid_context* h, size_t bytes)
 {
     if (!h) return TRID_EINVAL;
     if (h->ws_bytes >= bytes) return TRID_OK;
+    if (h->fixed) return TRID_ENOMEM;  // the no-allocation entry point never allocates
     if (h->ws) {
         (void)hipFree(h->ws);
         h->ws = nullptr;
@@ -83,6 +85,18 @@ int check_device()
 bool aligned16(const void* p)
 {
     return (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
+}
+
+// Device workspace the plan's kernels need (0 for the register, LDS and
+// in-LDS pcrGauss forms).
+template <typename T>
+size_t plan_ws_bytes(const trid::Plan& p, int method, int64_t n, int64_t N)
+{
+    if (p.family == trid::Family::PcrThomas && p.two_pass)
+        return (size_t)4 * (size_t)n * (size_t)N * sizeof(T);
+    if (p.family == trid::Family::Part)
+        return trid::part_ws_bytes<T>(method == TRID_CR ? TRID_CR : TRID_CRPCR, n, N);
+    return 0;
 }
 
 template <typename T>
@@ -153,7 +167,7 @@ int solve_impl(trid_handle h, int method, const T* a, const T* b, const T* c, co
             g.pad = p.pad;
             g.two_pass = p.two_pass;
             if (p.two_pass) {
-                st = ensure_ws(h, (size_t)4 * (size_t)n * (size_t)N * sizeof(T));
+                st = ensure_ws(h, plan_ws_bytes<T>(p, method, n, N));
                 if (st != TRID_OK) return st;
                 g.ws = static_cast<T*>(h->ws);
             }
@@ -162,7 +176,7 @@ int solve_impl(trid_handle h, int method, const T* a, const T* b, const T* c, co
         }
         case trid::Family::Part: {
             const int m = method == TRID_CR ? TRID_CR : TRID_CRPCR;
-            st = ensure_ws(h, trid::part_ws_bytes<T>(m, n, N));
+            st = ensure_ws(h, plan_ws_bytes<T>(p, method, n, N));
             if (st != TRID_OK) return st;
             trid::PartArgs<T> g{};
             g.a = a;
@@ -246,6 +260,61 @@ int strided_impl(trid_context* h, int method, const T* a, const T* b, const T* c
     g.io = p.io;
     g.stream = stream;
     return from_hip(g.L <= 64 ? trid::launch_reg_crpcr<T>(g) : trid::launch_reg_mw<T>(g));
+}
+
+// ---- solve without device allocations -------------------------------------
+// The workspace of trid_solve_noalloc: one fixed array per device, loaded
+// with the library's code object and held until the process ends. In host
+// code `scratch` names the runtime's host-side stand-in, not device memory;
+// kernels get the device address that hipGetSymbolAddress reports for the
+// current device (looked up on the first call on each device, kept here).
+__device__ __align__(256) char scratch[TRID_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];
+
+int scratch_ptr(void** out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return TRID_ENODEV;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return TRID_EHIP;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return TRID_OK;
+}
+
+// trid_solve_* with the workspace taken from `scratch`: the size check comes
+// first (a request that needs more than the array fails with TRID_ENOMEM
+// before anything touches the device), then the same kernels as the handle
+// path run with the array as their workspace.
+template <typename T>
+int noalloc_impl(int method, const T* a, const T* b, const T* c, const T* d, T* x, int64_t n,
+                 int64_t N, const trid_opts* opts, hipStream_t stream)
+{
+    trid_opts o{};
+    if (opts) o = *opts;
+    if (!a || !b || !c || !d || !x) return TRID_EINVAL;
+    if (n < 2 || N < 0) return TRID_EINVAL;
+    if (o.flags & ~TRID_FLAG_TWO_PASS) return TRID_EINVAL;
+    if (N == 0) return TRID_OK;
+    trid::Plan p;
+    int st = trid::make_plan(method, n, N, (int)sizeof(T), o, &p);
+    if (st != TRID_OK) return st;
+    if (plan_ws_bytes<T>(p, method, n, N) > TRID_NOALLOC_SCRATCH_BYTES) return TRID_ENOMEM;
+    st = check_device();
+    if (st != TRID_OK) return st;
+    trid_context ctx{};
+    st = scratch_ptr(&ctx.ws);
+    if (st != TRID_OK) return st;
+    (void)hipGetDevice(&ctx.device);
+    ctx.ws_bytes = TRID_NOALLOC_SCRATCH_BYTES;
+    ctx.fixed = true;
+    return solve_impl<T>(&ctx, method, a, b, c, d, x, n, N, &o, stream);
 }
 
 // Host-pointer drop-in: copy in, one solve on the null stream, copy out.
@@ -547,6 +616,24 @@ int trid_solve_f64(trid_handle handle, int method, const double* a, const double
 {
     return set_status(solve_impl<double>(handle, method, a, b, c, d, x, systemSize, numSystems,
                                          opts, static_cast<hipStream_t>(stream)));
+}
+
+int trid_solve_noalloc(int method, int elemSize, const void* a, const void* b, const void* c,
+                       const void* d, void* x, int64_t systemSize, int64_t numSystems,
+                       const trid_opts* opts, void* stream)
+{
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (elemSize == 4)
+        return set_status(noalloc_impl<float>(
+            method, static_cast<const float*>(a), static_cast<const float*>(b),
+            static_cast<const float*>(c), static_cast<const float*>(d), static_cast<float*>(x),
+            systemSize, numSystems, opts, s));
+    if (elemSize == 8)
+        return set_status(noalloc_impl<double>(
+            method, static_cast<const double*>(a), static_cast<const double*>(b),
+            static_cast<const double*>(c), static_cast<const double*>(d),
+            static_cast<double*>(x), systemSize, numSystems, opts, s));
+    return set_status(TRID_EINVAL);
 }
 
 int trid_solve_refined_f64(trid_handle handle, int method, const double* a, const double* b,
