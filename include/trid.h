@@ -348,9 +348,19 @@ typedef struct trid_ragged_plan_s* trid_ragged_plan;
  * holds larger ones). */
 #define TRID_RAGGED_FP64 2
 
+/* Reads an autotune table file: the JSON that tridiagonalsolvers_amd.autotune
+ * writes (tune/ragged_f64_refine.json, tune/ragged_f64_fp64.json), an object
+ * whose "buckets" array holds objects with integer "size", "per_lane",
+ * "lanes" and "rest" (other keys are skipped). The first `capacity` buckets
+ * go to table[], *count receives how many the file holds. Each entry is
+ * checked as trid_ragged_plan_create checks it. TRID_EINVAL for a file that
+ * cannot be read, malformed JSON, a bucket without one of the four keys or
+ * an invalid configuration. Host only (no device needed). */
+int trid_bucket_table_load(const char* path, trid_bucket_cfg* table, int capacity, int* count);
+
 /* offsets: host array of numSystems+1 element offsets. table: optional
- * per-bucket configurations (e.g. from the autotuner); buckets not listed use
- * the library default. */
+ * per-bucket configurations (e.g. from the autotuner, or read by
+ * trid_bucket_table_load); buckets not listed use the library default. */
 int trid_ragged_plan_create(trid_ragged_plan* plan, const int64_t* offsets, int64_t numSystems,
                             const trid_bucket_cfg* table, int tableSize);
 int trid_ragged_plan_destroy(trid_ragged_plan plan);

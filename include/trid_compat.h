@@ -68,6 +68,22 @@ inline void tridiagonalMKL(double* a, double* b, double* c, double*& d, double*&
     d = t;
 }
 
+/* tridiagonalMKL.cpp:28: the same per-system ?gtsv as tridiagonalMKL, spread
+ * by the reference over 4 OpenMP threads that each take num_systems / 4
+ * systems (the last num_systems mod 4 are left unsolved). Here every system
+ * is solved, by the same GPU pivoting solve as tridiagonalMKL, with the same
+ * in-d result and d/x pointer swap. */
+inline void multiThreadTridiagonalMKL(float* a, float* b, float* c, float*& d, float*& x,
+                                      int system_size, int num_systems)
+{
+    tridiagonalMKL(a, b, c, d, x, system_size, num_systems);
+}
+inline void multiThreadTridiagonalMKL(double* a, double* b, double* c, double*& d, double*& x,
+                                      int system_size, int num_systems)
+{
+    tridiagonalMKL(a, b, c, d, x, system_size, num_systems);
+}
+
 #undef TRID_COMPAT_7
 #undef TRID_COMPAT_8
 

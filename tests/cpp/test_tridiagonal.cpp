@@ -7,7 +7,8 @@
 //
 // Unlike the reference driver (which always returns EXIT_SUCCESS and runs
 // only pcrGaussWrapper(..., 64)), every method runs -- including the
-// partial-pivoting MKL baseline's name, tridiagonalMKL -- and the process
+// partial-pivoting MKL baseline's names, tridiagonalMKL and
+// multiThreadTridiagonalMKL -- and the process
 // exits with the number of failed (method, system) pairs, capped at 255.
 //
 // Test infrastructure: links the CPU oracle (oracle/liboracle.so) as the
@@ -79,6 +80,7 @@ static int run(int numSystems, int systemSize)
     }
     for (int g = 1; g <= n_pad / 2 && g <= 512; g *= 2) cases.push_back({"pcrGaussWrapper", g});
     cases.push_back({"tridiagonalMKL", 0});
+    cases.push_back({"multiThreadTridiagonalMKL", 0});
     std::vector<T> dwork(total);
 
     for (const Case& k : cases) {
@@ -94,13 +96,17 @@ static int run(int numSystems, int systemSize)
         else if (k.name == "pcrGaussWrapper")
             pcrGaussWrapper(a, b, c, d, x.data(), systemSize, numSystems, k.param);
         else {
-            // solves in the d buffer and swaps the pointers (tridiagonalMKL.cpp:22-25)
+            // solves in the d buffer and swaps the pointers (tridiagonalMKL.cpp:22-25,
+            // 56-59)
             dwork = arr[3];
             T* dp = dwork.data();
             T* xp = x.data();
-            tridiagonalMKL(a, b, c, dp, xp, systemSize, numSystems);
+            if (k.name == "tridiagonalMKL")
+                tridiagonalMKL(a, b, c, dp, xp, systemSize, numSystems);
+            else
+                multiThreadTridiagonalMKL(a, b, c, dp, xp, systemSize, numSystems);
             if (trid_last_status() == TRID_OK && (xp != dwork.data() || dp != x.data())) {
-                std::printf("tridiagonalMKL did not swap d and x\n");
+                std::printf("%s did not swap d and x\n", k.name.c_str());
                 failures += numSystems;
                 continue;
             }

@@ -281,3 +281,50 @@ def test_device_tensors_are_checked_before_the_c_abi(trid):
         h.solve_pivoting(*(FakeGpu(640) for _ in range(5)), 64, 10,
                          info=FakeGpu(10, dtype=torch.int64))
     h.close()
+
+
+def test_bucket_table_loader_reads_the_shipped_tables(trid, tmp_path):
+    """trid_bucket_table_load (the C reader of the autotune JSON) gives the
+    same entries as the Python reader for both shipped tables, and refuses
+    malformed files, missing keys and invalid configurations."""
+    import json
+
+    import tridiagonalsolvers_amd.ragged as R
+    from tridiagonalsolvers_amd._lib import BucketCfg
+    lib = trid.lib()
+
+    def load(path, cap=32):
+        arr = (BucketCfg * max(cap, 1))()
+        cnt = ctypes.c_int(-1)
+        st = lib.trid_bucket_table_load(str(path).encode(), ctypes.cast(arr, ctypes.c_void_p),
+                                        cap, ctypes.byref(cnt))
+        return st, cnt.value, [dict(size=arr[i].size, per_lane=arr[i].per_lane,
+                                    lanes=arr[i].lanes, rest=arr[i].rest)
+                               for i in range(min(cnt.value, cap))]
+
+    for path in (R.DEFAULT_TABLE, R.DEFAULT_TABLE_FP64):
+        st, cnt, got = load(path)
+        assert st == 0 and cnt >= 10
+        assert got == R.load_table(path)
+        st, cnt2, part = load(path, cap=3)          # count is the file's, 3 stored
+        assert st == 0 and cnt2 == cnt and part == got[:3]
+    good = {"device": "x \"quoted\" \\u00e9", "nested": {"a": [1, 2.5e3, True, None]},
+            "buckets": [{"size": 64, "per_lane": 8, "lanes": 8, "rest": 8, "ms": 0.1}]}
+    p = tmp_path / "good.json"
+    p.write_text(json.dumps(good))
+    assert load(p)[:2] == (0, 1)
+    bad = [
+        "",                                                         # empty
+        "[1, 2]",                                                   # not an object
+        '{"buckets": [{"size": 64, "per_lane": 8, "lanes": 8}]}',   # rest missing
+        '{"buckets": [{"size": 64, "per_lane": 8, "lanes": 4, "rest": 4}]}',  # 8*4 != 64
+        '{"buckets": [{"size": 64.5, "per_lane": 8, "lanes": 8, "rest": 8}]}',
+        '{"buckets": [{"size": 64, "per_lane": 8, "lanes": 8, "rest": 8}]',   # unterminated
+        '{"other": 1}',                                             # no buckets
+        '{"buckets": [{"size": 64, "per_lane": 8, "lanes": 8, "rest": 8}]} x',  # trailing text
+    ]
+    for k, text in enumerate(bad):
+        q = tmp_path / f"bad{k}.json"
+        q.write_text(text)
+        assert load(q)[0] == 1, text
+    assert load(tmp_path / "missing.json")[0] == 1

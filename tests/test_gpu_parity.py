@@ -317,3 +317,30 @@ def test_multiwave_shapes_and_io(oracle, trid, dtype):
         torch.cuda.synchronize()
         check(oracle, x.cpu().numpy(), a, b, c, d, 1024, dtype)
     h.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("kind", ["dominant", "weak"])
+def test_4096x8192_full_size(oracle, trid, kind, dtype):
+    """4,096 systems x 8,192 unknowns (one system per workgroup, several
+    waves each), strongly and weakly (b = |a|+|c|+U[0.01,0.1]) dominant,
+    whole batch against FP64 Thomas; CR-PCR and CR."""
+    import torch
+    n, N = 8192, 4096
+    k = 0 if kind == "dominant" else 1
+    tdt = torch.float32 if dtype == np.float32 else torch.float64
+    dev = [torch.empty(n * N, dtype=tdt, device="cuda") for _ in range(4)]
+    trid.generate(k, 2, *dev, n, N)
+    a, b, c, d = oracle.generate(kind, 2, n, N, dtype)
+    tol = TOL[dtype] * (10 if kind == "weak" else 1)
+    h = trid.Handle()
+    for method in ("crpcr", "cr"):
+        x = torch.full_like(dev[0], float("nan"))
+        h.solve(method, *dev, x, n, N)
+        torch.cuda.synchronize()
+        worst = check(oracle, x.cpu().numpy(), a, b, c, d, n, dtype, tol)
+        print(f"4096x8192 {kind} {np.dtype(dtype).name} {method}: max rel err {worst:.3e}",
+              flush=True)
+    h.close()
+    del dev
+    torch.cuda.empty_cache()

@@ -9,7 +9,9 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtrid.so")
+# TRID_LIBRARY selects another build of the same ABI (the LDS-poisoning
+# debug library libtrid_lds_poison.so); default: the in-tree libtrid.so
+LIB_PATH = os.environ.get("TRID_LIBRARY") or os.path.join(HERE, "libtrid.so")
 
 # trid_status
 OK, EINVAL, ENOMEM, EHIP, ENOTSUP, ENODEV, EAGAIN = range(7)
@@ -34,7 +36,7 @@ EXPORTS = (
     "trid_solve_pivoting_interleaved_f32", "trid_solve_pivoting_interleaved_f64",
     "trid_gtsv_f32", "trid_gtsv_f64",
     "trid_ragged_plan_create", "trid_ragged_plan_destroy", "trid_ragged_plan_info",
-    "trid_ragged_solve_f64",
+    "trid_ragged_solve_f64", "trid_bucket_table_load",
     "trid_server_create", "trid_server_destroy", "trid_server_register_operator",
     "trid_server_release_operator", "trid_server_submit", "trid_server_wait",
     "trid_server_cancel", "trid_server_get_stats", "trid_server_host_alloc",
@@ -190,6 +192,8 @@ def lib() -> ctypes.CDLL:
     L.trid_ragged_plan_destroy.restype = i32
     L.trid_ragged_plan_info.argtypes = [P, ctypes.POINTER(RaggedInfo), P, P, i32]
     L.trid_ragged_plan_info.restype = i32
+    L.trid_bucket_table_load.argtypes = [ctypes.c_char_p, P, i32, ctypes.POINTER(i32)]
+    L.trid_bucket_table_load.restype = i32
     L.trid_ragged_solve_f64.argtypes = [P, P, P, P, P, P, i32, i32, P]
     L.trid_ragged_solve_f64.restype = i32
     L.trid_server_create.argtypes = [ctypes.POINTER(P), ctypes.POINTER(ServerOpts)]

@@ -1,7 +1,9 @@
 """Resident ADI grid (trid_adi_* in include/trid.h): Peaceman-Rachford steps
-of the 2D heat equation on an n x n FP64 device grid, both sweeps on the
-constant-coefficient register CR-PCR kernel (the column sweep on the
-transposed grid)."""
+of the 2D heat equation on an n x n FP64 device grid. Every system of both
+sweeps has the operator tridiag(-r, 1+2r, -r): each call computes its LU
+factors once (closed-form pivots, resident beside the grid) and applies them
+to the rows and, on the transposed grid, to the columns (kern_adi.hip's
+factored-operator kernel: the two substitutions as lane scans)."""
 from __future__ import annotations
 
 import ctypes

@@ -84,6 +84,7 @@ __global__ __launch_bounds__(MAXT) void adi_lu_kernel(const double* __restrict__
     extern __shared__ __align__(16) char trid_smem[];
     char* lds = trid_smem + w * Stage<double, kAdiM>::kWaveBytes;
     __shared__ double wmap[2][16][2];  // per wave: forward (A, B), backward (G, H)
+    lds_poison_wg(wmap, (unsigned)sizeof(wmap));
 
     // q of the row before the lane's first (0 for the system's first row)
     const double q_prev_wave = (lane == 0 && row0 > 0) ? Q[row0 - 1] : 0.0;
@@ -294,6 +295,7 @@ __global__ __launch_bounds__(256) void adi_rhs_cols_t_kernel(const double* __res
                                                              double r, int vec)
 {
     __shared__ double tile[kTile][kTile + 3];
+    lds_poison_wg(tile, (unsigned)sizeof(tile));
     const int t = threadIdx.x;
     const int cx = (t & 31) * 2;  // column pair of the tile
     const int ry = t / 32;        // 0..7
@@ -330,6 +332,7 @@ __global__ __launch_bounds__(256) void adi_transpose_kernel(const double* __rest
                                                             int vec)
 {
     __shared__ double tile[kTile][kTile + 1];
+    lds_poison_wg(tile, (unsigned)sizeof(tile));
     const int t = threadIdx.x;
     const int cx = (t & 31) * 2;
     const int ry = t / 32;

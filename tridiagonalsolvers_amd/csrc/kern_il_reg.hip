@@ -132,6 +132,7 @@ __global__ __launch_bounds__((IlGeom<T, MI>::S * kWave)) void il_reg_kernel(
 #pragma unroll
     for (int c = 0; c < G::NC; ++c) {
         __syncthreads();  // the previous chunk has been read back
+        lds_poison_wg(tile, (unsigned)sizeof(tile));
         {
             const int r = t / G::VPR;
             const int sb = (t % G::VPR) * G::EV;
@@ -168,6 +169,7 @@ __global__ __launch_bounds__((IlGeom<T, MI>::S * kWave)) void il_reg_kernel(
 
     for (int c = 0; c < G::NC; ++c) {
         __syncthreads();
+        lds_poison_wg(tile, (unsigned)sizeof(tile));
         if (lane / G::LC == c) {
             const int base = w * G::SP + (lane % G::LC) * G::GP;
 #pragma unroll

@@ -75,6 +75,7 @@ __global__ __launch_bounds__(256) void lds_crpcr_kernel(const T* __restrict__ A,
     const int nt = blockDim.x;
     const int lane = tid & (kWave - 1);
     const int64_t base = (int64_t)blockIdx.x * n;
+    lds_poison_wg(trid_smem, (unsigned)(4 * np * sizeof(T)));  // pad slots included
 
     // ---- load (coalesced), identity rows past n ---------------------------
     for (int i = tid; i < n_pad; i += nt) {

@@ -131,6 +131,7 @@ __global__ __launch_bounds__(256) void part_reduce_kernel(const T* __restrict__ 
     // chunk is loaded), then leaves as one contiguous block
     T* rl = reinterpret_cast<T*>(lds);
     __builtin_amdgcn_wave_barrier();
+    lds_poison_wave(lds, Stage<T, kPM>::kWaveBytes, lane);
     if (lane == kWave - 1) put_eq(rl, e[kPM - 1]);
     if (lane == 0) put_eq(rl + 4 * 11, e[0]);
 

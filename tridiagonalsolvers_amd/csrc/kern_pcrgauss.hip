@@ -48,6 +48,7 @@ __global__ __launch_bounds__(1024) void pcr_thomas_lds_kernel(const T* __restric
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
     const int64_t base = (int64_t)blockIdx.x * n;
+    lds_poison_wg(trid_smem, (unsigned)(4 * np * sizeof(T)));
 
     for (int i = tid; i < n_pad; i += nt) {
         Eq<T> e = identity_eq<T>();
@@ -170,6 +171,7 @@ __global__ __launch_bounds__(TRID_PCRG_THREADS) void pcr_global_levels_kernel(
     const int64_t sys = (int64_t)blockIdx.x / tiles;
     const int64_t base = sys * n;
     const int64_t t0 = ((int64_t)blockIdx.x - sys * tiles) * tile;
+    lds_poison_wg(trid_smem, (unsigned)(len * sizeof(Eq<T>)));
 
     Eq<T> own[KPT];
 #pragma unroll
@@ -435,6 +437,7 @@ __global__ __launch_bounds__(kStagedWaves * 64) void strided_staged_kernel(
     const int64_t r0 = chain0 % chains;
     const int64_t m0 = chain0 < blocks * chains ? (block_len - r0 + stride - 1) / stride : 0;
     const int nsteps = (int)(((m0 + 1) / 2 + kRowsBlk - 1) / kRowsBlk);
+    lds_poison_wg(slot, (unsigned)sizeof(slot));
 
     if (wave == 0) {
         // ---- compute wave ---------------------------------------------------

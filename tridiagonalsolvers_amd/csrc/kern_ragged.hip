@@ -112,6 +112,7 @@ __device__ __forceinline__ void transpose_rows(const double (&t)[M], int L, int 
         const int seg_row = lane - ll;
 #pragma unroll
         for (int h = 0; h < S::kRounds; ++h) {
+            lds_poison_wave(stage, S::kBytes, lane);
 #pragma unroll
             for (int q = 0; q < S::kQ; ++q) {
                 const int j = (h * S::kQ + q) * L + ll;
@@ -186,6 +187,7 @@ __device__ __forceinline__ void scatter_rows(double* __restrict__ X, int64_t bas
         asm volatile("" : "+v"(ll), "+v"(n));
 #pragma unroll
         for (int h = 0; h < S::kRounds; ++h) {
+            lds_poison_wave(stage, S::kBytes, lane);
             if (lane / S::kRows == h) {
                 char* dst = stage + (lane - h * S::kRows) * S::kRowBytes;
 #pragma unroll
@@ -423,6 +425,8 @@ __global__ __launch_bounds__(kRaggedThreads) void ragged_kernel(RaggedArgs g)
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     char* stage = smem + ragged_xchg_bytes<MODE>() + (size_t)wave * kStageWaveBytes;
+    lds_poison_wg(smem, (unsigned)sizeof(smem));
+    lds_poison_wg(s_bk, (unsigned)sizeof(s_bk));
     if (threadIdx.x < g.n_buckets) s_bk[threadIdx.x] = g.buckets[threadIdx.x];
 
     // phase 1: workgroup items (long systems), largest bucket first. The

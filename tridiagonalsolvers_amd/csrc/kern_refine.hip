@@ -220,12 +220,13 @@ hipError_t launch_refine(const RefineArgs& g)
             default: return hipErrorInvalidValue;
         }
     }
+    // multi-wave systems: at most 8 unknowns per lane (make_refine_plan refuses
+    // more; 16 per lane would not fit the registers next to the FP64 rows)
     if (g.L % kWave != 0 || g.L / kWave > 16 || g.N > 0x7fffffffLL) return hipErrorInvalidValue;
     switch (g.M) {
         case 2: return launch_refine_wg_m<2>(g);
         case 4: return launch_refine_wg_m<4>(g);
         case 8: return launch_refine_wg_m<8>(g);
-        case 16: return launch_refine_wg_m<16>(g);
         default: return hipErrorInvalidValue;
     }
 }

@@ -190,6 +190,13 @@ __device__ __forceinline__ void wave_core(Eq<T> (&e)[M], T (&xv)[M], int lane, i
 
 }
 
+template <typename T, int M>
+constexpr size_t wg_core_lds_bytes(int W)
+{
+    return (size_t)W * (Log2<M>::value + 6) * sizeof(Eq<T>) + (size_t)kWave * W * sizeof(Eq<T>) +
+           (size_t)kWave * W * sizeof(T);
+}
+
 // Multi-wave form: W waves of one workgroup hold one system, wave w owning
 // unknowns [w*64*M, (w+1)*64*M) (see kern_reg_mw.hip). Every thread of the
 // workgroup must call it (it contains barriers). `xchg` is LDS for
@@ -207,6 +214,12 @@ __device__ __forceinline__ void wg_core(Eq<T> (&e)[M], T (&xv)[M], int w, int W,
 
     const Eq<T> top0 = e[M - 1];  // level-0 top equation (lane 63's chain restarts here)
     __syncthreads();              // staging rows become the exchange area
+    if constexpr (kLdsPoison) {
+        // the W waves of this system fill its exchange area (other groups of
+        // the workgroup fill theirs; every thread reaches the barrier)
+        lds_fill_nan(xchg, (unsigned)wg_core_lds_bytes<T, M>(W), w * kWave + lane, W * kWave);
+        __syncthreads();
+    }
 
     // ---- in-lane CR levels (lane 63's top slot is provisional) ------------
 #pragma unroll
@@ -363,11 +376,5 @@ __device__ __forceinline__ void wg_core(Eq<T> (&e)[M], T (&xv)[M], int w, int W,
 
 }
 
-template <typename T, int M>
-constexpr size_t wg_core_lds_bytes(int W)
-{
-    return (size_t)W * (Log2<M>::value + 6) * sizeof(Eq<T>) + (size_t)kWave * W * sizeof(Eq<T>) +
-           (size_t)kWave * W * sizeof(T);
-}
 
 }  // namespace trid

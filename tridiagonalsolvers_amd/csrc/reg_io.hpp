@@ -146,6 +146,7 @@ static __device__ __forceinline__ void load_staged(const T* __restrict__ A, cons
 #pragma unroll
     for (int arr = 0; arr < 4; ++arr) {
         if (arr < first) continue;
+        lds_poison_wave(lds, S::kWaveBytes, lane);
 #pragma unroll
         for (int q = 0; q < Q; ++q)
             *reinterpret_cast<u32x4*>(lds + S::chunk_addr(kWave * q + lane)) = r[arr][q];
@@ -168,6 +169,7 @@ static __device__ __forceinline__ void store_staged(T* __restrict__ X, int64_t w
 {
     using S = Stage<T, M>;
     constexpr int E = S::E, Q = S::Q;
+    lds_poison_wave(lds, S::kWaveBytes, lane);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         u32x4 t;

@@ -218,6 +218,46 @@ __device__ __forceinline__ Eq<T> lane_bcast_eq(const Eq<T>& e, int src)
                  lane_bcast(e.d, src)};
 }
 
+// ---------------------------------------------------------------------------
+// LDS poisoning (debug builds compiled with -DTRID_LDS_POISON): every LDS
+// staging and exchange area is filled with all-ones bits -- a NaN in FP32 and
+// in FP64 -- before each use, so that a read of an element that no thread
+// wrote since turns into NaN in the results and fails the parity tests.
+// Release builds compile all of this out.
+// ---------------------------------------------------------------------------
+#ifdef TRID_LDS_POISON
+constexpr bool kLdsPoison = true;
+#else
+constexpr bool kLdsPoison = false;
+#endif
+
+// Threads t = 0..nthr-1 fill [p, p + bytes) (p 4-byte aligned, bytes a
+// multiple of 4) with 4-byte stores.
+__device__ __forceinline__ void lds_fill_nan(void* p, unsigned bytes, int t, int nthr)
+{
+    if constexpr (kLdsPoison) {
+        unsigned* w = static_cast<unsigned*>(p);
+        for (unsigned i = t; i < bytes / 4; i += nthr) w[i] = ~0u;
+    }
+}
+// Area used by the whole workgroup; every thread calls it (barrier).
+__device__ __forceinline__ void lds_poison_wg(void* p, unsigned bytes)
+{
+    if constexpr (kLdsPoison) {
+        lds_fill_nan(p, bytes, (int)threadIdx.x, (int)blockDim.x);
+        __syncthreads();
+    }
+}
+// Area of one wave; every lane of the wave calls it. LDS operations of a
+// wave complete in order, so the wave's later accesses see the fill.
+__device__ __forceinline__ void lds_poison_wave(void* p, unsigned bytes, int lane)
+{
+    if constexpr (kLdsPoison) {
+        lds_fill_nan(p, bytes, lane, kWave);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 template <int V>
 struct Log2 {
     static constexpr int value = 1 + Log2<V / 2>::value;

@@ -8,8 +8,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "trid.h"
@@ -157,9 +161,198 @@ int ragged_build_host(const int64_t* offsets, int64_t N, const trid_bucket_cfg* 
     return TRID_OK;
 }
 
+// ---------------------------------------------------------------------------
+// The autotune table file: JSON written by tridiagonalsolvers_amd.autotune,
+// an object whose "buckets" array holds one object per bucket with integer
+// "size", "per_lane", "lanes" and "rest" (other keys are measurements and are
+// skipped). A small recursive-descent reader of that JSON, so that C and C++
+// hosts load the same table the Python binding does.
+namespace {
+
+struct JsonReader {
+    const char* p;
+    const char* end;
+    bool ok = true;
+
+    void ws()
+    {
+        while (p < end && std::isspace((unsigned char)*p)) ++p;
+    }
+    bool eat(char ch)
+    {
+        ws();
+        if (p < end && *p == ch) {
+            ++p;
+            return true;
+        }
+        return false;
+    }
+    bool string(std::string* out)
+    {
+        ws();
+        if (p >= end || *p != '"') return ok = false;
+        ++p;
+        std::string v;
+        while (p < end && *p != '"') {
+            if (*p == '\\') {
+                ++p;
+                if (p >= end) return ok = false;
+                if (*p == 'u') {  // keys of interest are ASCII: keep a placeholder
+                    if (end - p < 5) return ok = false;
+                    p += 4;
+                    v.push_back('?');
+                } else {
+                    v.push_back(*p);
+                }
+                ++p;
+                continue;
+            }
+            v.push_back(*p++);
+        }
+        if (p >= end) return ok = false;
+        ++p;
+        if (out) *out = v;
+        return true;
+    }
+    bool number(double* out)
+    {
+        ws();
+        const char* s0 = p;
+        if (p < end && (*p == '-' || *p == '+')) ++p;
+        while (p < end && (std::isdigit((unsigned char)*p) || *p == '.' || *p == 'e' ||
+                           *p == 'E' || *p == '-' || *p == '+'))
+            ++p;
+        if (p == s0) return ok = false;
+        std::string tok(s0, p);
+        char* e = nullptr;
+        const double v = std::strtod(tok.c_str(), &e);
+        if (!e || *e) return ok = false;
+        if (out) *out = v;
+        return true;
+    }
+    // skip any value (nesting bounded by the file)
+    bool skip(int depth = 0)
+    {
+        if (depth > 64) return ok = false;
+        ws();
+        if (p >= end) return ok = false;
+        if (*p == '"') return string(nullptr);
+        if (*p == '{') {
+            ++p;
+            if (eat('}')) return true;
+            do {
+                if (!string(nullptr) || !eat(':') || !skip(depth + 1)) return ok = false;
+            } while (eat(','));
+            return eat('}') || (ok = false);
+        }
+        if (*p == '[') {
+            ++p;
+            if (eat(']')) return true;
+            do {
+                if (!skip(depth + 1)) return ok = false;
+            } while (eat(','));
+            return eat(']') || (ok = false);
+        }
+        for (const char* lit : {"true", "false", "null"}) {
+            const size_t n = std::strlen(lit);
+            if ((size_t)(end - p) >= n && std::strncmp(p, lit, n) == 0) {
+                p += n;
+                return true;
+            }
+        }
+        return number(nullptr);
+    }
+};
+
+bool as_int32(double v, int32_t* out)
+{
+    if (!(v >= -2147483648.0 && v <= 2147483647.0) || v != (double)(int64_t)v) return false;
+    *out = (int32_t)v;
+    return true;
+}
+
+// One bucket object: the four integers, all required.
+bool read_bucket(JsonReader& r, trid_bucket_cfg* b)
+{
+    if (!r.eat('{')) return false;
+    bool have[4] = {false, false, false, false};
+    static const char* keys[4] = {"size", "per_lane", "lanes", "rest"};
+    int32_t* dst[4] = {&b->size, &b->per_lane, &b->lanes, &b->rest};
+    if (!r.eat('}')) {
+        do {
+            std::string key;
+            if (!r.string(&key) || !r.eat(':')) return false;
+            int k = -1;
+            for (int i = 0; i < 4; ++i)
+                if (key == keys[i]) k = i;
+            if (k < 0) {
+                if (!r.skip()) return false;
+                continue;
+            }
+            double v = 0;
+            if (!r.number(&v) || !as_int32(v, dst[k])) return false;
+            have[k] = true;
+        } while (r.eat(','));
+        if (!r.eat('}')) return false;
+    }
+    return have[0] && have[1] && have[2] && have[3];
+}
+
+}  // namespace
+
 }  // namespace trid
 
 extern "C" {
+
+int trid_bucket_table_load(const char* path, trid_bucket_cfg* table, int capacity, int* count)
+{
+    if (!path || !count || capacity < 0 || (capacity > 0 && !table)) return TRID_EINVAL;
+    *count = 0;
+    std::FILE* f = std::fopen(path, "rb");
+    if (!f) return TRID_EINVAL;
+    std::string text;
+    char buf[65536];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof(buf), f)) > 0) {
+        text.append(buf, got);
+        if (text.size() > (size_t(64) << 20)) {  // a table is a few KB
+            std::fclose(f);
+            return TRID_EINVAL;
+        }
+    }
+    std::fclose(f);
+    trid::JsonReader r{text.data(), text.data() + text.size()};
+    if (!r.eat('{')) return TRID_EINVAL;
+    bool found = false;
+    int n = 0;
+    if (!r.eat('}')) {
+        do {
+            std::string key;
+            if (!r.string(&key) || !r.eat(':')) return TRID_EINVAL;
+            if (key != "buckets") {
+                if (!r.skip()) return TRID_EINVAL;
+                continue;
+            }
+            if (found || !r.eat('[')) return TRID_EINVAL;
+            found = true;
+            if (r.eat(']')) continue;
+            do {
+                trid_bucket_cfg b{};
+                if (!trid::read_bucket(r, &b)) return TRID_EINVAL;
+                const trid::RaggedBucket rb{b.size, b.per_lane, b.lanes, b.rest};
+                if (trid::ragged_check_bucket(rb) != TRID_OK) return TRID_EINVAL;
+                if (n < capacity) table[n] = b;
+                ++n;
+            } while (r.eat(','));
+            if (!r.eat(']')) return TRID_EINVAL;
+        } while (r.eat(','));
+        if (!r.eat('}')) return TRID_EINVAL;
+    }
+    r.ws();
+    if (!found || r.p != r.end) return TRID_EINVAL;
+    *count = n;
+    return TRID_OK;
+}
 
 int trid_ragged_plan_create(trid_ragged_plan* out, const int64_t* offsets, int64_t numSystems,
                             const trid_bucket_cfg* table, int tableSize)
