@@ -1,53 +1,60 @@
-// kern_ragged.hip -- one persistent launch over a ragged batch.
+// kern_ragged.hip -- persistent launches over a ragged batch.
 //
 // No counterpart in the reference (every reference wrapper takes one
 // systemSize for the whole batch). A ragged batch is numSystems systems of
 // arbitrary sizes n_s >= 2 in flat FP64 arrays, system s occupying
 // [offsets[s], offsets[s+1]). Every system is solved in FP32 on the register
-// CR-PCR core and refined in FP64 (the kern_refine.hip scheme), padded to
-// the power of two above n_s with identity rows (exact).
+// CR-PCR core and refined in FP64 (the kern_refine.hip scheme), or solved once
+// in FP64, padded to the power of two above n_s with identity rows (exact).
 //
 // Work queue (built on the host, trid_ragged.cpp): systems are bucketed by
 // padded size; each bucket has a configuration (unknowns per lane M, lanes
-// per system, CR->PCR switch size) chosen by the autotune table.
-//   * wave items: buckets up to 512 unknowns; one item = up to 64/L systems
-//     of one bucket, solved by one wave, L lanes each (no barrier);
-//   * workgroup items: buckets of 1024..16384 unknowns; 8 unknowns per lane
-//     (16 for 16384), W = n/512 waves per system (16 for 16384), 16/W systems
-//     per item, one per group of W waves of the 1024-thread workgroup
-//     (barriers are workgroup-wide, and every group of an item runs the same
-//     bucket's code, so they match).
-// The grid is persistent (one workgroup per CU slot). Workgroups first drain
-// the workgroup items, largest bucket first, from an atomic queue counter
-// that a small kernel resets before every launch (so the pair can be
-// captured in a hipGraph and replayed); then each wave works through the
-// wave items dealt to it round-robin.
+// per system, CR->PCR switch size) chosen by the autotune table. Three item
+// classes, each run by a persistent kernel of its own (the three go into one
+// captured graph, so that each gets the register and LDS budget its shape
+// needs):
+//   * workgroup items, 1024..8192 unknowns (ragged_wg_kernel): 8 unknowns
+//     per lane, W = n/512 waves per system, 16/W systems per item, one per
+//     group of W waves of a 1024-thread workgroup (barriers are
+//     workgroup-wide, and every group of an item runs the same bucket's code,
+//     so they match); drained from an atomic queue counter;
+//   * wide items, 8193..16384 unknowns, refine mode only
+//     (ragged_wide_kernel): one system per 1024-thread workgroup at 16
+//     unknowns per lane, the coefficients read again on each pass;
+//   * wave items, up to 512 unknowns (ragged_wave_kernel): up to 64/L
+//     systems of one bucket per item, solved by one wave, L lanes each (no
+//     barrier); 256-thread workgroups, items dealt round-robin over the waves.
+// The refine mode reads each system's FP64 coefficients once: b and d stay in
+// registers and a and c in LDS across its passes. A small kernel resets the
+// queue counters before every launch (so the whole sequence can be captured
+// in a hipGraph and replayed).
 #include "reg_core.hpp"
 #include "trid_device.hpp"
 #include "trid_launch.hpp"
 
+#include <mutex>
 #include <type_traits>
 
 namespace trid {
 
 namespace {
 
-// Coefficient arrays whose loads are in flight at once in the refinement
-// passes (2 or 4). Measured on the 100,000-system batch: sequential 3.43 ms,
-// 4 in flight 3.68 ms (the registers spill), 2 in flight 3.34 ms. The FP64
-// mode always issues all four (1.95 ms, against 2.09 ms sequential).
+// Coefficient arrays whose loads are in flight at once in ragged_rows (the
+// wide items' passes): 2, or sequential above 8 unknowns per lane.
 #ifndef TRID_RAGGED_AHEAD
 #define TRID_RAGGED_AHEAD 2
 #endif
 
-constexpr int kRaggedThreads = 1024;
-constexpr int kRaggedWaves = kRaggedThreads / kWave;
-static_assert(kRaggedWaves == 16, "workgroup items assume 16 waves");
+constexpr int kWgThreads = 1024;    // workgroup-item and wide-item kernels
+constexpr int kWgWaves = kWgThreads / kWave;
+static_assert(kWgWaves == 16, "workgroup items assume 16 waves");
+constexpr int kWaveThreads = 256;   // wave-item kernel: 4 independent waves
+constexpr int kWaveWaves = kWaveThreads / kWave;
 
 // Transposition area of one wave (coalesced loads and stores, below).
 // Row r holds the M consecutive unknowns of lane r, padded by 16 bytes:
 // a row stride of M*8+16 bytes puts the 16 lanes of one pass of a 16-byte
-// LDS access on disjoint banks. M = 16 is staged in two rounds of 32 rows.
+// LDS access on disjoint banks.
 template <int M>
 struct RStage {
     static constexpr int kRowBytes = M * 8 + 16;
@@ -56,31 +63,58 @@ struct RStage {
     static constexpr int kRows = kWave / kRounds;  // rows per round
     static constexpr int kBytes = kRows * kRowBytes;
 };
-constexpr int kStageWaveBytes = RStage<8>::kBytes;  // 5120, the largest
-static_assert(RStage<16>::kBytes <= kStageWaveBytes, "stage area");
+constexpr int kStageWaveBytes = RStage<8>::kBytes;  // 5120
 
-// Kernel modes: FP32 solve + FP64 refinement (workgroup items at 8
-// unknowns per lane, or also 16 in the wide form for 16384-unknown
-// systems), or one FP64 solve (8 unknowns per lane, systems up to 8192).
-enum RaggedMode { kRefine = 0, kRefineWide = 1, kDirect = 2 };
+// Kernel modes: FP32 solve + FP64 refinement, or one FP64 solve (systems up
+// to 8192 unknowns).
+enum RaggedMode { kRefine = 0, kDirect = 1 };
 
-// LDS of the workgroup items: 16/W groups of W waves, each with the
-// exchange area of wg_core and 2 W boundary unknowns (wg_core_lds_bytes is
-// linear in W, so the 16-wave group is the largest). After it, one
-// transposition area per wave.
+// The refine mode keeps each system's FP64 a and c in LDS across its
+// passes (and b, d in registers), so the coefficients are read from HBM once:
+// every wave owns a home area of 64 rows of 8 doubles for each of a and c,
+// row r holding the operator rows of lane r. A row's four 16-byte chunks are
+// XOR-permuted by bits 2..3 of the row index, so that 16 lanes reading one
+// chunk of their own rows (rows 0..15) touch 16 distinct groups of four
+// banks, and 32 lanes writing four whole consecutive rows do too. The home
+// area also transposes b, d (on their way to the registers) and x.
+struct Home {
+    static constexpr int kRowBytes = 8 * 8;
+    static constexpr int kArrBytes = kWave * kRowBytes;  // 4 KiB
+    static constexpr int kWaveBytes = 2 * kArrBytes;     // a, then c
+    __device__ static __forceinline__ int at(int r, int k)
+    {
+        return r * kRowBytes + (((k >> 1) ^ ((r >> 2) & 3)) << 4) + ((k & 1) << 3);
+    }
+};
+
+// Per-wave area (home area or transposition area) of a mode.
+template <int MODE>
+constexpr size_t ragged_wave_bytes()
+{
+    return MODE == kRefine ? Home::kWaveBytes : kStageWaveBytes;
+}
+// Exchange areas of the workgroup items: 16/W groups of W waves, each with
+// the exchange area of wg_core and 2 W boundary unknowns (wg_core_lds_bytes
+// is linear in W, so the 16-wave group is the largest).
 template <int MODE>
 constexpr size_t ragged_xchg_bytes()
 {
     return MODE == kDirect ? wg_core_lds_bytes<double, 8>(16) + 2 * 16 * sizeof(double)
-                           : wg_core_lds_bytes<float, 16>(16) + 2 * 16 * sizeof(double);
+                           : wg_core_lds_bytes<float, 8>(16) + 2 * 16 * sizeof(double);
 }
 template <int MODE>
-constexpr size_t ragged_lds_bytes()
+constexpr size_t wg_lds_bytes()
 {
-    return ragged_xchg_bytes<MODE>() + (size_t)kRaggedWaves * kStageWaveBytes;
+    return (size_t)kWgWaves * ragged_wave_bytes<MODE>() + ragged_xchg_bytes<MODE>();
 }
-static_assert(ragged_lds_bytes<kDirect>() <= 160 * 1024, "LDS of the FP64 mode");
-static_assert(ragged_lds_bytes<kRefineWide>() <= 160 * 1024, "LDS of the refine mode");
+// wide items: the transposition areas (16 unknowns per lane), then the
+// exchange area and 2 W boundary unknowns
+constexpr size_t kWideStageBytes = (size_t)kWgWaves * RStage<16>::kBytes;
+constexpr size_t kWideLdsBytes =
+    kWideStageBytes + wg_core_lds_bytes<float, 16>(16) + 2 * 16 * sizeof(double);
+static_assert(wg_lds_bytes<kDirect>() <= 160 * 1024 - 512, "LDS of the FP64 workgroup items");
+static_assert(wg_lds_bytes<kRefine>() <= 160 * 1024 - 512, "LDS of the refine workgroup items");
+static_assert(kWideLdsBytes <= 160 * 1024 - 512, "LDS of the wide items");
 
 // Lane-blocked gathers. The system (segment of L lanes, first row of the
 // segment at lane - ll) starts at `base` and has n unknowns; lane ll must
@@ -208,6 +242,250 @@ __device__ __forceinline__ void scatter_rows(double* __restrict__ X, int64_t bas
     }
 }
 
+// FP64 operator of the lane's unknowns [j0, j0+M) (arguments as
+// ragged_rows). Identity rows past n; a[first] and c[last] ignored.
+template <int M>
+__device__ __forceinline__ void direct_rows(const RaggedArgs& g, int64_t base, int nb,
+                                            int j0, int n, int L, int ll, int lane,
+                                            bool live, char* stage, Eq<double> (&e)[M])
+{
+    double va[M], vb[M], vc[M], vd[M];
+    gather4<M>(g, base, nb, L, ll, lane, live, stage, va, vb, vc, vd);
+#pragma unroll
+    for (int s = 0; s < M; ++s)
+        e[s] = Eq<double>{j0 + s > 0 ? va[s] : 0.0, vb[s], j0 + s < n - 1 ? vc[s] : 0.0, vd[s]};
+}
+
+// ---- resident refine mode ---------------------------------------------------
+// Column element q of lane ll (unknown q*L + ll of the segment, as
+// fetch_cols loads it) into the row layout of a home array.
+template <int M>
+__device__ __forceinline__ void home_put_cols(char* arr, const double (&t)[M], int L, int ll,
+                                              int lane)
+{
+    const int seg_row = lane - ll;
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+        const int j = q * L + ll;
+        *reinterpret_cast<double*>(arr + Home::at(seg_row + j / M, j % M)) = t[q];
+    }
+}
+template <int M>
+__device__ __forceinline__ void home_get_row(const char* arr, int lane, double (&v)[M])
+{
+    if constexpr (M == 1) {
+        v[0] = *reinterpret_cast<const double*>(arr + Home::at(lane, 0));
+    } else {
+#pragma unroll
+        for (int k = 0; k < M; k += 2) {
+            const double2 w = *reinterpret_cast<const double2*>(arr + Home::at(lane, k));
+            v[k] = w.x;
+            v[k + 1] = w.y;
+        }
+    }
+}
+
+// One coalesced read of the lane's rows (arguments as ragged_rows): b and d
+// end in registers, a and c in the wave's home area (identity rows past n).
+template <int M>
+__device__ __forceinline__ void gather_resident(const RaggedArgs& g, int64_t base, int nb, int L,
+                                                int ll, int lane, bool live, char* home,
+                                                double (&vb)[M], double (&vd)[M])
+{
+    asm volatile("" : "+v"(ll), "+v"(nb));
+    double ta[M], tb[M], tc[M], td[M];
+    fetch_cols<M>(g.d, base, nb, L, ll, live, 0.0, td);
+    fetch_cols<M>(g.b, base, nb, L, ll, live, 1.0, tb);
+    fetch_cols<M>(g.a, base, nb, L, ll, live, 0.0, ta);
+    fetch_cols<M>(g.c, base, nb, L, ll, live, 0.0, tc);
+    char* ha = home;
+    char* hc = home + Home::kArrBytes;
+    // b and d pass through a's area on their way to the registers
+    lds_poison_wave(ha, Home::kArrBytes, lane);
+    home_put_cols<M>(ha, tb, L, ll, lane);
+    __builtin_amdgcn_wave_barrier();
+    home_get_row<M>(ha, lane, vb);
+    __builtin_amdgcn_wave_barrier();
+    lds_poison_wave(ha, Home::kArrBytes, lane);
+    home_put_cols<M>(ha, td, L, ll, lane);
+    __builtin_amdgcn_wave_barrier();
+    home_get_row<M>(ha, lane, vd);
+    __builtin_amdgcn_wave_barrier();
+    lds_poison_wave(home, Home::kWaveBytes, lane);
+    home_put_cols<M>(ha, ta, L, ll, lane);
+    home_put_cols<M>(hc, tc, L, ll, lane);
+    __builtin_amdgcn_wave_barrier();
+}
+
+// FP32 operator + FP64 residual rows of the lane's unknowns [j0, j0+M) of a
+// system of n unknowns, from the resident operator (the residual's operations
+// in the order of ragged_rows: d, then b, a, c). Identity rows past n;
+// a[first] and c[last] ignored.
+template <int M>
+__device__ __forceinline__ void resident_rows(const char* home, int lane, const double (&vb)[M],
+                                              const double (&vd)[M], const double (&x)[M],
+                                              double x_left, double x_right, int j0, int n,
+                                              bool live, Eq<float> (&e)[M])
+{
+    const char* ha = home;
+    const char* hc = home + Home::kArrBytes;
+#pragma unroll
+    for (int k = 0; k < M; k += 2) {
+        double av[2], cv[2];
+        if constexpr (M == 1) {
+            av[0] = *reinterpret_cast<const double*>(ha + Home::at(lane, 0));
+            cv[0] = *reinterpret_cast<const double*>(hc + Home::at(lane, 0));
+        } else {
+            const double2 a2 = *reinterpret_cast<const double2*>(ha + Home::at(lane, k));
+            const double2 c2 = *reinterpret_cast<const double2*>(hc + Home::at(lane, k));
+            av[0] = a2.x;
+            av[1] = a2.y;
+            cv[0] = c2.x;
+            cv[1] = c2.y;
+        }
+#pragma unroll
+        for (int h = 0; h < 2 && k + h < M; ++h) {
+            const int s = k + h;
+            const double al = j0 + s > 0 ? av[h] : 0.0;
+            const double cr = j0 + s < n - 1 ? cv[h] : 0.0;
+            const double xl = s > 0 ? x[s > 0 ? s - 1 : 0] : x_left;
+            const double xr = s < M - 1 ? x[s < M - 1 ? s + 1 : 0] : x_right;
+            double r = vd[s];
+            r = fma(-vb[s], x[s], r);
+            r = fma(-al, xl, r);
+            r = fma(-cr, xr, r);
+            e[s] = Eq<float>{(float)al, (float)vb[s], (float)cr,
+                             (live && j0 + s < n) ? (float)r : 0.0f};
+        }
+    }
+}
+
+// The lane's M unknowns back to unknowns [ll*M, ll*M+M) of the segment,
+// transposed through the home area's a rows (dead after the last pass),
+// written with coalesced stores (nothing past nb).
+template <int M>
+__device__ __forceinline__ void scatter_resident(double* __restrict__ X, int64_t base, int nb,
+                                                 int L, int ll, int lane, bool live, char* home,
+                                                 const double (&v)[M])
+{
+    char* ha = home;
+    // addresses formed here, not kept from the gather across the passes
+    asm volatile("" : "+v"(ll), "+v"(nb), "+v"(lane));
+    const int seg_row = lane - ll;
+    __builtin_amdgcn_wave_barrier();
+    lds_poison_wave(ha, Home::kArrBytes, lane);
+    if constexpr (M == 1) {
+        *reinterpret_cast<double*>(ha + Home::at(lane, 0)) = v[0];
+    } else {
+#pragma unroll
+        for (int k = 0; k < M; k += 2)
+            *reinterpret_cast<double2*>(ha + Home::at(lane, k)) = double2{v[k], v[k + 1]};
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+        const int j = q * L + ll;
+        const double t = *reinterpret_cast<const double*>(ha + Home::at(seg_row + j / M, j % M));
+        if (live && j < nb) X[base + j] = t;
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int M>
+__device__ void wave_item_resident(const RaggedArgs& g, const RaggedItem& it,
+                                   const RaggedBucket& bk, int lane, char* home)
+{
+    asm volatile("" : "+v"(lane));  // formed per item (see wide_item)
+    const int L = bk.lanes;
+    const int ll = lane & (L - 1);
+    const int seg = lane / L;
+    const bool live = seg < it.count;
+    int64_t base = 0;
+    int n = 2;
+    if (live) {
+        const RaggedSys sy = g.sys[it.first + seg];
+        base = sy.base;
+        n = (int)sy.n;
+    }
+    const int j0 = ll * M;
+    double vb[M], vd[M];
+    gather_resident<M>(g, base, n, L, ll, lane, live, home, vb, vd);
+    double x[M];
+#pragma unroll
+    for (int s = 0; s < M; ++s) x[s] = 0.0;
+    Eq<float> e[M];
+    float xf[M];
+    for (int step = 0; step <= g.iters; ++step) {
+        const double xl = lane_step_or_zero<-1>(x[M - 1], ll >= 1);
+        const double xr = lane_step_or_zero<1>(x[0], ll + 1 < L);
+        resident_rows<M>(home, lane, vb, vd, x, xl, xr, j0, n, live, e);
+        wave_core<float, M, 1>(e, xf, lane, ll, L, bk.rest);
+#pragma unroll
+        for (int s = 0; s < M; ++s) x[s] += (double)xf[s];
+    }
+    scatter_resident<M>(g.x, base, n, L, ll, lane, live, home, x);
+}
+
+// All 16 waves of the workgroup call this (wg_core has barriers). 8 unknowns
+// per lane.
+__device__ void wg_item_resident(const RaggedArgs& g, const RaggedItem& it,
+                                 const RaggedBucket& bk, int wave, int lane, char* xchg_base,
+                                 char* home)
+{
+    asm volatile("" : "+v"(lane));  // formed per item (see wide_item)
+    constexpr int M = 8;
+    const int W = bk.lanes / kWave;
+    const int grp = wave / W;
+    const int w = wave % W;
+    const size_t grp_bytes = wg_core_lds_bytes<float, M>(W) + (size_t)2 * W * sizeof(double);
+    char* xchg = xchg_base + grp * grp_bytes;
+    double* edge = reinterpret_cast<double*>(xchg + wg_core_lds_bytes<float, M>(W));
+    const bool live = grp < it.count;
+    int64_t base = 0;
+    int n = 2;
+    if (live) {
+        const RaggedSys sy = g.sys[it.first + grp];
+        base = sy.base;
+        n = (int)sy.n;
+    }
+    // this wave's block of the system: unknowns [w*64*M, (w+1)*64*M)
+    const int wb = w * kWave * M;
+    const int j0 = wb + lane * M;
+    double vb[M], vd[M];
+    gather_resident<M>(g, base + wb, n - wb, kWave, lane, lane, live, home, vb, vd);
+    double x[M];
+#pragma unroll
+    for (int s = 0; s < M; ++s) x[s] = 0.0;
+    Eq<float> e[M];
+    float xf[M];
+    for (int step = 0; step <= g.iters; ++step) {
+        // lane-derived values (masks, permute addresses) are cheap: left
+        // visible, the compiler keeps them all across the passes and spills
+        int ln = lane, jj = j0, nn = n;
+        asm volatile("" : "+v"(ln), "+v"(jj), "+v"(nn));
+        double xl = 0.0, xr = 0.0;
+        if (step > 0) {
+            if (ln == 0) edge[2 * w] = x[0];
+            if (ln == kWave - 1) edge[2 * w + 1] = x[M - 1];
+            __syncthreads();
+            const double xl0 = lane_read(x[M - 1], (ln - 1) & (kWave - 1));
+            const double xr0 = lane_read(x[0], (ln + 1) & (kWave - 1));
+            xl = ln > 0 ? xl0 : (w > 0 ? edge[2 * w - 1] : 0.0);
+            xr = ln < kWave - 1 ? xr0 : (w + 1 < W ? edge[2 * w + 2] : 0.0);
+        }
+        resident_rows<M>(home, ln, vb, vd, x, xl, xr, jj, nn, live, e);
+        wg_core<float, M>(e, xf, w, W, ln, bk.rest, xchg);
+#pragma unroll
+        for (int s = 0; s < M; ++s) x[s] += (double)xf[s];
+    }
+    scatter_resident<M>(g.x, base + wb, n - wb, kWave, lane, lane, live, home, x);
+    __syncthreads();  // exchange areas are reused by the next item
+}
+
+// ---- wide items (refine mode, 8193..16384 unknowns) ---------------------------
+// One system per 1024-thread workgroup, 16 unknowns per lane, x in
+// registers; each pass reads the system's FP64 rows again (coalesced, one
+// array at a time through the wave's transposition area).
 // FP32 operator + FP64 residual rows (as kern_refine.hip's build_rows) of
 // the lane's unknowns [j0, j0+M) of a system with n unknowns. `base`, `nb`
 // and `j0b` locate the lane's segment for the gathers: the segment's first
@@ -294,24 +572,61 @@ __device__ __forceinline__ void ragged_rows(const RaggedArgs& g, const double (&
     }
 }
 
-// FP64 operator of the lane's unknowns [j0, j0+M) (arguments as
-// ragged_rows). Identity rows past n; a[first] and c[last] ignored.
-template <int M>
-__device__ __forceinline__ void direct_rows(const RaggedArgs& g, int64_t base, int nb,
-                                            int j0, int n, int L, int ll, int lane,
-                                            bool live, char* stage, Eq<double> (&e)[M])
+
+__device__ void wide_item(const RaggedArgs& g, const RaggedItem& it, const RaggedBucket& bk,
+                          int wave, int lane, char* xchg_base, char* stage)
 {
-    double va[M], vb[M], vc[M], vd[M];
-    gather4<M>(g, base, nb, L, ll, lane, live, stage, va, vb, vc, vd);
+    // Lane-derived addresses are cheap to form; left visible, the compiler
+    // forms every one of them once for the whole persistent loop and spills
+    // them. Made opaque here, they are formed per item.
+    asm volatile("" : "+v"(lane));
+    constexpr int M = 16;
+    const int W = bk.lanes / kWave;  // 16: one system per item
+    const int w = wave;
+    const size_t grp_bytes = wg_core_lds_bytes<float, M>(W) + (size_t)2 * W * sizeof(double);
+    (void)grp_bytes;
+    char* xchg = xchg_base;
+    double* edge = reinterpret_cast<double*>(xchg + wg_core_lds_bytes<float, M>(W));
+    const bool live = it.count > 0;
+    const RaggedSys sy = g.sys[it.first];
+    const int64_t base = sy.base;
+    const int n = (int)sy.n;
+    // this wave's block of the system: unknowns [w*64*M, (w+1)*64*M)
+    const int wb = w * kWave * M;
+    const int j0 = wb + lane * M;
+    double x[M];
 #pragma unroll
-    for (int s = 0; s < M; ++s)
-        e[s] = Eq<double>{j0 + s > 0 ? va[s] : 0.0, vb[s], j0 + s < n - 1 ? vc[s] : 0.0, vd[s]};
+    for (int s = 0; s < M; ++s) x[s] = 0.0;
+    Eq<float> e[M];
+    float xf[M];
+    for (int step = 0; step <= g.iters; ++step) {
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
+        double xl = 0.0, xr = 0.0;
+        if (step > 0) {
+            if (ln == 0) edge[2 * w] = x[0];
+            if (ln == kWave - 1) edge[2 * w + 1] = x[M - 1];
+            __syncthreads();
+            const double xl0 = lane_read(x[M - 1], (ln - 1) & (kWave - 1));
+            const double xr0 = lane_read(x[0], (ln + 1) & (kWave - 1));
+            xl = ln > 0 ? xl0 : (w > 0 ? edge[2 * w - 1] : 0.0);
+            xr = ln < kWave - 1 ? xr0 : (w + 1 < W ? edge[2 * w + 2] : 0.0);
+        }
+        ragged_rows<M>(g, x, xl, xr, base + wb, n - wb, j0, n, kWave, ln, ln, live, stage, e);
+        wg_core<float, M>(e, xf, w, W, ln, bk.rest, xchg);
+#pragma unroll
+        for (int s = 0; s < M; ++s) x[s] += (double)xf[s];
+    }
+    scatter_rows<M>(g.x, base + wb, n - wb, kWave, lane, lane, live, stage, x);
+    __syncthreads();  // exchange areas are reused by the next item
 }
 
-template <bool DIRECT, int M>
-__device__ void wave_item(const RaggedArgs& g, const RaggedItem& it, const RaggedBucket& bk,
-                          int lane, char* stage)
+// ---- FP64 mode items (one solve) -------------------------------------------------
+template <int M>
+__device__ void wave_item_direct(const RaggedArgs& g, const RaggedItem& it,
+                                 const RaggedBucket& bk, int lane, char* stage)
 {
+    asm volatile("" : "+v"(lane));  // formed per item (see wide_item)
     const int L = bk.lanes;
     const int ll = lane & (L - 1);
     const int seg = lane / L;
@@ -325,40 +640,23 @@ __device__ void wave_item(const RaggedArgs& g, const RaggedItem& it, const Ragge
     }
     const int j0 = ll * M;
     double x[M];
-    if constexpr (DIRECT) {
-        Eq<double> e[M];
-        direct_rows<M>(g, base, n, j0, n, L, ll, lane, live, stage, e);
-        wave_core<double, M, 1>(e, x, lane, ll, L, bk.rest);
-        scatter_rows<M>(g.x, base, n, L, ll, lane, live, stage, x);
-        return;
-    }
-#pragma unroll
-    for (int s = 0; s < M; ++s) x[s] = 0.0;
-    Eq<float> e[M];
-    float xf[M];
-    for (int step = 0; step <= g.iters; ++step) {
-        const double xl = lane_step_or_zero<-1>(x[M - 1], ll >= 1);
-        const double xr = lane_step_or_zero<1>(x[0], ll + 1 < L);
-        ragged_rows<M>(g, x, xl, xr, base, n, j0, n, L, ll, lane, live, stage, e);
-        wave_core<float, M, 1>(e, xf, lane, ll, L, bk.rest);
-#pragma unroll
-        for (int s = 0; s < M; ++s) x[s] += (double)xf[s];
-    }
+    Eq<double> e[M];
+    direct_rows<M>(g, base, n, j0, n, L, ll, lane, live, stage, e);
+    wave_core<double, M, 1>(e, x, lane, ll, L, bk.rest);
     scatter_rows<M>(g.x, base, n, L, ll, lane, live, stage, x);
 }
 
 // All 16 waves of the workgroup call this (wg_core has barriers).
-template <bool DIRECT, int M>
-__device__ void wg_item(const RaggedArgs& g, const RaggedItem& it, const RaggedBucket& bk,
-                        int wave, int lane, char* smem, char* stage)
+__device__ void wg_item_direct(const RaggedArgs& g, const RaggedItem& it, const RaggedBucket& bk,
+                               int wave, int lane, char* xchg_base, char* stage)
 {
-    using TC = typename std::conditional<DIRECT, double, float>::type;  // core type
+    asm volatile("" : "+v"(lane));  // formed per item (see wide_item)
+    constexpr int M = 8;
     const int W = bk.lanes / kWave;
     const int grp = wave / W;
     const int w = wave % W;
-    const size_t grp_bytes = wg_core_lds_bytes<TC, M>(W) + (size_t)2 * W * sizeof(double);
-    char* xchg = smem + grp * grp_bytes;
-    double* edge = reinterpret_cast<double*>(xchg + wg_core_lds_bytes<TC, M>(W));
+    const size_t grp_bytes = wg_core_lds_bytes<double, M>(W) + (size_t)2 * W * sizeof(double);
+    char* xchg = xchg_base + grp * grp_bytes;
     const bool live = grp < it.count;
     int64_t base = 0;
     int n = 2;
@@ -371,37 +669,33 @@ __device__ void wg_item(const RaggedArgs& g, const RaggedItem& it, const RaggedB
     const int wb = w * kWave * M;
     const int j0 = wb + lane * M;
     double x[M];
-    if constexpr (DIRECT) {
-        Eq<double> e[M];
-        direct_rows<M>(g, base + wb, n - wb, j0, n, kWave, lane, lane, live, stage, e);
-        wg_core<double, M>(e, x, w, W, lane, bk.rest, xchg);
-        scatter_rows<M>(g.x, base + wb, n - wb, kWave, lane, lane, live, stage, x);
-        __syncthreads();  // exchange areas are reused by the next item
-        return;
-    }
-#pragma unroll
-    for (int s = 0; s < M; ++s) x[s] = 0.0;
-    Eq<float> e[M];
-    float xf[M];
-    for (int step = 0; step <= g.iters; ++step) {
-        double xl = 0.0, xr = 0.0;
-        if (step > 0) {
-            if (lane == 0) edge[2 * w] = x[0];
-            if (lane == kWave - 1) edge[2 * w + 1] = x[M - 1];
-            __syncthreads();
-            const double xl0 = lane_read(x[M - 1], (lane - 1) & (kWave - 1));
-            const double xr0 = lane_read(x[0], (lane + 1) & (kWave - 1));
-            xl = lane > 0 ? xl0 : (w > 0 ? edge[2 * w - 1] : 0.0);
-            xr = lane < kWave - 1 ? xr0 : (w + 1 < W ? edge[2 * w + 2] : 0.0);
-        }
-        ragged_rows<M>(g, x, xl, xr, base + wb, n - wb, j0, n, kWave, lane, lane, live, stage,
-                       e);
-        wg_core<float, M>(e, xf, w, W, lane, bk.rest, xchg);
-#pragma unroll
-        for (int s = 0; s < M; ++s) x[s] += (double)xf[s];
-    }
+    Eq<double> e[M];
+    direct_rows<M>(g, base + wb, n - wb, j0, n, kWave, lane, lane, live, stage, e);
+    wg_core<double, M>(e, x, w, W, lane, bk.rest, xchg);
     scatter_rows<M>(g.x, base + wb, n - wb, kWave, lane, lane, live, stage, x);
     __syncthreads();  // exchange areas are reused by the next item
+}
+
+// Items [lo, hi) from the atomic queue counter `ctr`; every thread of the
+// workgroup calls it, `body(item)` ends with a workgroup barrier. The next
+// item's queue slot is claimed when an item starts and published when it
+// ends, so the atomic's latency hides behind the item.
+template <typename F>
+__device__ __forceinline__ void drain_queue(int lo, int hi, int* ctr, int* s_item, F&& body)
+{
+    if (threadIdx.x == 0) s_item[0] = lo + atomicAdd(ctr, 1);
+    __syncthreads();
+    int item = s_item[0];
+    for (int k = 1; item < hi; ++k) {
+        int next = 0;
+        if (threadIdx.x == 0) next = lo + atomicAdd(ctr, 1);
+        body(item);
+        // body ends with a barrier, after which every thread has read
+        // s_item[(k - 1) & 1]; slot k & 1 was last read before that barrier
+        if (threadIdx.x == 0) s_item[k & 1] = next;
+        __syncthreads();
+        item = s_item[k & 1];
+    }
 }
 
 }  // namespace
@@ -411,100 +705,165 @@ __global__ void ragged_reset_kernel(int* counters)
     if (threadIdx.x < 2) counters[threadIdx.x] = 0;
 }
 
-// Modes (RaggedMode): the wide refine form handles 16384-unknown systems
-// (16 unknowns per lane in the workgroup items); it needs more registers
-// than the others, so it is a separate instantiation and the common one
-// keeps its register budget.
+// Workgroup items [n_wide_items, n_wg_items), largest bucket first.
 template <int MODE>
-__global__ __launch_bounds__(kRaggedThreads) void ragged_kernel(RaggedArgs g)
+__global__ __launch_bounds__(kWgThreads) void ragged_wg_kernel(RaggedArgs g)
 {
-    constexpr bool DIRECT = MODE == kDirect;
-    __shared__ __align__(16) char smem[ragged_lds_bytes<MODE>()];
+    __shared__ __align__(16) char smem[wg_lds_bytes<MODE>()];
     __shared__ int s_item[2];
     __shared__ RaggedBucket s_bk[kRaggedMaxBuckets];
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x & (kWave - 1);
-    char* stage = smem + ragged_xchg_bytes<MODE>() + (size_t)wave * kStageWaveBytes;
+    char* area = smem + (size_t)wave * ragged_wave_bytes<MODE>();
+    char* xchg_base = smem + (size_t)kWgWaves * ragged_wave_bytes<MODE>();
     lds_poison_wg(smem, (unsigned)sizeof(smem));
     lds_poison_wg(s_bk, (unsigned)sizeof(s_bk));
     if (threadIdx.x < g.n_buckets) s_bk[threadIdx.x] = g.buckets[threadIdx.x];
-
-    // phase 1: workgroup items (long systems), largest bucket first. The
-    // next item's queue slot is claimed when an item starts and published
-    // when it ends, so the atomic's latency hides behind the item.
-    if (threadIdx.x == 0) s_item[0] = atomicAdd(&g.counters[0], 1);
-    __syncthreads();
-    int item = s_item[0];
-    for (int k = 1; item < g.n_wg_items; ++k) {
-        int next = 0;
-        if (threadIdx.x == 0) next = atomicAdd(&g.counters[0], 1);
+    drain_queue(g.n_wide_items, g.n_wg_items, &g.counters[1], s_item, [&](int item) {
         const RaggedItem it = g.items[item];
         const RaggedBucket bk = s_bk[it.bucket];
-        bool done = false;
-        if constexpr (MODE == kRefineWide) {
-            if (bk.per_lane == 16) {
-                wg_item<false, 16>(g, it, bk, wave, lane, smem, stage);
-                done = true;
-            }
-        }
-        if (!done) wg_item<DIRECT, 8>(g, it, bk, wave, lane, smem, stage);
-        // wg_item ends with a barrier, after which every thread has read
-        // s_item[(k - 1) & 1]; slot k & 1 was last read before that barrier
-        if (threadIdx.x == 0) s_item[k & 1] = next;
-        __syncthreads();
-        item = s_item[k & 1];
-    }
-    // phase 2: wave items, each wave on its own, dealt round-robin over every
-    // wave of the grid (largest bucket first, so each wave gets a similar
-    // mix). A shared queue counter here would be claimed by thousands of
-    // waves at once, and each claim would queue behind the others at one L2
-    // address: on 15,000 items of 512 unknowns that cost 5x the solve.
-    const int waves_total = (int)gridDim.x * kRaggedWaves;
-    for (int witem = (int)blockIdx.x * kRaggedWaves + wave; witem < g.n_wave_items;
+        if constexpr (MODE == kRefine) wg_item_resident(g, it, bk, wave, lane, xchg_base, area);
+        else wg_item_direct(g, it, bk, wave, lane, xchg_base, area);
+    });
+}
+
+// Wide items [0, n_wide_items) (refine mode).
+__global__ __launch_bounds__(kWgThreads) void ragged_wide_kernel(RaggedArgs g)
+{
+    __shared__ __align__(16) char smem[kWideLdsBytes];
+    __shared__ int s_item[2];
+    __shared__ RaggedBucket s_bk[kRaggedMaxBuckets];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    lds_poison_wg(smem, (unsigned)sizeof(smem));
+    lds_poison_wg(s_bk, (unsigned)sizeof(s_bk));
+    if (threadIdx.x < g.n_buckets) s_bk[threadIdx.x] = g.buckets[threadIdx.x];
+    drain_queue(0, g.n_wide_items, &g.counters[0], s_item, [&](int item) {
+        const RaggedItem it = g.items[item];
+        wide_item(g, it, s_bk[it.bucket], wave, lane, smem + kWideStageBytes,
+                  smem + (size_t)wave * RStage<16>::kBytes);
+    });
+}
+
+// Wave items, each wave on its own, dealt round-robin over every wave of the
+// grid (largest bucket first, so each wave gets a similar mix). A shared
+// queue counter here would be claimed by thousands of waves at once, and each
+// claim would queue behind the others at one L2 address: on 15,000 items of
+// 512 unknowns that cost 5x the solve.
+template <int MODE>
+__global__ __launch_bounds__(kWaveThreads) void ragged_wave_kernel(RaggedArgs g)
+{
+    __shared__ __align__(16) char smem[(size_t)kWaveWaves * ragged_wave_bytes<MODE>()];
+    __shared__ RaggedBucket s_bk[kRaggedMaxBuckets];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    char* area = smem + (size_t)wave * ragged_wave_bytes<MODE>();
+    lds_poison_wg(smem, (unsigned)sizeof(smem));
+    lds_poison_wg(s_bk, (unsigned)sizeof(s_bk));
+    if (threadIdx.x < g.n_buckets) s_bk[threadIdx.x] = g.buckets[threadIdx.x];
+    __syncthreads();
+    const int waves_total = (int)gridDim.x * kWaveWaves;
+    for (int witem = (int)blockIdx.x * kWaveWaves + wave; witem < g.n_wave_items;
          witem += waves_total) {
         const RaggedItem it = g.items[g.n_wg_items + witem];
         const RaggedBucket bk = s_bk[it.bucket];
-        switch (bk.per_lane) {
-            case 1: wave_item<DIRECT, 1>(g, it, bk, lane, stage); break;
-            case 2: wave_item<DIRECT, 2>(g, it, bk, lane, stage); break;
-            case 4: wave_item<DIRECT, 4>(g, it, bk, lane, stage); break;
-            case 8: wave_item<DIRECT, 8>(g, it, bk, lane, stage); break;
-            default: break;
+        if constexpr (MODE == kRefine) {
+            switch (bk.per_lane) {
+                case 1: wave_item_resident<1>(g, it, bk, lane, area); break;
+                case 2: wave_item_resident<2>(g, it, bk, lane, area); break;
+                case 4: wave_item_resident<4>(g, it, bk, lane, area); break;
+                case 8: wave_item_resident<8>(g, it, bk, lane, area); break;
+                default: break;
+            }
+        } else {
+            switch (bk.per_lane) {
+                case 1: wave_item_direct<1>(g, it, bk, lane, area); break;
+                case 2: wave_item_direct<2>(g, it, bk, lane, area); break;
+                case 4: wave_item_direct<4>(g, it, bk, lane, area); break;
+                case 8: wave_item_direct<8>(g, it, bk, lane, area); break;
+                default: break;
+            }
         }
     }
 }
 
-hipError_t launch_ragged(const RaggedArgs& g, int grid, hipStream_t stream)
-{
-    hipLaunchKernelGGL(ragged_reset_kernel, dim3(1), dim3(64), 0, stream, g.counters);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    if (g.direct) {
-        if (g.wide) return hipErrorInvalidValue;  // no 16384-unknown FP64 form
-        hipLaunchKernelGGL(ragged_kernel<kDirect>, dim3((unsigned)grid), dim3(kRaggedThreads), 0,
-                           stream, g);
-    } else if (g.wide) {
-        hipLaunchKernelGGL(ragged_kernel<kRefineWide>, dim3((unsigned)grid),
-                           dim3(kRaggedThreads), 0, stream, g);
-    } else {
-        hipLaunchKernelGGL(ragged_kernel<kRefine>, dim3((unsigned)grid), dim3(kRaggedThreads), 0,
-                           stream, g);
-    }
-    return hipGetLastError();
-}
+namespace {
 
-// Persistent grid: every workgroup slot of the device.
-int ragged_max_grid(int device)
+int slots_of(const void* kern, int threads, int device)
 {
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ragged_kernel<kRefine>,
-                                                     kRaggedThreads, 0) !=
-        hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, 0) != hipSuccess)
         per_cu = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         cus = 256;
     if (per_cu < 1) per_cu = 1;
     return per_cu * cus;
+}
+
+int clamp_grid(int64_t work, int slots)
+{
+    return (int)(work < slots ? (work > 0 ? work : 1) : slots);
+}
+
+}  // namespace
+
+RaggedGrids ragged_grids(int device, const RaggedArgs& g)
+{
+    // workgroup slots of each kernel on each device (the current device when
+    // first asked), found once
+    static std::mutex mu;
+    static int cache[64][5];
+    int slots[5];
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        int* c = cache[device >= 0 && device < 64 ? device : 0];
+        if (!c[0]) {
+            c[1] = slots_of((const void*)ragged_wg_kernel<kRefine>, kWgThreads, device);
+            c[2] = slots_of((const void*)ragged_wg_kernel<kDirect>, kWgThreads, device);
+            c[3] = slots_of((const void*)ragged_wide_kernel, kWgThreads, device);
+            c[4] = slots_of((const void*)ragged_wave_kernel<kRefine>, kWaveThreads, device);
+            c[0] = slots_of((const void*)ragged_wave_kernel<kDirect>, kWaveThreads, device);
+        }
+        for (int k = 0; k < 5; ++k) slots[k] = c[k];
+    }
+    RaggedGrids r{};
+    r.wide = clamp_grid(g.n_wide_items, slots[3]);
+    r.wg = clamp_grid(g.n_wg_items - g.n_wide_items, g.direct ? slots[2] : slots[1]);
+    r.wave = clamp_grid((g.n_wave_items + kWaveWaves - 1) / kWaveWaves,
+                        g.direct ? slots[0] : slots[4]);
+    return r;
+}
+
+hipError_t launch_ragged(const RaggedArgs& g, const RaggedGrids& grid, hipStream_t stream)
+{
+    if (g.direct && g.n_wide_items > 0) return hipErrorInvalidValue;  // no 16384 FP64 form
+    hipLaunchKernelGGL(ragged_reset_kernel, dim3(1), dim3(64), 0, stream, g.counters);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (g.n_wide_items > 0) {
+        hipLaunchKernelGGL(ragged_wide_kernel, dim3((unsigned)grid.wide), dim3(kWgThreads), 0,
+                           stream, g);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (g.n_wg_items > g.n_wide_items) {
+        if (g.direct)
+            hipLaunchKernelGGL(ragged_wg_kernel<kDirect>, dim3((unsigned)grid.wg),
+                               dim3(kWgThreads), 0, stream, g);
+        else
+            hipLaunchKernelGGL(ragged_wg_kernel<kRefine>, dim3((unsigned)grid.wg),
+                               dim3(kWgThreads), 0, stream, g);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (g.n_wave_items > 0) {
+        if (g.direct)
+            hipLaunchKernelGGL(ragged_wave_kernel<kDirect>, dim3((unsigned)grid.wave),
+                               dim3(kWaveThreads), 0, stream, g);
+        else
+            hipLaunchKernelGGL(ragged_wave_kernel<kRefine>, dim3((unsigned)grid.wave),
+                               dim3(kWaveThreads), 0, stream, g);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace trid

@@ -194,19 +194,28 @@ struct RaggedArgs {
     const double* d;
     double* x;
     const RaggedSys* sys;     // device, numSystems, grouped by bucket
-    const RaggedItem* items;  // device, workgroup items then wave items
+    const RaggedItem* items;  // device: wide items, workgroup items, then wave items
     const RaggedBucket* buckets;
-    int* counters;            // device, 2 queue counters
-    int n_wg_items;
+    int* counters;            // device, 2 queue counters (wide items, workgroup items)
+    int n_wide_items;         // items of 8193..16384-unknown systems (the first items)
+    int n_wg_items;           // wide and workgroup items together
     int n_wave_items;
     int n_buckets;            // <= kRaggedMaxBuckets
     int iters;
-    int wide;                 // 1 if any 16384-unknown bucket is present
     int direct;               // 1: one FP64 solve per system (no FP32 stage)
 };
 
-hipError_t launch_ragged(const RaggedArgs& g, int grid, hipStream_t stream);
-int ragged_max_grid(int device);
+// Grids of the three persistent kernels (wide items, workgroup items, wave
+// items) for a batch: every slot of the device, or fewer when the batch has
+// fewer items.
+struct RaggedGrids {
+    int wide;
+    int wg;
+    int wave;
+};
+RaggedGrids ragged_grids(int device, const RaggedArgs& g);
+// Reset of the queue counters, then the kernels of the item classes present.
+hipError_t launch_ragged(const RaggedArgs& g, const RaggedGrids& grid, hipStream_t stream);
 
 // Host side of a ragged plan (trid_ragged.cpp): systems grouped by padded
 // size, the bucket table and the work items of the persistent kernel.
@@ -215,9 +224,10 @@ constexpr int kRaggedMaxBuckets = 16;  // padded sizes 2 .. 16384
 struct RaggedHostPlan {
     std::vector<int32_t> order;
     std::vector<RaggedSys> sys;     // systems in `order`
-    std::vector<RaggedItem> items;  // workgroup items, then wave items
+    std::vector<RaggedItem> items;  // wide items, workgroup items, then wave items
     std::vector<RaggedBucket> buckets;
     std::vector<int64_t> bucket_systems;
+    int n_wide_items = 0;           // of the n_wg_items, those of 16 unknowns per lane
     int n_wg_items = 0;
     int n_wave_items = 0;
 };

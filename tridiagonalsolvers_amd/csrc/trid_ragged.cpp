@@ -24,10 +24,12 @@ struct trid_ragged_plan_s {
     int64_t total = 0;
     int device = 0;
     int grid = 0;
+    int n_wide_items = 0;
     int n_wg_items = 0;
     int n_wave_items = 0;
     int n_buckets = 0;
-    int wide = 0;
+    trid::RaggedGrids grids{};  // of the refine mode and of the FP64 mode
+    trid::RaggedGrids grids_direct{};
     std::vector<trid::RaggedBucket> buckets;  // host copy (info)
     std::vector<int64_t> bucket_systems;
     char* dev = nullptr;  // one device block: systems, items, buckets, counters
@@ -149,6 +151,9 @@ int ragged_build_host(const int64_t* offsets, int64_t N, const trid_bucket_cfg* 
         pos = end;
     }
     out->n_wg_items = (int)wg_items.size();
+    out->n_wide_items = 0;  // the largest buckets come first
+    for (const RaggedItem& it : wg_items)
+        if (out->buckets[it.bucket].per_lane == 16) ++out->n_wide_items;
     out->n_wave_items = (int)wave_items.size();
     out->items = wg_items;
     out->items.insert(out->items.end(), wave_items.begin(), wave_items.end());
@@ -369,10 +374,9 @@ int trid_ragged_plan_create(trid_ragged_plan* out, const int64_t* offsets, int64
     p->buckets = hp.buckets;
     p->bucket_systems = hp.bucket_systems;
     p->n_buckets = (int)hp.buckets.size();
+    p->n_wide_items = hp.n_wide_items;
     p->n_wg_items = hp.n_wg_items;
     p->n_wave_items = hp.n_wave_items;
-    for (const auto& b : hp.buckets)
-        if (b.per_lane == 16) p->wide = 1;
     const std::vector<trid::RaggedItem>& items = hp.items;
     const std::vector<trid::RaggedBucket>& used = hp.buckets;
     // device block
@@ -402,9 +406,14 @@ int trid_ragged_plan_create(trid_ragged_plan* out, const int64_t* offsets, int64
         return TRID_EHIP;
     }
     (void)hipGetDevice(&p->device);
-    p->grid = trid::ragged_max_grid(p->device);
-    const int64_t work = (int64_t)p->n_wg_items + (p->n_wave_items + 15) / 16;
-    if (work < p->grid) p->grid = (int)std::max<int64_t>(1, work);
+    trid::RaggedArgs g{};
+    g.n_wide_items = p->n_wide_items;
+    g.n_wg_items = p->n_wg_items;
+    g.n_wave_items = p->n_wave_items;
+    p->grids = trid::ragged_grids(p->device, g);
+    g.direct = 1;
+    p->grids_direct = trid::ragged_grids(p->device, g);
+    p->grid = std::max(p->grids.wg, std::max(p->grids.wide, p->grids.wave));
     *out = p;
     return TRID_OK;
 }
@@ -448,7 +457,7 @@ int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* 
     if (flags & ~(TRID_RAGGED_NO_GRAPH | TRID_RAGGED_FP64)) return TRID_EINVAL;
     const int direct = (flags & TRID_RAGGED_FP64) ? 1 : 0;
     if (direct && iters != 0) return TRID_EINVAL;
-    if (direct && plan->wide) return TRID_ENOTSUP;
+    if (direct && plan->n_wide_items > 0) return TRID_ENOTSUP;
     trid::RaggedArgs g{};
     g.a = a;
     g.b = b;
@@ -459,15 +468,16 @@ int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* 
     g.items = plan->d_items;
     g.buckets = plan->d_buckets;
     g.counters = plan->d_counters;
+    g.n_wide_items = plan->n_wide_items;
     g.n_wg_items = plan->n_wg_items;
     g.n_wave_items = plan->n_wave_items;
     g.n_buckets = plan->n_buckets;
     g.iters = iters;
-    g.wide = plan->wide;
     g.direct = direct;
+    const trid::RaggedGrids& grids = direct ? plan->grids_direct : plan->grids;
     hipStream_t s = static_cast<hipStream_t>(stream);
     if ((flags & TRID_RAGGED_NO_GRAPH) || s == nullptr) {
-        hipError_t e = trid::launch_ragged(g, plan->grid, s);
+        hipError_t e = trid::launch_ragged(g, grids, s);
         return e == hipSuccess ? TRID_OK : TRID_EHIP;
     }
     const void* key[5] = {a, b, c, d, x};
@@ -481,7 +491,7 @@ int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* 
         plan->graph = nullptr;
         hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return TRID_EHIP;
-        hipError_t le = trid::launch_ragged(g, plan->grid, s);
+        hipError_t le = trid::launch_ragged(g, grids, s);
         e = hipStreamEndCapture(s, &plan->graph);
         if (le != hipSuccess || e != hipSuccess) {
             if (plan->graph) (void)hipGraphDestroy(plan->graph);

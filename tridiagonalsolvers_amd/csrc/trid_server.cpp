@@ -268,7 +268,6 @@ struct trid_server_s {
     size_t in_use = 0;
     trid_server_stats st{};
     std::vector<trid_bucket_cfg> table;
-    int grid = 1;
 
     size_t budget_free() const
     {
@@ -415,8 +414,6 @@ int trid_server_s::launch(Slot& s, std::unique_lock<std::mutex>& lk)
                                                                  q->pos * sizeof(double)),
                                 q->pos, nb});
     }
-    const int64_t work = (int64_t)hp.n_wg_items + (hp.n_wave_items + 15) / 16;
-    const int grid_now = work < grid ? (int)std::max<int64_t>(1, work) : grid;
     char* D = d_pool + s.dev_off;
 
     lk.unlock();
@@ -450,14 +447,12 @@ int trid_server_s::launch(Slot& s, std::unique_lock<std::mutex>& lk)
         g.items = reinterpret_cast<const trid::RaggedItem*>(desc + L.sys);
         g.buckets = reinterpret_cast<const trid::RaggedBucket*>(desc + L.sys + L.itm);
         g.counters = reinterpret_cast<int*>(desc + L.sys + L.itm + L.bkt);
+        g.n_wide_items = hp.n_wide_items;
         g.n_wg_items = hp.n_wg_items;
         g.n_wave_items = hp.n_wave_items;
         g.n_buckets = (int)hp.buckets.size();
         g.iters = o.iters;
-        g.wide = 0;
-        for (const auto& b : hp.buckets)
-            if (b.per_lane == 16) g.wide = 1;
-        e = trid::launch_ragged(g, grid_now, s.stream);
+        e = trid::launch_ragged(g, trid::ragged_grids(device, g), s.stream);
     }
     for (const XCopy& c : xcopies) {
         if (e != hipSuccess) break;
@@ -711,7 +706,6 @@ int trid_server_create(trid_server* out, const trid_server_opts* opts)
             trid_server_destroy(s);
             return st;
         }
-        s->grid = trid::ragged_max_grid(s->device);
     }
     s->st.budget_bytes = o.hbm_budget_bytes;
     s->th = std::thread([s] { s->run(); });
