@@ -74,7 +74,7 @@ typedef struct trid_opts {
 } trid_opts;
 
 /* PCRGAUSS: use the two-pass form (global PCR pass, then strided Thomas pass
- * through a handle workspace of 4*n*N elements) even when the system fits
+ * through a handle workspace of 3*n*N elements) even when the system fits
  * one workgroup's LDS. It is always used when it does not fit. */
 #define TRID_FLAG_TWO_PASS 1
 
@@ -184,8 +184,8 @@ int trid_solve_f64(trid_handle handle, int method,
  * TRID_NOALLOC_SCRATCH_BYTES per device, part of the library and held until
  * the process ends; a request that needs more returns TRID_ENOMEM before
  * anything is enqueued (64 MiB holds the separator records of 64 systems of
- * 2^20 unknowns in FP32 or FP64, and the two-pass pcrGauss workspace of 2^22
- * FP32 / 2^21 FP64 unknowns). The call allocates nothing; its first call on
+ * 2^20 unknowns in FP32 or FP64, and the two-pass pcrGauss workspace of
+ * 5.6e6 FP32 / 2.8e6 FP64 unknowns). The call allocates nothing; its first call on
  * a device looks up that device's array address. Every call on one device
  * shares the array and finds the previous call's contents (no kernel here
  * needs it cleared): calls that overlap on one device (different streams or

@@ -75,6 +75,8 @@ def test_gauss_sweep_line_reports_the_separator_form():
     for k in KEYS:
         assert k in j, k
     assert [r["depth"] for r in j["sweep"] if r["in_sweep"]] == list(range(2, 9))
+    for r in j["sweep"]:      # every depth against FP64 Thomas at full size
+        assert r["max_rel_err_vs_fp64_thomas"] <= 1e-4 and r["reference_rms_failures"] == 0
     sf = j["separator_form"]
     assert sf["family"] == 3 and sf["ms"] > 0
     assert sf["max_rel_err_vs_fp64_thomas"] <= 1e-4 and sf["reference_rms_failures"] == 0

@@ -27,10 +27,10 @@ def _noalloc(trid, method, dev, x, n, N, opts=None):
 
 def test_noalloc_rejects_requests_beyond_the_scratch_array(trid):
     """Size check first: no device is needed to see the refusal."""
-    # two-pass pcrGauss needs 4 n N elements: 65,536 x 128 FP32 = 128 MiB
+    # two-pass pcrGauss needs 3 n N elements: 65,536 x 128 FP32 = 96 MiB
     o = trid.make_opts(gaussian=16, flags=1)
     p = ctypes.c_void_p(256)
-    assert 4 * 65536 * 128 * 4 > SCRATCH
+    assert 3 * 65536 * 128 * 4 > SCRATCH
     st = trid.lib().trid_solve_noalloc(trid.METHODS["pcrgauss"], 4, p, p, p, p, p, 65536, 128,
                                        ctypes.byref(o), None)
     assert st == ENOMEM

@@ -138,7 +138,10 @@ __global__ __launch_bounds__(1024) void pcr_thomas_lds_kernel(const T* __restric
 // thread FP32, 10 FP64) or LDS; TRID_PCRG_THREADS threads per workgroup.
 // Measured on 64 x 2^20 FP32 (depth 2..10): 2048 / 8 G / 512 threads is
 // the best of the shapes tried (tiles of 2048..8192 equations, 256..1024
-// threads).
+// threads; with the normalized output at depth 8: 4096 / 16 G 1.27 ms,
+// 1024 threads with 4096 / 16 G 1.13 ms or 8192 / 32 G 1.44 ms, against
+// 1.14 ms): the levels' LDS records (16 bytes written, 32 read per equation
+// and level) bound the pass, and larger tiles leave fewer workgroups per CU.
 #ifndef TRID_PCRG_TILE_MIN
 #define TRID_PCRG_TILE_MIN 2048
 #endif
@@ -153,13 +156,14 @@ __global__ __launch_bounds__(1024) void pcr_thomas_lds_kernel(const T* __restric
 // G-1 on each side. Each thread keeps its equations (p = tid + j*nt of the
 // window) in registers; a level publishes them to LDS as one record per
 // equation, and reads the two neighbours back (48 bytes of LDS traffic per
-// equation and level for FP32). Output (a', b', c', d') goes to the
-// workspace arrays.
+// equation and level for FP32). Output: each reduced row divided by its
+// diagonal, (a'/b', c'/b', d'/b') in three workspace arrays (A2, C2, D2): the
+// second pass treats the diagonal as 1, so b' is neither written nor read.
 template <typename T, int KPT>
 __global__ __launch_bounds__(TRID_PCRG_THREADS) void pcr_global_levels_kernel(
     const T* __restrict__ A, const T* __restrict__ B, const T* __restrict__ C,
-    const T* __restrict__ D, T* __restrict__ A2, T* __restrict__ B2, T* __restrict__ C2,
-    T* __restrict__ D2, int64_t n, int G, int tile, int64_t tiles)
+    const T* __restrict__ D, T* __restrict__ A2, T* __restrict__ C2, T* __restrict__ D2,
+    int64_t n, int G, int tile, int64_t tiles)
 {
     extern __shared__ __align__(16) char trid_smem[];
     const int H = G - 1;
@@ -212,10 +216,10 @@ __global__ __launch_bounds__(TRID_PCRG_THREADS) void pcr_global_levels_kernel(
         const int p = tid + j * nt;
         const int64_t g = t0 - H + p;
         if (p >= H && p < H + tile && g < n) {
-            A2[base + g] = own[j].a;
-            B2[base + g] = own[j].b;
-            C2[base + g] = own[j].c;
-            D2[base + g] = own[j].d;
+            const T ib = tdiv(T(1), own[j].b);
+            A2[base + g] = own[j].a * ib;
+            C2[base + g] = own[j].c * ib;
+            D2[base + g] = own[j].d * ib;
         }
     }
 }
@@ -249,7 +253,11 @@ __global__ __launch_bounds__(TRID_PCRG_THREADS) void pcr_global_levels_kernel(
 // point  x_{h-1} + c' x_h = d'_top  and  x_h + a' x_{h-1} = d'_bottom.
 // The factors q', d' go to FC, FD (which may be C, D themselves, or X for
 // FD: each location is read before the same lane overwrites it).
-template <typename T, int U>
+//
+// NORM: the rows are normalized (diagonal 1, B not read; the pcrGauss
+// second pass): den = 1 - p q'_{i-1}. The backward sweep streams UB steps
+// of factors ahead (2 loads and 1 store per step share the same counter).
+template <typename T, int U, int UB, bool NORM>
 __global__ __launch_bounds__(64) void thomas_strided_kernel(const T* A, const T* B, const T* C,
                                                             const T* D, T* FC, T* FD, T* X,
                                                             int64_t block_len, int64_t blocks,
@@ -284,7 +292,7 @@ __global__ __launch_bounds__(64) void thomas_strided_kernel(const T* A, const T*
             const int64_t i = i0 + u;
             if (i < len) {
                 np[u] = pp[i * step];
-                nb[u] = pb[i * step];
+                if constexpr (!NORM) nb[u] = pb[i * step];
                 nq[u] = pq[i * step];
                 nd[u] = pd[i * step];
             }
@@ -296,7 +304,7 @@ __global__ __launch_bounds__(64) void thomas_strided_kernel(const T* A, const T*
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             tp[u] = np[u];
-            tb[u] = nb[u];
+            tb[u] = NORM ? T(1) : nb[u];
             tq[u] = nq[u];
             td[u] = nd[u];
         }
@@ -307,8 +315,14 @@ __global__ __launch_bounds__(64) void thomas_strided_kernel(const T* A, const T*
             if (i < len) {
                 const T pk = i > 0 ? tp[u] : T(0);
                 const T den = tb[u] - pk * qv;
-                qv = tdiv(tq[u], den);
-                dv = tdiv(td[u] - dv * pk, den);
+                if constexpr (NORM) {
+                    const T iden = tdiv(T(1), den);
+                    qv = tq[u] * iden;
+                    dv = (td[u] - dv * pk) * iden;
+                } else {
+                    qv = tdiv(tq[u], den);
+                    dv = tdiv(td[u] - dv * pk, den);
+                }
                 wq[i * step] = qv;
                 wd[i * step] = dv;
             }
@@ -328,27 +342,28 @@ __global__ __launch_bounds__(64) void thomas_strided_kernel(const T* A, const T*
     }
     if (len > 0) px[(len - 1) * step] = xv;
     // back-substitution, i = len-2 .. 0
+    T bq[UB], bd[UB];
     auto fetch_back = [&](int64_t i0) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
+        for (int u = 0; u < UB; ++u) {
             const int64_t i = i0 - u;
             if (i >= 0) {
-                nq[u] = wq[i * step];
-                nd[u] = wd[i * step];
+                bq[u] = wq[i * step];
+                bd[u] = wd[i * step];
             }
         }
     };
     fetch_back(len - 2);
-    for (int64_t i0 = len - 2; i0 >= 0; i0 -= U) {
-        T tq[U], td[U];
+    for (int64_t i0 = len - 2; i0 >= 0; i0 -= UB) {
+        T tq[UB], td[UB];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            tq[u] = nq[u];
-            td[u] = nd[u];
+        for (int u = 0; u < UB; ++u) {
+            tq[u] = bq[u];
+            td[u] = bd[u];
         }
-        fetch_back(i0 - U);
+        fetch_back(i0 - UB);
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
+        for (int u = 0; u < UB; ++u) {
             const int64_t i = i0 - u;
             if (i >= 0) {
                 xv = td[u] - tq[u] * xv;
@@ -409,7 +424,8 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-template <typename T>
+// NORM as in thomas_strided_kernel (B not read, diagonal 1).
+template <typename T, bool NORM>
 __global__ __launch_bounds__(kStagedWaves * 64) void strided_staged_kernel(
     const T* A, const T* B, const T* C, const T* D, T* FC, T* FD, T* X, int64_t block_len,
     int64_t blocks, int64_t chains, int64_t stride)
@@ -455,12 +471,18 @@ __global__ __launch_bounds__(kStagedWaves * 64) void strided_staged_kernel(
                 const int64_t i = (int64_t)blk * kRowsBlk + row;
                 if (i < len) {
                     const T pk = i > 0 ? op[row * kStagedChains] : T(0);
-                    const T bk = op[kArrStride + row * kStagedChains];
+                    const T bk = NORM ? T(1) : op[kArrStride + row * kStagedChains];
                     const T qk = op[2 * kArrStride + row * kStagedChains];
                     const T dk = op[3 * kArrStride + row * kStagedChains];
                     const T den = bk - pk * qv;
-                    qv = tdiv(qk, den);
-                    dv = tdiv(dk - dv * pk, den);
+                    if constexpr (NORM) {
+                        const T iden = tdiv(T(1), den);
+                        qv = qk * iden;
+                        dv = (dk - dv * pk) * iden;
+                    } else {
+                        qv = tdiv(qk, den);
+                        dv = tdiv(dk - dv * pk, den);
+                    }
                     FC[o0 + i * step] = qv;
                     FD[o0 + i * step] = dv;
                 }
@@ -506,11 +528,13 @@ __global__ __launch_bounds__(kStagedWaves * 64) void strided_staged_kernel(
         const T* ptr = base + s * block_len + r + (top ? 0 : m - 1) * stride;
         T* dst0 = &slot[0][0] + arr * kArrStride + (lh & 1) * kRowsBlk * kStagedChains + cl;
         T pre[PF][kRowsBlk];
+        // (NORM: the half-waves of the diagonal have nothing to stream)
+        const bool active = live && !(NORM && arr == 1);
         auto fetch = [&](int blk, T (&v)[kRowsBlk]) {
 #pragma unroll
             for (int row = 0; row < kRowsBlk; ++row) {
                 const int64_t i = (int64_t)blk * kRowsBlk + row;
-                v[row] = (live && i < len) ? ptr[i * step] : T(0);
+                v[row] = (active && i < len) ? ptr[i * step] : T(0);
             }
         };
 #pragma unroll
@@ -583,7 +607,23 @@ static int device_cus()
     return cached[dev];
 }
 
-template <typename T>
+// Steps of loads in flight per lane of the direct kernel, forward (4 loads,
+// 3 when normalized, and 2 stores per step) and backward (2 loads, 1 store),
+// within the 63 memory operations a wave can have outstanding. Measured on
+// 64 x 2^20 FP32 at depth 8 (normalized rows): forward / backward 8 / 8:
+// 1.142 ms, 10 / 12: 1.094 ms, 12 / 16: 1.076 ms (4 rows per step as before
+// the normalization: 1.208 ms).
+#ifndef TRID_STRIDED_U
+#define TRID_STRIDED_U 8
+#endif
+#ifndef TRID_STRIDED_UN
+#define TRID_STRIDED_UN 12
+#endif
+#ifndef TRID_STRIDED_UB
+#define TRID_STRIDED_UB 16
+#endif
+
+template <typename T, bool NORM>
 static hipError_t launch_strided(const T* A, const T* B, const T* C, const T* D, T* FC, T* FD,
                                  T* X, int64_t block_len, int64_t blocks, int64_t chains,
                                  int64_t stride, hipStream_t stream)
@@ -597,15 +637,19 @@ static hipError_t launch_strided(const T* A, const T* B, const T* C, const T* D,
         total / kStagedChains <= (int64_t)TRID_STAGED_MAX_WG_PER_CU * device_cus()) {
         const int64_t grid = total / kStagedChains;
         if (grid > 0x7fffffffLL) return hipErrorInvalidConfiguration;
-        hipLaunchKernelGGL((strided_staged_kernel<T>), dim3((unsigned)grid),
+        hipLaunchKernelGGL((strided_staged_kernel<T, NORM>), dim3((unsigned)grid),
                            dim3(kStagedWaves * kWave), 0,
                            stream, A, B, C, D, FC, FD, X, block_len, blocks, chains, stride);
         return hipGetLastError();
     }
     const int64_t grid = (total + kWave / 2 - 1) / (kWave / 2);
     if (grid > 0x7fffffffLL) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL((thomas_strided_kernel<T, 8>), dim3((unsigned)grid), dim3(kWave), 0, stream,
-                       A, B, C, D, FC, FD, X, block_len, blocks, chains, stride);
+    constexpr int U = NORM ? TRID_STRIDED_UN : TRID_STRIDED_U;
+    static_assert((NORM ? 3 : 4) * U + 2 * U <= 63 && 3 * TRID_STRIDED_UB <= 63,
+                  "outstanding memory operations of a wave");
+    hipLaunchKernelGGL((thomas_strided_kernel<T, U, TRID_STRIDED_UB, NORM>), dim3((unsigned)grid),
+                       dim3(kWave), 0, stream, A, B, C, D, FC, FD, X, block_len, blocks, chains,
+                       stride);
     return hipGetLastError();
 }
 
@@ -620,8 +664,8 @@ hipError_t launch_interleaved(const InterleavedArgs<T>& g)
     if (taken || e != hipSuccess) return e;
     // otherwise one block of n * stride elements whose N chains are the
     // systems; the c' factors go to the workspace, the d' factors to x
-    return launch_strided<T>(g.a, g.b, g.c, g.d, g.ws, g.x, g.x, g.n * g.stride, 1, g.N,
-                             g.stride, g.stream);
+    return launch_strided<T, false>(g.a, g.b, g.c, g.d, g.ws, g.x, g.x, g.n * g.stride, 1, g.N,
+                                    g.stride, g.stream);
 }
 
 template <typename T>
@@ -653,11 +697,11 @@ hipError_t launch_pcr_thomas(const PcrThomasArgs<T>& g)
         if (kpt <= 8) return go(pcr_thomas_lds_kernel<T, 8>);
         return hipErrorInvalidValue;
     }
-    // two passes through the workspace (4 arrays of n*N elements)
+    // two passes through the workspace (3 arrays of n*N elements: the
+    // normalized rows)
     const int64_t total = g.n * g.N;
     T* A2 = g.ws;
-    T* B2 = A2 + total;
-    T* C2 = B2 + total;
+    T* C2 = A2 + total;
     T* D2 = C2 + total;
     const int G = g.G;
     const int nt = TRID_PCRG_THREADS;
@@ -681,7 +725,7 @@ hipError_t launch_pcr_thomas(const PcrThomasArgs<T>& g)
                 if (e != hipSuccess) return e;
             }
             hipLaunchKernelGGL(kern, dim3((unsigned)(tiles * g.N)), dim3(nt), lds, g.stream,
-                               g.a, g.b, g.c, g.d, A2, B2, C2, D2, g.n, G, tile, tiles);
+                               g.a, g.b, g.c, g.d, A2, C2, D2, g.n, G, tile, tiles);
             return hipGetLastError();
         };
         // G = 1 runs no PCR level: pass 1 only copies (and zeroes a[0], c[n-1])
@@ -694,7 +738,7 @@ hipError_t launch_pcr_thomas(const PcrThomasArgs<T>& g)
         else return hipErrorInvalidValue;
         if (e != hipSuccess) return e;
     }
-    return launch_strided<T>(A2, B2, C2, D2, C2, D2, g.x, g.n, g.N, G, G, g.stream);
+    return launch_strided<T, true>(A2, nullptr, C2, D2, C2, D2, g.x, g.n, g.N, G, G, g.stream);
 }
 
 template hipError_t launch_pcr_thomas<float>(const PcrThomasArgs<float>&);

@@ -83,8 +83,16 @@ __device__ __forceinline__ void reg_mw_body(const T* __restrict__ A, const T* __
     __syncthreads();  // exchange area becomes staging rows again
     if constexpr (M % E == 0 && IO >= 2) {
         if (vec) {
-            store_staged<T, M, IO == 3>(X, base + (int64_t)w * kWave * M, total, lane, lds_stage,
-                                        xv);
+            // the store addresses are formed here from the thread index
+            // (cheap); kept from before the solve they would stay live
+            // across it, and at 16 unknowns per lane the core needs every
+            // register
+            int tid = (int)threadIdx.x;
+            asm volatile("" : "+v"(tid));
+            const int w2 = tid / kWave;
+            char* stage2 = trid_smem + w2 * Stage<T, M>::kWaveBytes;
+            store_staged<T, M, IO == 3>(X, (int64_t)blockIdx.x * bs + (int64_t)w2 * kWave * M,
+                                        (N - 1) * bs + n, tid & (kWave - 1), stage2, xv);
             return;
         }
     } else if constexpr (M % E == 0) {

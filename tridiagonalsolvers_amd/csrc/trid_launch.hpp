@@ -71,7 +71,7 @@ struct PcrThomasArgs {
     int G;          // subsystems; PCR depth log2(G)
     int pad;        // in-LDS form: pad period in elements, 0 = none
     int two_pass;   // 1: global PCR pass + strided Thomas pass through `ws`
-    T* ws;          // two-pass workspace: 4 * n * N elements
+    T* ws;          // two-pass workspace: 3 * n * N elements
     hipStream_t stream;
 };
 
