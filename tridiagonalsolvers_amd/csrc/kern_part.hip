@@ -226,7 +226,7 @@ __global__ __launch_bounds__(256) void part_solve_kernel(const T* __restrict__ A
     if (lane == kWave - 1) e[kPM - 1] = Eq<T>{T(0), T(1), T(0), xr};
 
     T xv[kPM];
-    wave_core<T, kPM, R>(e, xv, lane, lane, kWave, rest);
+    wave_core<T, kPM, R, true>(e, xv, lane, lane, kWave, rest);
 
     if (vec) {
         store_staged<T, kPM, true>(X, base, base + rows, lane, lds, xv);

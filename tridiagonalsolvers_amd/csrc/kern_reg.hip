@@ -15,8 +15,10 @@
 //   5. CR back-substitution across lanes, then inside lanes
 //
 // No LDS storage and no barrier is used. The arithmetic of each step is the
-// reference's (reduce_eq / solve2 / backsub in trid_device.hpp), and the
-// sequence of steps is exactly CR-PCR with switch size `rest`:
+// reference's (reduce_eq / solve2 / backsub in trid_device.hpp; PCR levels
+// past one equation per lane run on rows scaled to a unit diagonal, the same
+// elimination divided through: reduce_neq), and the sequence of
+// steps is exactly CR-PCR with switch size `rest`:
 // rest = 2 is CR (crpcr with no PCR level equals cr_kernel.cu), rest = n is
 // PCR (pcr_kernel.cu: log2(n/2) PCR levels, then n/2 2x2 solves).
 //
@@ -33,8 +35,8 @@ namespace trid {
 
 // T: element type. M: unknowns per lane. R: equations per lane handed to PCR
 // (R = 1: the PCR system has at most one equation per lane, rest <= L;
-//  R > 1: rest = R * L).
-template <typename T, int M, int R, int IO>
+//  R > 1: rest = R * L). FULL (R > 1 only): one system per wave, L = 64.
+template <typename T, int M, int R, int IO, bool FULL>
 __global__ __launch_bounds__(256) void reg_crpcr_kernel(const T* __restrict__ A,
                                                         const T* __restrict__ B,
                                                         const T* __restrict__ C,
@@ -61,20 +63,23 @@ __global__ __launch_bounds__(256) void reg_crpcr_kernel(const T* __restrict__ A,
     const int64_t total = (N - 1) * bs + n;
     const int seg_log2 = (bs == n || L == kWave) ? -1 : logL + Log2<M>::value;
 
+    // Batches that are not 16-byte aligned (vec = 0) are launched with IO = 0,
+    // the one instantiation that carries the element-wise path: a second
+    // load path merged into the others raises their register pressure
+    // through the whole solve (FP32 PCR: 256 VGPRs with it, 121 without).
+    const bool scalar = M % E != 0 || (IO == 0 && !vec);
     Eq<T> e[M];
     if constexpr (M % E == 0 && IO >= 2) {
-        if (vec) {
-            load_staged<T, M, IO == 3>(A, B, C, D, wave_base, total, lane, lds, e,
-                                       kNoConstCoef<T>, seg_log2, bs);
-            if (!live) set_identity<T, M>(e);
-        }
+        load_staged<T, M, IO == 3>(A, B, C, D, wave_base, total, lane, lds, e, kNoConstCoef<T>,
+                                   seg_log2, bs);
+        if (!live) set_identity<T, M>(e);
     } else if constexpr (M % E == 0) {
-        if (vec) {
+        if (!scalar) {
             if (live) load_direct<T, M, IO == 0>(A, B, C, D, base + j0, e, kNoConstCoef<T>);
             else set_identity<T, M>(e);
         }
     }
-    if (!vec || M % E != 0) {
+    if (scalar) {
 #pragma unroll
         for (int s = 0; s < M; ++s) {
             const int64_t j = j0 + s;
@@ -91,15 +96,13 @@ __global__ __launch_bounds__(256) void reg_crpcr_kernel(const T* __restrict__ A,
         if (j0 + s == n - 1) e[s].c = T(0);
 
     T xv[M];
-    wave_core<T, M, R>(e, xv, lane, ll, L, rest);
+    wave_core<T, M, R, FULL>(e, xv, lane, ll, L, rest);
 
     if constexpr (M % E == 0 && IO >= 2) {
-        if (vec) {
-            store_staged<T, M, IO == 3>(X, wave_base, total, lane, lds, xv, seg_log2, bs);
-            return;
-        }
+        store_staged<T, M, IO == 3>(X, wave_base, total, lane, lds, xv, seg_log2, bs);
+        return;
     } else if constexpr (M % E == 0) {
-        if (vec) {
+        if (!scalar) {
             if (live) store_direct<T, M, IO == 0>(X, base + j0, xv);
             return;
         }
@@ -118,14 +121,18 @@ static hipError_t launch_reg_io(const RegArgs<T>& g, int R, dim3 grid, dim3 bloc
 {
     const int logL = __builtin_ctz((unsigned)g.L);
     const size_t lds = (IO >= 2 && g.vec) ? (size_t)(block.x / kWave) * Stage<T, M>::kWaveBytes : 0;
-#define TRID_REG_CASE(RR)                                                                    \
-    case RR:                                                                                 \
-        if constexpr (RR <= M) {                                                             \
-            hipLaunchKernelGGL((reg_crpcr_kernel<T, M, RR, IO>), grid, block, lds, g.stream, \
-                               g.a, g.b, g.c, g.d, g.x, g.n, g.N, g.L, logL, g.rest, g.vec,    \
-                               g.bs ? g.bs : g.n);                                          \
-            return hipGetLastError();                                                        \
-        }                                                                                    \
+    const bool full = g.L == kWave;
+#define TRID_REG_LAUNCH(RR, FF)                                                                \
+    hipLaunchKernelGGL((reg_crpcr_kernel<T, M, RR, IO, FF>), grid, block, lds, g.stream, g.a, \
+                       g.b, g.c, g.d, g.x, g.n, g.N, g.L, logL, g.rest, g.vec,               \
+                       g.bs ? g.bs : g.n)
+#define TRID_REG_CASE(RR)                                                                      \
+    case RR:                                                                                   \
+        if constexpr (RR <= M) {                                                               \
+            if (RR > 1 && full) TRID_REG_LAUNCH(RR, (RR > 1));                                \
+            else TRID_REG_LAUNCH(RR, false);                                                   \
+            return hipGetLastError();                                                          \
+        }                                                                                      \
         break;
     switch (R) {
         TRID_REG_CASE(1)
@@ -136,13 +143,14 @@ static hipError_t launch_reg_io(const RegArgs<T>& g, int R, dim3 grid, dim3 bloc
         default: break;
     }
 #undef TRID_REG_CASE
+#undef TRID_REG_LAUNCH
     return hipErrorInvalidValue;
 }
 
 template <typename T, int M>
 static hipError_t launch_reg_m(const RegArgs<T>& g, int R, dim3 grid, dim3 block)
 {
-    switch (g.io) {
+    switch (g.vec ? g.io : 0) {  // element-wise batches: IO = 0 (see the kernel)
         case 0: return launch_reg_io<T, M, 0>(g, R, grid, block);
         case 1: return launch_reg_io<T, M, 1>(g, R, grid, block);
         case 2: return launch_reg_io<T, M, 2>(g, R, grid, block);

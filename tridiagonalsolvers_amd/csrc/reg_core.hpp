@@ -7,19 +7,118 @@
 //                              the right lane's "prefix" equation by bpermute)
 //   2. cross-lane CR levels   (while more than `rest` equations remain)
 //   3. PCR on the `rest` system, cross-lane, all reads before writes because
-//      every level works on registers
+//      every level works on registers (R > 1: on rows scaled to a unit
+//      diagonal, pcr_rows)
 //   4. 2x2 Cramer solves at lane distance L/2
 //   5. CR back-substitution across lanes, then inside lanes
 //
 // R: equations per lane handed to PCR (R = 1: rest <= L; R > 1: rest = R*L).
-// On return xv holds the lane's M unknowns; e holds the reduced equations.
+// FULL: the segment is the whole wave (L == 64), known at compile time.
+// On return xv holds the lane's M unknowns; e holds the CR-reduced equations
+// (for R > 1 its PCR slots keep their values from before PCR).
 #pragma once
 
 #include "trid_device.hpp"
 
 namespace trid {
 
-template <typename T, int M, int R>
+// PCR on the rest = R * L system of a segment of L lanes: lane ll holds PCR
+// equations q = ll*R + r at slots S*r + S - 1 (S = M / R), r = 0..R-1. The
+// rows are scaled to a unit diagonal first (three values per row, one
+// reciprocal per reduction). Every lane read stays inside the segment, so a
+// read that lies outside the system returns a finite row of this system,
+// which meets a coefficient that is exactly 0: no selects, and a non-finite
+// system cannot reach the other systems of its wave. FULL (L == 64, known at
+// compile time): one-lane steps are DPP wave shifts (the ends of the wave
+// shift in zeros); otherwise every read is a ds_bpermute whose lane index
+// wraps within the segment. On return the PCR slots of xv hold their
+// unknowns.
+template <typename T, int M, int R, bool FULL>
+__device__ __forceinline__ void pcr_rows(Eq<T> (&e)[M], T (&xv)[M], int lane, int ll, int L)
+{
+    constexpr int S = M / R;
+    if constexpr (FULL) {
+        L = kWave;
+        ll = lane;
+    }
+    NEq<T> q[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) q[r] = normalize_eq(e[S * r + S - 1]);
+    const int seg = lane - ll;  // first lane of the segment
+    auto src = [&](int off) { return seg + ((ll + off) & (L - 1)); };
+    const int sm1 = src(-1), sp1 = src(1);
+    auto left1 = [&](const NEq<T>& v) {
+        if constexpr (FULL) return lane_step_neq<-1>(v);
+        else return lane_read_neq(v, sm1);
+    };
+    auto right1 = [&](const NEq<T>& v) {
+        if constexpr (FULL) return lane_step_neq<1>(v);
+        else return lane_read_neq(v, sp1);
+    };
+    // ---- 3a. PCR levels with distance < R (partly inside the lane) -------
+#pragma unroll
+    for (int p = 0; (1 << p) < R; ++p) {
+        const int dl = 1 << p;
+        NEq<T> fromL[R], fromR[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            if (k < dl) {
+                fromL[k] = left1(q[R - dl + k]);
+                fromR[k] = right1(q[k]);
+            }
+        }
+        NEq<T> nw[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const NEq<T> lft = (r - dl >= 0) ? q[(r - dl >= 0) ? r - dl : 0]
+                                             : fromL[(r - dl >= 0) ? 0 : r];
+            const NEq<T> rgt = (r + dl < R) ? q[(r + dl < R) ? r + dl : 0]
+                                            : fromR[(r + dl < R) ? 0 : r + dl - R];
+            nw[r] = reduce_neq(q[r], lft, rgt);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) q[r] = nw[r];
+    }
+    // ---- 3b. PCR levels with distance >= R: lane distance 1 .. L/4 -------
+    if (1 < (L >> 1)) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) q[r] = reduce_neq(q[r], left1(q[r]), right1(q[r]));
+    }
+    // (row r of a lane meets only row r of other lanes here, so each is
+    // updated in place once its neighbours are read)
+    for (int ld = 2; ld < (L >> 1); ld <<= 1) {
+        const int sl = src(-ld), sr = src(ld);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            q[r] = reduce_neq(q[r], lane_read_neq(q[r], sl), lane_read_neq(q[r], sr));
+    }
+    // ---- 4. 2x2 solves at lane distance L/2 ------------------------------
+    // Row i of the lower half has a = 0 exactly and its partner (upper half)
+    // c = 0, and the other way round, so with k = a + c of the own row and
+    // m = a + c of the partner row both lanes solve
+    //   x = (d - k d_partner) / (1 - k m)
+    // (x1 = (d1 - c1 d2) / (1 - c1 a2), x2 = (d2 - a2 d1) / (1 - a2 c1)).
+    const int half = L >> 1;
+    const bool lower = ll < half;
+    auto solve_pairs = [&](auto partner_of) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const NEq<T> p = partner_of(q[r]);
+            const T k = q[r].a + q[r].c, m = p.a + p.c;
+            xv[S * r + S - 1] = (q[r].d - k * p.d) * tdiv(T(1), T(1) - k * m);
+        }
+    };
+    if (L == 64)
+        solve_pairs([&](const NEq<T>& v) { return lane_xor_neq<32>(v, lane); });
+    else if (L == 32)
+        solve_pairs([&](const NEq<T>& v) { return lane_xor_neq<16>(v, lane); });
+    else
+        solve_pairs([&](const NEq<T>& v) {
+            return lane_read_neq(v, lower ? lane + half : lane - half);
+        });
+}
+
+template <typename T, int M, int R, bool FULL = false>
 __device__ __forceinline__ void wave_core(Eq<T> (&e)[M], T (&xv)[M], int lane, int ll, int L,
                                           int rest)
 {
@@ -105,74 +204,7 @@ __device__ __forceinline__ void wave_core(Eq<T> (&e)[M], T (&xv)[M], int lane, i
             bs_level(1, lane_step_or_zero<-1>(xv[M - 1], ll >= 1),
                      lane_step_or_zero<1>(xv[M - 1], ll + 1 < L));
     } else {
-        // rest = R * L: lane ll holds PCR equations q = ll*R + r at slots
-        // S*r + S - 1, r = 0..R-1.
-        // ---- 3a. PCR levels with distance < R (partly inside the lane) ---
-#pragma unroll
-        for (int p = 0; (1 << p) < R; ++p) {
-            const int dl = 1 << p;
-            Eq<T> fromL[R], fromR[R];
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-                if (q < dl) {
-                    fromL[q] = lane_step_eq<-1>(e[S * (R - dl + q) + S - 1], ll >= 1);
-                    fromR[q] = lane_step_eq<1>(e[S * q + S - 1], ll + 1 < L);
-                }
-            }
-            Eq<T> nw[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const Eq<T> lft = (r - dl >= 0) ? e[S * ((r - dl >= 0) ? r - dl : 0) + S - 1]
-                                                : fromL[(r - dl >= 0) ? 0 : r];
-                const Eq<T> rgt = (r + dl < R) ? e[S * ((r + dl < R) ? r + dl : 0) + S - 1]
-                                               : fromR[(r + dl < R) ? 0 : r + dl - R];
-                nw[r] = reduce_eq(e[S * r + S - 1], lft, rgt);
-            }
-#pragma unroll
-            for (int r = 0; r < R; ++r) e[S * r + S - 1] = nw[r];
-        }
-        // ---- 3b. PCR levels with distance >= R: lane distance 1 .. L/4 ---
-        if (1 < (L >> 1)) {  // lane distance 1: DPP
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const Eq<T> lft = lane_step_eq<-1>(e[S * r + S - 1], ll >= 1);
-                const Eq<T> rgt = lane_step_eq<1>(e[S * r + S - 1], ll + 1 < L);
-                e[S * r + S - 1] = reduce_eq(e[S * r + S - 1], lft, rgt);
-            }
-        }
-        // (equation r of a lane meets only equation r of other lanes here, so
-        // each is updated in place once its neighbours are read)
-        for (int ld = 2; ld < (L >> 1); ld <<= 1) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const Eq<T> lft = lane_read_eq(e[S * r + S - 1], lane - ld, ll - ld >= 0);
-                const Eq<T> rgt = lane_read_eq(e[S * r + S - 1], lane + ld, ll + ld < L);
-                e[S * r + S - 1] = reduce_eq(e[S * r + S - 1], lft, rgt);
-            }
-        }
-        // ---- 4. 2x2 solves at lane distance L/2 --------------------------
-        const int half = L >> 1;
-        const bool lower = ll < half;
-        auto solve_pairs = [&](auto partner_of) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const Eq<T> own = e[S * r + S - 1];
-                const Eq<T> partner = partner_of(own);
-                const Eq<T> lo = lower ? own : partner;
-                const Eq<T> hi = lower ? partner : own;
-                T x1, x2;
-                solve2(lo, hi, x1, x2);
-                xv[S * r + S - 1] = lower ? x1 : x2;
-            }
-        };
-        if (L == 64)
-            solve_pairs([&](const Eq<T>& q) { return lane_xor_eq<32>(q, lane); });
-        else if (L == 32)
-            solve_pairs([&](const Eq<T>& q) { return lane_xor_eq<16>(q, lane); });
-        else
-            solve_pairs([&](const Eq<T>& q) {
-                return lane_read_eq(q, lower ? lane + half : lane - half, true);
-            });
+        pcr_rows<T, M, R, FULL>(e, xv, lane, ll, L);
     }
 
     // ---- 5b. in-lane CR back-substitution --------------------------------

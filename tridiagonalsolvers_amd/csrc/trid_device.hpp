@@ -81,6 +81,37 @@ __device__ __forceinline__ T backsub(const Eq<T>& e, T xl, T xr)
     return tdiv(e.d - e.a * xl - e.c * xr, e.b);
 }
 
+// Row scaled to a unit diagonal: x_i + a x_{i-delta} + c x_{i+delta} = d.
+// PCR on such rows moves three values per row between lanes instead of four
+// and takes one reciprocal per reduction instead of two.
+template <typename T>
+struct NEq {
+    T a, c, d;
+};
+
+template <typename T>
+__device__ __forceinline__ NEq<T> normalize_eq(const Eq<T>& e)
+{
+    const T r = tdiv(T(1), e.b);
+    return NEq<T>{e.a * r, e.c * r, e.d * r};
+}
+
+// PCR reduction of unit-diagonal rows (the step of reduce_eq, pcr_kernel.cu:39-56,
+// divided through by the new diagonal):
+//   beta = 1 - a_i c_L - c_i a_R
+//   a_i <- -a_i a_L / beta,  c_i <- -c_i c_R / beta
+//   d_i <- (d_i - a_i d_L - c_i d_R) / beta
+// A neighbour lying outside the system meets a coefficient that is exactly 0
+// (a_i = 0 while i - delta < 0, c_i = 0 while i + delta >= n, kept so by the
+// step itself), so any finite row may stand in for it.
+template <typename T>
+__device__ __forceinline__ NEq<T> reduce_neq(const NEq<T>& e, const NEq<T>& L, const NEq<T>& R)
+{
+    const T r = tdiv(T(1), T(1) - e.a * L.c - e.c * R.a);
+    const T ar = e.a * r, cr = e.c * r;
+    return NEq<T>{-ar * L.a, -cr * R.c, e.d * r - ar * L.d - cr * R.d};
+}
+
 // ---------------------------------------------------------------------------
 // cross-lane moves (ds_bpermute through the LDS crossbar; no LDS storage)
 // ---------------------------------------------------------------------------
@@ -121,8 +152,10 @@ __device__ __forceinline__ T lane_read_or_zero(T v, int src, bool valid)
 // Value of lane (lane + D), D = +1 or -1, by a DPP wave shift (wave_shl:1 /
 // wave_shr:1): a VALU move that reads the neighbouring lane's register, with
 // no trip through the LDS crossbar that ds_bpermute takes. The lane shifted in
-// at the end of the wave reads 0; callers mask the lanes whose neighbour lies
-// outside their system, as they do for ds_bpermute reads. D is a template
+// at the end of the wave reads 0 (bound_ctrl set: an out-of-range source
+// reads 0, so the move needs no prior copy of an old value into its
+// destination, one VALU instruction fewer per move); callers mask the lanes
+// whose neighbour lies outside their system, as they do for ds_bpermute reads. D is a template
 // argument so that the choice of instruction is made at compile time. Call
 // it where every lane of the wave is active (not in a branch that some lanes
 // skip): a lane whose source lane is inactive reads 0.
@@ -130,7 +163,7 @@ template <int D>
 __device__ __forceinline__ int lane_step_i32(int v)
 {
     static_assert(D == 1 || D == -1, "DPP wave shift by one lane");
-    return __builtin_amdgcn_update_dpp(0, v, D == 1 ? 0x130 : 0x138, 0xf, 0xf, false);
+    return __builtin_amdgcn_update_dpp(0, v, D == 1 ? 0x130 : 0x138, 0xf, 0xf, true);
 }
 template <int D>
 __device__ __forceinline__ float lane_step(float v)
@@ -198,6 +231,24 @@ __device__ __forceinline__ Eq<T> lane_xor_eq(const Eq<T>& e, int lane)
 {
     return Eq<T>{lane_xor<X>(e.a, lane), lane_xor<X>(e.b, lane), lane_xor<X>(e.c, lane),
                  lane_xor<X>(e.d, lane)};
+}
+
+// The same moves for unit-diagonal rows (three values each).
+template <typename T>
+__device__ __forceinline__ NEq<T> lane_read_neq(const NEq<T>& e, int src)
+{
+    src &= (kWave - 1);
+    return NEq<T>{lane_read(e.a, src), lane_read(e.c, src), lane_read(e.d, src)};
+}
+template <int D, typename T>
+__device__ __forceinline__ NEq<T> lane_step_neq(const NEq<T>& e)
+{
+    return NEq<T>{lane_step<D>(e.a), lane_step<D>(e.c), lane_step<D>(e.d)};
+}
+template <int X, typename T>
+__device__ __forceinline__ NEq<T> lane_xor_neq(const NEq<T>& e, int lane)
+{
+    return NEq<T>{lane_xor<X>(e.a, lane), lane_xor<X>(e.c, lane), lane_xor<X>(e.d, lane)};
 }
 
 // Value of `v` in lane `src` (wave-uniform index), broadcast to all lanes.
