@@ -19,15 +19,26 @@
 //   P = 16 (crNBC.cu:8-17, written for 16 banks) leaves 2-way conflicts.
 //   P = 0 (no padding) is the crNBC32 control: its map (crNBC32banks.cu:8-17)
 //   is the identity.
-// * once 64 or fewer equations remain, wave 0 takes them into registers and
-//   finishes the CR levels / the PCR on `rest` / the 2x2 solves with
-//   cross-lane moves and no barrier, then writes the solved unknowns back;
-// * a `rest` larger than 64 runs its PCR levels in place in LDS (registers
+// * all loads of a thread are issued before the first LDS write (16-byte
+//   vectors for dense, aligned batches), with at least one vector per thread
+//   and array;
+// * once 256 equations remain (64 below n = 256), wave 0 takes them into
+//   registers, 4 consecutive ones per lane, and finishes the CR levels / the
+//   PCR on `rest` / the 2x2 solves / the back-substitution of those levels
+//   with the register core (reg_core.hpp: in-lane levels, DPP and
+//   ds_bpermute moves, no barrier), then writes the solved unknowns back:
+//   against a one-equation-per-lane tail at 64, two LDS levels and their
+//   barriers fewer each way;
+// * the LDS back-substitution takes two levels per barrier (the thread that
+//   solves row i then solves rows i -+ h/2, whose neighbours it holds);
+// * a `rest` larger than 256 runs its PCR levels in place in LDS (registers
 //   hold the new values between the two barriers of a level);
 // * x overwrites d in LDS (d_i is dead once x_i is known), so four arrays of
 //   n elements fit: FP32 up to n = 8192, FP64 up to n = 4096 in 160 KiB;
 // * neighbours outside the system are identity rows, so any n is padded to a
 //   power of two exactly.
+#include "reg_core.hpp"
+#include "reg_io.hpp"
 #include "trid_device.hpp"
 #include "trid_launch.hpp"
 
@@ -56,8 +67,15 @@ struct LdsArrays {
     }
 };
 
-// KPT: most PCR equations one thread updates per level when rest > 64.
-template <typename T, int KPT>
+// Equations handed to the register tail of wave 0 (kTailM per lane).
+constexpr int kTailM = 4;
+constexpr int kTail = kWave * kTailM;
+
+// KPT: most PCR equations one thread updates per level when rest > 256.
+// V > 0: dense, 16-byte aligned batch; each thread moves V 16-byte vectors
+// per array (vector j = tid + nt*v holds elements [E*j, E*j + E)), and all of
+// its loads are issued before the first LDS write. V = 0: element loop.
+template <typename T, int KPT, int V>
 __global__ __launch_bounds__(256) void lds_crpcr_kernel(const T* __restrict__ A,
                                                         const T* __restrict__ B,
                                                         const T* __restrict__ C,
@@ -78,19 +96,46 @@ __global__ __launch_bounds__(256) void lds_crpcr_kernel(const T* __restrict__ A,
     lds_poison_wg(trid_smem, (unsigned)(4 * np * sizeof(T)));  // pad slots included
 
     // ---- load (coalesced), identity rows past n ---------------------------
-    for (int i = tid; i < n_pad; i += nt) {
-        Eq<T> e = identity_eq<T>();
-        if (i < n) e = Eq<T>{A[base + i], B[base + i], C[base + i], D[base + i]};
-        if (i == 0) e.a = T(0);          // a[first] ignored (tridiagonal_gold.h:28)
-        if (i == n - 1) e.c = T(0);      // c[last] ignored
-        s.put(i, e);
+    constexpr int E = 16 / (int)sizeof(T);
+    if constexpr (V > 0) {
+        u32x4 r[4][V];
+        const T* src[4] = {A, B, C, D};
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int arr = 0; arr < 4; ++arr)
+                r[arr][v] = gload16<true>(src[arr] + base + (int64_t)E * (tid + nt * v));
+        T* dst[4] = {s.a, s.b, s.c, s.d};
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const int i0 = E * (tid + nt * v);
+#pragma unroll
+            for (int arr = 0; arr < 4; ++arr) {
+                T t[E];
+                __builtin_memcpy(t, &r[arr][v], 16);
+                if (arr == 0 && i0 == 0) t[0] = T(0);           // a[first] ignored
+                if (arr == 2 && i0 + E == n) t[E - 1] = T(0);   // c[last] ignored
+#pragma unroll
+                for (int q = 0; q < E; ++q) dst[arr][s.at(i0 + q)] = t[q];
+            }
+        }
+    } else {
+        for (int i = tid; i < n_pad; i += nt) {
+            Eq<T> e = identity_eq<T>();
+            if (i < n) e = Eq<T>{A[base + i], B[base + i], C[base + i], D[base + i]};
+            if (i == 0) e.a = T(0);          // a[first] ignored (tridiagonal_gold.h:28)
+            if (i == n - 1) e.c = T(0);      // c[last] ignored
+            s.put(i, e);
+        }
     }
     __syncthreads();
 
-    // ---- CR forward levels in LDS while more than max(rest, 64) remain ------
+    // ---- CR forward levels in LDS while more than max(rest, 256) remain -----
     int st = 1;           // stride of the remaining equations (index st*k + st - 1)
     int count = n_pad;
-    const int lds_stop = rest > kWave ? rest : kWave;
+    // (systems shorter than kTail stop at 64 equations, one per lane, so that
+    // every form keeps CR-PCR's split at `rest`)
+    const int lds_stop = rest > kTail ? rest : (n_pad >= kTail ? kTail : kWave);
     while (count > lds_stop) {
         const int st2 = st << 1;
         for (int k = tid; k < (count >> 1); k += nt) {
@@ -104,47 +149,33 @@ __global__ __launch_bounds__(256) void lds_crpcr_kernel(const T* __restrict__ A,
         __syncthreads();
     }
 
-    if (count <= kWave) {
-        // ---- wave tail: `count` equations, one per lane of wave 0 ----------
-        if (tid < kWave) {
+    if (count <= kTail) {
+        // ---- wave tail: the last `count` equations in the registers of wave
+        // 0 (kTailM consecutive ones per lane when count = kTail, else one
+        // per lane of a segment of count lanes), solved by the register
+        // CR / CR-PCR core: in-lane levels, DPP and ds_bpermute moves, no
+        // barrier -------------------------------------------------------
+        if (tid < kWave && count == kTail) {
+            Eq<T> e[kTailM];
+#pragma unroll
+            for (int m = 0; m < kTailM; ++m) e[m] = s.get(st * (lane * kTailM + m) + st - 1);
+            T xv[kTailM];
+            if (rest <= kWave) wave_core<T, kTailM, 1, true>(e, xv, lane, lane, kWave, rest);
+            else if (rest == 2 * kWave) wave_core<T, kTailM, 2, true>(e, xv, lane, lane, kWave, rest);
+            else wave_core<T, kTailM, kTailM, true>(e, xv, lane, lane, kWave, rest);
+#pragma unroll
+            for (int m = 0; m < kTailM; ++m) s.d[s.at(st * (lane * kTailM + m) + st - 1)] = xv[m];
+        } else if (tid < kWave) {
             const int L = count;
             const int ll = lane & (L - 1);
             const int idx = st * ll + st - 1;
-            Eq<T> t = s.get(idx);
-            const int r = rest < L ? rest : L;
-            int h2 = 1;
-            for (int cnt = L; cnt > r; cnt >>= 1, h2 <<= 1) {
-                const bool active = ((ll + 1) & (2 * h2 - 1)) == 0;
-                const Eq<T> lft = lane_read_eq(t, lane - h2, ll - h2 >= 0);
-                const Eq<T> rgt = lane_read_eq(t, lane + h2, ll + h2 < L);
-                const Eq<T> rr = reduce_eq(t, lft, rgt);
-                if (active) t = rr;
-            }
-            const int LS = h2;
-            const bool pact = ((ll + 1) & (LS - 1)) == 0;
-            for (int dist = LS; dist < (L >> 1); dist <<= 1) {
-                const Eq<T> lft = lane_read_eq(t, lane - dist, ll - dist >= 0);
-                const Eq<T> rgt = lane_read_eq(t, lane + dist, ll + dist < L);
-                const Eq<T> rr = reduce_eq(t, lft, rgt);
-                if (pact) t = rr;
-            }
-            const int half = L >> 1;
-            const bool lower = ll < half;
-            const Eq<T> partner = lane_read_eq(t, lower ? lane + half : lane - half, true);
-            T x1, x2;
-            solve2(lower ? t : partner, lower ? partner : t, x1, x2);
-            T xt = lower ? x1 : x2;
-            for (int h = LS >> 1; h >= 1; h >>= 1) {
-                const T xl = lane_read_or_zero(xt, lane - h, ll - h >= 0);
-                const T xr = lane_read_or_zero(xt, lane + h, ll + h < L);
-                const bool solve = ((ll + 1) & (2 * h - 1)) == h;
-                const T v = backsub(t, xl, xr);
-                if (solve) xt = v;
-            }
-            if (lane < L) s.d[s.at(idx)] = xt;
+            Eq<T> e[1] = {s.get(idx)};
+            T xv[1];
+            wave_core<T, 1, 1>(e, xv, lane, ll, L, rest < L ? rest : L);
+            if (lane < L) s.d[s.at(idx)] = xv[0];
         }
     } else {
-        // ---- PCR on rest (> 64) equations in place in LDS ------------------
+        // ---- PCR on rest (> 256) equations in place in LDS -----------------
         for (int delta = 1; delta < (rest >> 1); delta <<= 1) {
             Eq<T> nw[KPT];
 #pragma unroll
@@ -191,20 +222,46 @@ __global__ __launch_bounds__(256) void lds_crpcr_kernel(const T* __restrict__ A,
     __syncthreads();
 
     // ---- CR back-substitution in LDS ----------------------------------------
+    // Two levels per barrier: the thread that solves row i at level h (from
+    // x at i - h and i + h) then solves its two rows i -+ h/2 of the next
+    // level, whose neighbours are x_i and those same two values.
+    auto solve_row = [&](int i, T xl, T xr) {
+        const int p = s.at(i);
+        return tdiv(s.d[p] - s.a[p] * xl - s.c[p] * xr, s.b[p]);
+    };
     while (st > 1) {
-        const int h = st >> 1;
+        const int h = st >> 1, q = st >> 2;
         for (int k = tid; k < n_pad / st; k += nt) {
             const int i = st * k + h - 1;
             const T xl = (i - h >= 0) ? s.d[s.at(i - h)] : T(0);
             const T xr = s.d[s.at(i + h)];
-            const int p = s.at(i);
-            s.d[p] = tdiv(s.d[p] - s.a[p] * xl - s.c[p] * xr, s.b[p]);
+            const T xi = solve_row(i, xl, xr);
+            if (q > 0) {
+                const T xa = solve_row(i - q, xl, xi);
+                const T xb = solve_row(i + q, xi, xr);
+                s.d[s.at(i - q)] = xa;
+                s.d[s.at(i + q)] = xb;
+            }
+            s.d[s.at(i)] = xi;
         }
-        st = h;
+        st = q > 0 ? q : h;
         __syncthreads();
     }
 
-    for (int i = tid; i < n; i += nt) X[base + i] = s.d[s.at(i)];
+    if constexpr (V > 0) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const int i0 = E * (tid + nt * v);
+            T t[E];
+#pragma unroll
+            for (int q = 0; q < E; ++q) t[q] = s.d[s.at(i0 + q)];
+            u32x4 w;
+            __builtin_memcpy(&w, t, 16);
+            gstore16<true>(X + base + i0, w);
+        }
+    } else {
+        for (int i = tid; i < n; i += nt) X[base + i] = s.d[s.at(i)];
+    }
 }
 
 template <typename T>
@@ -216,10 +273,20 @@ hipError_t launch_lds_crpcr(const LdsArgs<T>& g)
     const int np = n_pad + (n_pad >> sh);
     const size_t lds = (size_t)4 * np * sizeof(T);
     if (lds > 160 * 1024) return hipErrorInvalidConfiguration;
-    int nt = n_pad / 2;
+    // threads per system: at least one 16-byte vector per array each, so that
+    // dense batches take the vector loads (131,072 x 512 FP32 crNBC: 0.24 ms
+    // at 128 threads, 0.39 ms at 256 threads and element loads); at most 256
+    constexpr int E = 16 / (int)sizeof(T);
+    int nt = n_pad / E;
     if (nt > 256) nt = 256;
     if (nt < kWave) nt = kWave;
-    const int kpt = g.rest > kWave ? (g.rest + nt - 1) / nt : 1;
+    const int kpt = g.rest > kTail ? (g.rest + nt - 1) / nt : 1;
+    auto al16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
+    int V = 0;
+    if (g.n == n_pad && n_pad >= E * nt && al16(g.a) && al16(g.b) && al16(g.c) && al16(g.d) &&
+        al16(g.x))
+        V = n_pad / (E * nt);
+    if (V > 8) V = 0;
     auto go = [&](auto kern) -> hipError_t {
         if (lds > 64 * 1024) {
             hipError_t e = hipFuncSetAttribute((const void*)kern,
@@ -231,9 +298,19 @@ hipError_t launch_lds_crpcr(const LdsArgs<T>& g)
                            g.d, g.x, g.n, n_pad, g.rest, sh);
         return hipGetLastError();
     };
-    if (kpt <= 1) return go(lds_crpcr_kernel<T, 1>);
-    if (kpt <= 4) return go(lds_crpcr_kernel<T, 4>);
-    if (kpt <= 16) return go(lds_crpcr_kernel<T, 16>);
+    auto by_v = [&](auto kpt_tag) -> hipError_t {
+        constexpr int K = decltype(kpt_tag)::value;
+        switch (V) {
+            case 1: return go(lds_crpcr_kernel<T, K, 1>);
+            case 2: return go(lds_crpcr_kernel<T, K, 2>);
+            case 4: return go(lds_crpcr_kernel<T, K, 4>);
+            case 8: return go(lds_crpcr_kernel<T, K, 8>);
+            default: return go(lds_crpcr_kernel<T, K, 0>);
+        }
+    };
+    if (kpt <= 1) return by_v(std::integral_constant<int, 1>{});
+    if (kpt <= 4) return by_v(std::integral_constant<int, 4>{});
+    if (kpt <= 16) return by_v(std::integral_constant<int, 16>{});
     return hipErrorInvalidValue;
 }
 
