@@ -144,6 +144,11 @@ def test_plan_query_separator_family(trid):
             assert q["rest"] == (2 if m == "cr" else 64)
     assert trid.plan_query("crpcr", 16384, 3, 4)["family"] == 0
     assert trid.plan_query("crpcr", 8192, 3, 8)["family"] == 0
+    # multi-wave systems: PCR on one equation per lane of the workgroup
+    q = trid.plan_query("crpcr", 8192, 4096, 8)
+    assert q["lanes"] == 512 and q["per_lane"] == 16 and q["rest"] == 512
+    q = trid.plan_query("crpcr", 8192, 4096, 4)
+    assert q["lanes"] * q["per_lane"] == 8192 and q["rest"] == q["lanes"]
     q = trid.plan_query("crpcr", 1 << 20, 1, 4, trid.make_opts(rest=512))
     assert q["family"] == 3 and q["rest"] == 512
     for bad, status in ((trid.make_opts(rest=1024), 4), (trid.make_opts(rest=96), 1),

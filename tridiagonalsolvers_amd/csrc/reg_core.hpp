@@ -18,6 +18,8 @@
 // (for R > 1 its PCR slots keep their values from before PCR).
 #pragma once
 
+#include <type_traits>
+
 #include "trid_device.hpp"
 
 namespace trid {
@@ -222,18 +224,28 @@ __device__ __forceinline__ void wave_core(Eq<T> (&e)[M], T (&xv)[M], int lane, i
 
 }
 
+// Bytes of the reduced-system area per lane of the workgroup: W wave-top
+// equations, or two buffers of up to 64 W unit-diagonal rows for the PCR
+// levels across the workgroup.
+template <typename T>
+constexpr size_t wg_red_bytes_per_lane()
+{
+    return sizeof(Eq<T>) > 2 * sizeof(NEq<T>) ? sizeof(Eq<T>) : 2 * sizeof(NEq<T>);
+}
+
 template <typename T, int M>
 constexpr size_t wg_core_lds_bytes(int W)
 {
-    return (size_t)W * (Log2<M>::value + 6) * sizeof(Eq<T>) + (size_t)kWave * W * sizeof(Eq<T>) +
-           (size_t)kWave * W * sizeof(T);
+    return (size_t)W * (Log2<M>::value + 6) * sizeof(Eq<T>) +
+           (size_t)kWave * W * wg_red_bytes_per_lane<T>() + (size_t)kWave * W * sizeof(T);
 }
 
 // Multi-wave form: W waves of one workgroup hold one system, wave w owning
 // unknowns [w*64*M, (w+1)*64*M) (see kern_reg_mw.hip). Every thread of the
-// workgroup must call it (it contains barriers). `xchg` is LDS for
-// W*(log2(M)+6) + 64*W equations and 64*W values; the caller's staging area
-// may alias it (a barrier is taken before first use).
+// workgroup must call it (it contains barriers). `xchg` is LDS of
+// wg_core_lds_bytes(W): W*(log2(M)+6) prefix equations, the reduced-system
+// area and 64*W values; the caller's staging area may alias it (a barrier is
+// taken before first use).
 template <typename T, int M>
 __device__ __forceinline__ void wg_core(Eq<T> (&e)[M], T (&xv)[M], int w, int W, int lane,
                                         int rest, char* xchg)
@@ -241,8 +253,9 @@ __device__ __forceinline__ void wg_core(Eq<T> (&e)[M], T (&xv)[M], int w, int W,
     constexpr int LOGM = Log2<M>::value;
     constexpr int NPRE = LOGM + 6;  // published prefix equations per wave
     Eq<T>* pre = reinterpret_cast<Eq<T>*>(xchg);   // [W][NPRE]
-    Eq<T>* red = pre + W * NPRE;                     // [64*W]
-    T* xs = reinterpret_cast<T*>(red + kWave * W);   // [64*W]
+    Eq<T>* red = pre + W * NPRE;                     // [W] wave tops, or the PCR rows
+    T* xs = reinterpret_cast<T*>(reinterpret_cast<char*>(red) +
+                                 (size_t)kWave * W * wg_red_bytes_per_lane<T>());  // [64*W]
 
     const Eq<T> top0 = e[M - 1];  // level-0 top equation (lane 63's chain restarts here)
     __syncthreads();              // staging rows become the exchange area
@@ -350,32 +363,53 @@ __device__ __forceinline__ void wg_core(Eq<T> (&e)[M], T (&xv)[M], int w, int W,
         xprevw = w > 0 ? xs[w - 1] : T(0);
     } else {
         // ---- PCR on `rest` equations spread over the workgroup ------------
+        // Rows scaled to a unit diagonal (three values per row, one
+        // reciprocal per reduction); level k reads buffer k&1 and writes the
+        // other, so one barrier per level suffices. A neighbour outside the
+        // system is the zero row: it meets a coupling that is exactly 0.
         const int LS = hs;  // lanes between consecutive PCR equations
         const bool pact = ((lane + 1) & (LS - 1)) == 0;
         const int q = (w * kWave + lane + 1) / LS - 1;
-        if (pact) red[q] = e[M - 1];
+        // two buffers of `rest` rows, each as (a, c) pairs then d values
+        // (aligned 8- / 16-byte and 4- / 8-byte LDS accesses)
+        using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+        T* const rows = reinterpret_cast<T*>(red);
+        auto put = [&](int b, int i, const NEq<T>& v) {
+            T* bb = rows + (size_t)b * 3 * rest;
+            reinterpret_cast<T2*>(bb)[i] = T2{v.a, v.c};
+            bb[2 * rest + i] = v.d;
+        };
+        auto get = [&](int b, int i) {
+            const T* bb = rows + (size_t)b * 3 * rest;
+            const T2 ac = reinterpret_cast<const T2*>(bb)[i];
+            return NEq<T>{ac.x, ac.y, bb[2 * rest + i]};
+        };
+        NEq<T> me = NEq<T>{T(0), T(0), T(0)};
+        if (pact) {
+            me = normalize_eq(e[M - 1]);
+            put(0, q, me);
+        }
         __syncthreads();
+        int cb = 0;  // buffer holding the current rows
         for (int delta = 1; delta < (rest >> 1); delta <<= 1) {
-            Eq<T> nw = e[M - 1];
             if (pact) {
-                const Eq<T> lft = q - delta >= 0 ? red[q - delta] : identity_eq<T>();
-                const Eq<T> rgt = q + delta < rest ? red[q + delta] : identity_eq<T>();
-                nw = reduce_eq(e[M - 1], lft, rgt);
+                const NEq<T> zero{T(0), T(0), T(0)};
+                const NEq<T> lft = q - delta >= 0 ? get(cb, q - delta) : zero;
+                const NEq<T> rgt = q + delta < rest ? get(cb, q + delta) : zero;
+                me = reduce_neq(me, lft, rgt);
+                put(cb ^ 1, q, me);
             }
             __syncthreads();
-            if (pact) {
-                red[q] = nw;
-                e[M - 1] = nw;
-            }
-            __syncthreads();
+            cb ^= 1;
         }
         if (pact) {
+            // x_q + c x_{q+half} = d (lower half), a x_{q-half} + x_q = d (upper)
             const int half = rest >> 1;
             const bool lower = q < half;
-            const Eq<T> partner = red[lower ? q + half : q - half];
-            T x1, x2;
-            solve2(lower ? e[M - 1] : partner, lower ? partner : e[M - 1], x1, x2);
-            xv[M - 1] = lower ? x1 : x2;
+            const NEq<T> p = get(cb, lower ? q + half : q - half);
+            const T cm = lower ? me.c : me.a;  // my coupling to the partner
+            const T cp = lower ? p.a : p.c;    // the partner's coupling to me
+            xv[M - 1] = tdiv(me.d - cm * p.d, T(1) - cm * cp);
             xs[q] = xv[M - 1];
         }
         __syncthreads();
