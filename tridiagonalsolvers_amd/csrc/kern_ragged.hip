@@ -9,15 +9,19 @@
 //
 // Work queue (built on the host, trid_ragged.cpp): systems are bucketed by
 // padded size; each bucket has a configuration (unknowns per lane M, lanes
-// per system, CR->PCR switch size) chosen by the autotune table. Three item
-// classes, each run by a persistent kernel of its own (the three go into one
+// per system, CR->PCR switch size) chosen by the autotune table. Four item
+// classes, each run by a persistent kernel of its own (the four go into one
 // captured graph, so that each gets the register and LDS budget its shape
 // needs):
-//   * workgroup items, 1024..8192 unknowns (ragged_wg_kernel): 8 unknowns
-//     per lane, W = n/512 waves per system, 16/W systems per item, one per
-//     group of W waves of a 1024-thread workgroup (barriers are
-//     workgroup-wide, and every group of an item runs the same bucket's code,
-//     so they match); drained from an atomic queue counter;
+//   * workgroup items, 1024..8192 unknowns: 8 unknowns per lane, W = n/512
+//     waves per system, one system per group of W waves of a workgroup
+//     (barriers are workgroup-wide, and every group of an item runs the same
+//     bucket's code, so they match); drained from atomic queue counters.
+//     8192-unknown systems (W = 16) take a 1024-thread workgroup each
+//     (ragged_wg_kernel); the others go 8/W to an item of a 512-thread
+//     workgroup (ragged_half_kernel), two of which share a CU: the solve's
+//     reduced-system phase is a chain of LDS round trips and barriers, and
+//     the second workgroup works while the first waits;
 //   * wide items, 8193..16384 unknowns, refine mode only
 //     (ragged_wide_kernel): one system per 1024-thread workgroup at 16
 //     unknowns per lane, the coefficients read again on each pass;
@@ -45,9 +49,11 @@ namespace {
 #define TRID_RAGGED_AHEAD 2
 #endif
 
-constexpr int kWgThreads = 1024;    // workgroup-item and wide-item kernels
+constexpr int kWgThreads = 1024;    // 16-wave workgroup items and wide items
 constexpr int kWgWaves = kWgThreads / kWave;
 static_assert(kWgWaves == 16, "workgroup items assume 16 waves");
+constexpr int kHalfThreads = 512;   // workgroup items of at most 8 waves
+constexpr int kHalfWaves = kHalfThreads / kWave;
 constexpr int kWaveThreads = 256;   // wave-item kernel: 4 independent waves
 constexpr int kWaveWaves = kWaveThreads / kWave;
 
@@ -93,27 +99,38 @@ constexpr size_t ragged_wave_bytes()
 {
     return MODE == kRefine ? Home::kWaveBytes : kStageWaveBytes;
 }
-// Exchange areas of the workgroup items: 16/W groups of W waves, each with
-// the exchange area of wg_core and 2 W boundary unknowns (wg_core_lds_bytes
-// is linear in W, so the 16-wave group is the largest).
-template <int MODE>
+// Exchange areas of the workgroup items of a WAVES-wave workgroup: WAVES/W
+// groups of W waves, each with the exchange area of wg_core and 2 W boundary
+// unknowns (wg_core_lds_bytes is linear in W, so one group of all the waves
+// is the largest).
+template <int MODE, int WAVES>
 constexpr size_t ragged_xchg_bytes()
 {
-    return MODE == kDirect ? wg_core_lds_bytes<double, 8>(16) + 2 * 16 * sizeof(double)
-                           : wg_core_lds_bytes<float, 8>(16) + 2 * 16 * sizeof(double);
+    return MODE == kDirect ? wg_core_lds_bytes<double, 8>(WAVES) + 2 * WAVES * sizeof(double)
+                           : wg_core_lds_bytes<float, 8>(WAVES) + 2 * WAVES * sizeof(double);
 }
-template <int MODE>
+template <int MODE, int WAVES>
 constexpr size_t wg_lds_bytes()
 {
-    return (size_t)kWgWaves * ragged_wave_bytes<MODE>() + ragged_xchg_bytes<MODE>();
+    return (size_t)WAVES * ragged_wave_bytes<MODE>() + ragged_xchg_bytes<MODE, WAVES>();
 }
+// Other LDS of the workgroup-item kernels: the item slots and the bucket table.
+constexpr size_t kWgSmallLds = 2 * sizeof(int) + kRaggedMaxBuckets * sizeof(RaggedBucket);
 // wide items: the transposition areas (16 unknowns per lane), then the
 // exchange area and 2 W boundary unknowns
 constexpr size_t kWideStageBytes = (size_t)kWgWaves * RStage<16>::kBytes;
 constexpr size_t kWideLdsBytes =
     kWideStageBytes + wg_core_lds_bytes<float, 16>(16) + 2 * 16 * sizeof(double);
-static_assert(wg_lds_bytes<kDirect>() <= 160 * 1024 - 512, "LDS of the FP64 workgroup items");
-static_assert(wg_lds_bytes<kRefine>() <= 160 * 1024 - 512, "LDS of the refine workgroup items");
+static_assert(wg_lds_bytes<kDirect, kWgWaves>() <= 160 * 1024 - 512,
+              "LDS of the FP64 workgroup items");
+static_assert(wg_lds_bytes<kRefine, kWgWaves>() <= 160 * 1024 - 512,
+              "LDS of the refine workgroup items");
+// two 512-thread workgroups per CU (LDS is allocated in 512-byte blocks)
+constexpr size_t lds_blocks(size_t v) { return (v + 511) / 512 * 512; }
+static_assert(2 * lds_blocks(wg_lds_bytes<kDirect, kHalfWaves>() + kWgSmallLds) <= 160 * 1024,
+              "two FP64 8-wave workgroups per CU");
+static_assert(2 * lds_blocks(wg_lds_bytes<kRefine, kHalfWaves>() + kWgSmallLds) <= 160 * 1024,
+              "two refine 8-wave workgroups per CU");
 static_assert(kWideLdsBytes <= 160 * 1024 - 512, "LDS of the wide items");
 
 // Lane-blocked gathers. The system (segment of L lanes, first row of the
@@ -702,29 +719,50 @@ __device__ __forceinline__ void drain_queue(int lo, int hi, int* ctr, int* s_ite
 
 __global__ void ragged_reset_kernel(int* counters)
 {
-    if (threadIdx.x < 2) counters[threadIdx.x] = 0;
+    if (threadIdx.x < 3) counters[threadIdx.x] = 0;
 }
 
-// Workgroup items [n_wide_items, n_wg_items), largest bucket first.
-template <int MODE>
-__global__ __launch_bounds__(kWgThreads) void ragged_wg_kernel(RaggedArgs g)
+namespace {
+
+// Workgroup items [lo, hi) of a WAVES-wave workgroup, claimed from `ctr`.
+template <int MODE, int WAVES>
+__device__ __forceinline__ void wg_items(const RaggedArgs& g, int lo, int hi, int* ctr)
 {
-    __shared__ __align__(16) char smem[wg_lds_bytes<MODE>()];
+    __shared__ __align__(16) char smem[wg_lds_bytes<MODE, WAVES>()];
     __shared__ int s_item[2];
     __shared__ RaggedBucket s_bk[kRaggedMaxBuckets];
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     char* area = smem + (size_t)wave * ragged_wave_bytes<MODE>();
-    char* xchg_base = smem + (size_t)kWgWaves * ragged_wave_bytes<MODE>();
+    char* xchg_base = smem + (size_t)WAVES * ragged_wave_bytes<MODE>();
     lds_poison_wg(smem, (unsigned)sizeof(smem));
     lds_poison_wg(s_bk, (unsigned)sizeof(s_bk));
     if (threadIdx.x < g.n_buckets) s_bk[threadIdx.x] = g.buckets[threadIdx.x];
-    drain_queue(g.n_wide_items, g.n_wg_items, &g.counters[1], s_item, [&](int item) {
+    drain_queue(lo, hi, ctr, s_item, [&](int item) {
         const RaggedItem it = g.items[item];
         const RaggedBucket bk = s_bk[it.bucket];
         if constexpr (MODE == kRefine) wg_item_resident(g, it, bk, wave, lane, xchg_base, area);
         else wg_item_direct(g, it, bk, wave, lane, xchg_base, area);
     });
+}
+
+}  // namespace
+
+// 16-wave workgroup items [n_wide_items, n_full_items) (8192 unknowns).
+template <int MODE>
+__global__ __launch_bounds__(kWgThreads) void ragged_wg_kernel(RaggedArgs g)
+{
+    wg_items<MODE, kWgWaves>(g, g.n_wide_items, g.n_full_items, &g.counters[1]);
+}
+
+// Workgroup items of at most 8 waves [n_full_items, n_wg_items), largest
+// bucket first; at most 128 registers per lane, so that two workgroups share
+// a CU.
+template <int MODE>
+__global__ __launch_bounds__(kHalfThreads) __attribute__((amdgpu_waves_per_eu(4)))
+void ragged_half_kernel(RaggedArgs g)
+{
+    wg_items<MODE, kHalfWaves>(g, g.n_full_items, g.n_wg_items, &g.counters[2]);
 }
 
 // Wide items [0, n_wide_items) (refine mode).
@@ -812,8 +850,8 @@ RaggedGrids ragged_grids(int device, const RaggedArgs& g)
     // workgroup slots of each kernel on each device (the current device when
     // first asked), found once
     static std::mutex mu;
-    static int cache[64][5];
-    int slots[5];
+    static int cache[64][7];
+    int slots[7];
     {
         std::lock_guard<std::mutex> lk(mu);
         int* c = cache[device >= 0 && device < 64 ? device : 0];
@@ -822,13 +860,16 @@ RaggedGrids ragged_grids(int device, const RaggedArgs& g)
             c[2] = slots_of((const void*)ragged_wg_kernel<kDirect>, kWgThreads, device);
             c[3] = slots_of((const void*)ragged_wide_kernel, kWgThreads, device);
             c[4] = slots_of((const void*)ragged_wave_kernel<kRefine>, kWaveThreads, device);
+            c[5] = slots_of((const void*)ragged_half_kernel<kRefine>, kHalfThreads, device);
+            c[6] = slots_of((const void*)ragged_half_kernel<kDirect>, kHalfThreads, device);
             c[0] = slots_of((const void*)ragged_wave_kernel<kDirect>, kWaveThreads, device);
         }
-        for (int k = 0; k < 5; ++k) slots[k] = c[k];
+        for (int k = 0; k < 7; ++k) slots[k] = c[k];
     }
     RaggedGrids r{};
     r.wide = clamp_grid(g.n_wide_items, slots[3]);
-    r.wg = clamp_grid(g.n_wg_items - g.n_wide_items, g.direct ? slots[2] : slots[1]);
+    r.wg = clamp_grid(g.n_full_items - g.n_wide_items, g.direct ? slots[2] : slots[1]);
+    r.half = clamp_grid(g.n_wg_items - g.n_full_items, g.direct ? slots[6] : slots[5]);
     r.wave = clamp_grid((g.n_wave_items + kWaveWaves - 1) / kWaveWaves,
                         g.direct ? slots[0] : slots[4]);
     return r;
@@ -845,13 +886,22 @@ hipError_t launch_ragged(const RaggedArgs& g, const RaggedGrids& grid, hipStream
                            stream, g);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    if (g.n_wg_items > g.n_wide_items) {
+    if (g.n_full_items > g.n_wide_items) {
         if (g.direct)
             hipLaunchKernelGGL(ragged_wg_kernel<kDirect>, dim3((unsigned)grid.wg),
                                dim3(kWgThreads), 0, stream, g);
         else
             hipLaunchKernelGGL(ragged_wg_kernel<kRefine>, dim3((unsigned)grid.wg),
                                dim3(kWgThreads), 0, stream, g);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (g.n_wg_items > g.n_full_items) {
+        if (g.direct)
+            hipLaunchKernelGGL(ragged_half_kernel<kDirect>, dim3((unsigned)grid.half),
+                               dim3(kHalfThreads), 0, stream, g);
+        else
+            hipLaunchKernelGGL(ragged_half_kernel<kRefine>, dim3((unsigned)grid.half),
+                               dim3(kHalfThreads), 0, stream, g);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (g.n_wave_items > 0) {

@@ -196,21 +196,23 @@ struct RaggedArgs {
     const RaggedSys* sys;     // device, numSystems, grouped by bucket
     const RaggedItem* items;  // device: wide items, workgroup items, then wave items
     const RaggedBucket* buckets;
-    int* counters;            // device, 2 queue counters (wide items, workgroup items)
+    int* counters;            // device, 3 queue counters (wide, 16-wave, <= 8-wave items)
     int n_wide_items;         // items of 8193..16384-unknown systems (the first items)
-    int n_wg_items;           // wide and workgroup items together
+    int n_full_items;         // end of the 16-wave workgroup items (8192 unknowns)
+    int n_wg_items;           // end of the workgroup items (<= 8 waves after n_full_items)
     int n_wave_items;
     int n_buckets;            // <= kRaggedMaxBuckets
     int iters;
     int direct;               // 1: one FP64 solve per system (no FP32 stage)
 };
 
-// Grids of the three persistent kernels (wide items, workgroup items, wave
-// items) for a batch: every slot of the device, or fewer when the batch has
-// fewer items.
+// Grids of the four persistent kernels (wide items, 16-wave and <= 8-wave
+// workgroup items, wave items) for a batch: every slot of the device, or
+// fewer when the batch has fewer items.
 struct RaggedGrids {
     int wide;
     int wg;
+    int half;
     int wave;
 };
 RaggedGrids ragged_grids(int device, const RaggedArgs& g);
@@ -228,6 +230,7 @@ struct RaggedHostPlan {
     std::vector<RaggedBucket> buckets;
     std::vector<int64_t> bucket_systems;
     int n_wide_items = 0;           // of the n_wg_items, those of 16 unknowns per lane
+    int n_full_items = 0;           // end of the 16-wave workgroup items
     int n_wg_items = 0;
     int n_wave_items = 0;
 };

@@ -25,6 +25,7 @@ struct trid_ragged_plan_s {
     int device = 0;
     int grid = 0;
     int n_wide_items = 0;
+    int n_full_items = 0;
     int n_wg_items = 0;
     int n_wave_items = 0;
     int n_buckets = 0;
@@ -140,7 +141,10 @@ int ragged_build_host(const int64_t* offsets, int64_t N, const trid_bucket_cfg* 
         const int bi = (int)out->buckets.size();
         out->buckets.push_back(b);
         out->bucket_systems.push_back(end - pos);
-        const int per_item = b.n_pad <= 512 ? 64 / b.lanes : 16 / (b.lanes / 64);
+        // wave items: 64/L systems; workgroup items: one system of 16 waves,
+        // or 8/W systems of W <= 8 waves (a 512-thread workgroup)
+        const int W = b.lanes / 64;
+        const int per_item = b.n_pad <= 512 ? 64 / b.lanes : (W >= 16 ? 1 : 8 / W);
         for (int64_t f = pos; f < end; f += per_item) {
             RaggedItem it{};
             it.bucket = bi;
@@ -151,9 +155,13 @@ int ragged_build_host(const int64_t* offsets, int64_t N, const trid_bucket_cfg* 
         pos = end;
     }
     out->n_wg_items = (int)wg_items.size();
-    out->n_wide_items = 0;  // the largest buckets come first
-    for (const RaggedItem& it : wg_items)
-        if (out->buckets[it.bucket].per_lane == 16) ++out->n_wide_items;
+    out->n_wide_items = 0;  // the largest buckets come first: wide, 16-wave, <= 8-wave
+    out->n_full_items = 0;
+    for (const RaggedItem& it : wg_items) {
+        const RaggedBucket& b = out->buckets[it.bucket];
+        if (b.per_lane == 16) ++out->n_wide_items;
+        if (b.lanes >= 1024) ++out->n_full_items;
+    }
     out->n_wave_items = (int)wave_items.size();
     out->items = wg_items;
     out->items.insert(out->items.end(), wave_items.begin(), wave_items.end());
@@ -375,6 +383,7 @@ int trid_ragged_plan_create(trid_ragged_plan* out, const int64_t* offsets, int64
     p->bucket_systems = hp.bucket_systems;
     p->n_buckets = (int)hp.buckets.size();
     p->n_wide_items = hp.n_wide_items;
+    p->n_full_items = hp.n_full_items;
     p->n_wg_items = hp.n_wg_items;
     p->n_wave_items = hp.n_wave_items;
     const std::vector<trid::RaggedItem>& items = hp.items;
@@ -408,12 +417,14 @@ int trid_ragged_plan_create(trid_ragged_plan* out, const int64_t* offsets, int64
     (void)hipGetDevice(&p->device);
     trid::RaggedArgs g{};
     g.n_wide_items = p->n_wide_items;
+    g.n_full_items = p->n_full_items;
     g.n_wg_items = p->n_wg_items;
     g.n_wave_items = p->n_wave_items;
     p->grids = trid::ragged_grids(p->device, g);
     g.direct = 1;
     p->grids_direct = trid::ragged_grids(p->device, g);
-    p->grid = std::max(p->grids.wg, std::max(p->grids.wide, p->grids.wave));
+    p->grid = std::max(std::max(p->grids.wg, p->grids.half),
+                       std::max(p->grids.wide, p->grids.wave));
     *out = p;
     return TRID_OK;
 }
@@ -469,6 +480,7 @@ int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* 
     g.buckets = plan->d_buckets;
     g.counters = plan->d_counters;
     g.n_wide_items = plan->n_wide_items;
+    g.n_full_items = plan->n_full_items;
     g.n_wg_items = plan->n_wg_items;
     g.n_wave_items = plan->n_wave_items;
     g.n_buckets = plan->n_buckets;

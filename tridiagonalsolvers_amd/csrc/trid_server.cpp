@@ -448,6 +448,7 @@ int trid_server_s::launch(Slot& s, std::unique_lock<std::mutex>& lk)
         g.buckets = reinterpret_cast<const trid::RaggedBucket*>(desc + L.sys + L.itm);
         g.counters = reinterpret_cast<int*>(desc + L.sys + L.itm + L.bkt);
         g.n_wide_items = hp.n_wide_items;
+        g.n_full_items = hp.n_full_items;
         g.n_wg_items = hp.n_wg_items;
         g.n_wave_items = hp.n_wave_items;
         g.n_buckets = (int)hp.buckets.size();
