@@ -74,8 +74,9 @@ typedef struct trid_opts {
 } trid_opts;
 
 /* PCRGAUSS: use the two-pass form (global PCR pass, then strided Thomas pass
- * through a handle workspace of 3*n*N elements) even when the system fits
- * one workgroup's LDS. It is always used when it does not fit. */
+ * through a handle workspace of about 5*n*N elements: the normalized rows and
+ * the elimination's factor pairs) even when the system fits one workgroup's
+ * LDS. It is always used when it does not fit. */
 #define TRID_FLAG_TWO_PASS 1
 
 /* Human-readable text for a status code (static storage). */
@@ -185,7 +186,7 @@ int trid_solve_f64(trid_handle handle, int method,
  * the process ends; a request that needs more returns TRID_ENOMEM before
  * anything is enqueued (64 MiB holds the separator records of 64 systems of
  * 2^20 unknowns in FP32 or FP64, and the two-pass pcrGauss workspace of
- * 5.6e6 FP32 / 2.8e6 FP64 unknowns). The call allocates nothing; its first call on
+ * 3.3e6 FP32 / 1.6e6 FP64 unknowns). The call allocates nothing; its first call on
  * a device looks up that device's array address. Every call on one device
  * shares the array and finds the previous call's contents (no kernel here
  * needs it cleared): calls that overlap on one device (different streams or

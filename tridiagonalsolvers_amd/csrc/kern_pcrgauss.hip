@@ -24,6 +24,8 @@
 #include "trid_device.hpp"
 #include "trid_launch.hpp"
 
+#include <type_traits>
+
 namespace trid {
 
 // ---------------------------------------------------------------------------
@@ -138,10 +140,10 @@ __global__ __launch_bounds__(1024) void pcr_thomas_lds_kernel(const T* __restric
 // thread FP32, 10 FP64) or LDS; TRID_PCRG_THREADS threads per workgroup.
 // Measured on 64 x 2^20 FP32 (depth 2..10): 2048 / 8 G / 512 threads is
 // the best of the shapes tried (tiles of 2048..8192 equations, 256..1024
-// threads; with the normalized output at depth 8: 4096 / 16 G 1.27 ms,
-// 1024 threads with 4096 / 16 G 1.13 ms or 8192 / 32 G 1.44 ms, against
-// 1.14 ms): the levels' LDS records (16 bytes written, 32 read per equation
-// and level) bound the pass, and larger tiles leave fewer workgroups per CU.
+// threads; depth 8 with the unit-diagonal records: 2048 / 512 threads
+// 1.036 ms, 4096 / 512 1.191 ms, 4096 / 1024 1.031 ms): the levels' LDS
+// records and barriers bound the pass, and larger tiles leave fewer
+// workgroups per CU.
 #ifndef TRID_PCRG_TILE_MIN
 #define TRID_PCRG_TILE_MIN 2048
 #endif
@@ -154,59 +156,79 @@ __global__ __launch_bounds__(1024) void pcr_thomas_lds_kernel(const T* __restric
 
 // Pass 1: log2(G) PCR levels on tiles of `tile` equations with a halo of
 // G-1 on each side. Each thread keeps its equations (p = tid + j*nt of the
-// window) in registers; a level publishes them to LDS as one record per
-// equation, and reads the two neighbours back (48 bytes of LDS traffic per
-// equation and level for FP32). Output: each reduced row divided by its
-// diagonal, (a'/b', c'/b', d'/b') in three workspace arrays (A2, C2, D2): the
-// second pass treats the diagonal as 1, so b' is neither written nor read.
+// window) in registers, scaled to a unit diagonal (x_p + a x_{p-delta} +
+// c x_{p+delta} = d: three values per row, one reciprocal per reduction); a
+// level publishes them to LDS, the (a, c) pairs in one array and d in another
+// (aligned 8- / 16-byte and 4- / 8-byte accesses), and reads the two
+// neighbours back: 36 bytes of LDS traffic per equation and level for FP32
+// (48 with whole rows), 12 bytes of LDS per window equation. Output: the
+// reduced rows (a', c', d' with diagonal 1) in three workspace arrays (A2,
+// C2, D2); the second pass treats the diagonal as 1. (Two LDS buffers with
+// one barrier per level were slower: their doubled LDS leaves fewer
+// workgroups per CU.)
+template <typename T>
+__host__ __device__ constexpr size_t pcrg_lds_bytes(int len)
+{
+    return (size_t)3 * len * sizeof(T);
+}
+
 template <typename T, int KPT>
 __global__ __launch_bounds__(TRID_PCRG_THREADS) void pcr_global_levels_kernel(
     const T* __restrict__ A, const T* __restrict__ B, const T* __restrict__ C,
     const T* __restrict__ D, T* __restrict__ A2, T* __restrict__ C2, T* __restrict__ D2,
     int64_t n, int G, int tile, int64_t tiles)
 {
+    using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
     extern __shared__ __align__(16) char trid_smem[];
     const int H = G - 1;
     const int len = tile + 2 * H;
-    Eq<T>* se = reinterpret_cast<Eq<T>*>(trid_smem);
+    T2* const sac = reinterpret_cast<T2*>(trid_smem);
+    T* const sd = reinterpret_cast<T*>(trid_smem) + 2 * (size_t)len;
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
-    // one-dimensional grid of tiles x systems (no 65,535 cap on the systems)
     const int64_t sys = (int64_t)blockIdx.x / tiles;
     const int64_t base = sys * n;
     const int64_t t0 = ((int64_t)blockIdx.x - sys * tiles) * tile;
-    lds_poison_wg(trid_smem, (unsigned)(len * sizeof(Eq<T>)));
-
-    Eq<T> own[KPT];
+    lds_poison_wg(trid_smem, (unsigned)pcrg_lds_bytes<T>(len));
+    const NEq<T> zero{T(0), T(0), T(0)};
+    NEq<T> own[KPT];
 #pragma unroll
     for (int j = 0; j < KPT; ++j) {
         const int p = tid + j * nt;
         const int64_t g = t0 - H + p;
-        own[j] = identity_eq<T>();
+        own[j] = zero;
         if (p < len && g >= 0 && g < n) {
-            own[j] = Eq<T>{A[base + g], B[base + g], C[base + g], D[base + g]};
-            if (g == 0) own[j].a = T(0);
-            if (g == n - 1) own[j].c = T(0);
+            Eq<T> e{A[base + g], B[base + g], C[base + g], D[base + g]};
+            if (g == 0) e.a = T(0);
+            if (g == n - 1) e.c = T(0);
+            own[j] = normalize_eq(e);
         }
     }
     for (int delta = 1; delta < G; delta <<= 1) {
 #pragma unroll
         for (int j = 0; j < KPT; ++j) {
             const int p = tid + j * nt;
-            if (p < len) se[p] = own[j];
+            if (p < len) {
+                sac[p] = T2{own[j].a, own[j].c};
+                sd[p] = own[j].d;
+            }
         }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < KPT; ++j) {
             const int p = tid + j * nt;
             const int64_t g = t0 - H + p;
-            // rows outside the system stay identity rows; outside the loaded
-            // window the value is never used by the tile (the window shrinks
-            // by delta per level, G-1 in total)
             if (p < len && g >= 0 && g < n) {
-                const Eq<T> lft = p - delta >= 0 ? se[p - delta] : identity_eq<T>();
-                const Eq<T> rgt = p + delta < len ? se[p + delta] : identity_eq<T>();
-                own[j] = reduce_eq(own[j], lft, rgt);
+                NEq<T> lft = zero, rgt = zero;
+                if (p - delta >= 0) {
+                    const T2 ac = sac[p - delta];
+                    lft = NEq<T>{ac.x, ac.y, sd[p - delta]};
+                }
+                if (p + delta < len) {
+                    const T2 ac = sac[p + delta];
+                    rgt = NEq<T>{ac.x, ac.y, sd[p + delta]};
+                }
+                own[j] = reduce_neq(own[j], lft, rgt);
             }
         }
         __syncthreads();
@@ -216,10 +238,9 @@ __global__ __launch_bounds__(TRID_PCRG_THREADS) void pcr_global_levels_kernel(
         const int p = tid + j * nt;
         const int64_t g = t0 - H + p;
         if (p >= H && p < H + tile && g < n) {
-            const T ib = tdiv(T(1), own[j].b);
-            A2[base + g] = own[j].a * ib;
-            C2[base + g] = own[j].c * ib;
-            D2[base + g] = own[j].d * ib;
+            A2[base + g] = own[j].a;
+            C2[base + g] = own[j].c;
+            D2[base + g] = own[j].d;
         }
     }
 }
@@ -257,12 +278,18 @@ __global__ __launch_bounds__(TRID_PCRG_THREADS) void pcr_global_levels_kernel(
 // NORM: the rows are normalized (diagonal 1, B not read; the pcrGauss
 // second pass): den = 1 - p q'_{i-1}. The backward sweep streams UB steps
 // of factors ahead (2 loads and 1 store per step share the same counter).
-template <typename T, int U, int UB, bool NORM>
+// PACK: the two factors of a row go to F2 as one pair (one 8-byte FP32 /
+// 16-byte FP64 store per step, one load per step back), so that more steps
+// fit the 63 outstanding operations: 3 loads + 1 store forward, 1 load +
+// 1 store backward.
+template <typename T, int U, int UB, bool NORM, bool PACK = false>
 __global__ __launch_bounds__(64) void thomas_strided_kernel(const T* A, const T* B, const T* C,
                                                             const T* D, T* FC, T* FD, T* X,
                                                             int64_t block_len, int64_t blocks,
-                                                            int64_t chains, int64_t stride)
+                                                            int64_t chains, int64_t stride,
+                                                            T* F2 = nullptr)
 {
+    using T2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
     const int lane = threadIdx.x;
     const bool top = lane < kWave / 2;
     const int64_t chain = (int64_t)blockIdx.x * (kWave / 2) + (lane & (kWave / 2 - 1));
@@ -282,6 +309,7 @@ __global__ __launch_bounds__(64) void thomas_strided_kernel(const T* A, const T*
     const T* pd = D + o0;
     T* wq = FC + o0;
     T* wd = FD + o0;
+    T2* wf = PACK ? reinterpret_cast<T2*>(F2) + o0 : nullptr;
     T* px = X + o0;
 
     T qv = T(0), dv = T(0);
@@ -323,8 +351,12 @@ __global__ __launch_bounds__(64) void thomas_strided_kernel(const T* A, const T*
                     qv = tdiv(tq[u], den);
                     dv = tdiv(td[u] - dv * pk, den);
                 }
-                wq[i * step] = qv;
-                wd[i * step] = dv;
+                if constexpr (PACK) {
+                    wf[i * step] = T2{qv, dv};
+                } else {
+                    wq[i * step] = qv;
+                    wd[i * step] = dv;
+                }
             }
         }
     }
@@ -348,8 +380,14 @@ __global__ __launch_bounds__(64) void thomas_strided_kernel(const T* A, const T*
         for (int u = 0; u < UB; ++u) {
             const int64_t i = i0 - u;
             if (i >= 0) {
-                bq[u] = wq[i * step];
-                bd[u] = wd[i * step];
+                if constexpr (PACK) {
+                    const T2 f = wf[i * step];
+                    bq[u] = f.x;
+                    bd[u] = f.y;
+                } else {
+                    bq[u] = wq[i * step];
+                    bd[u] = wd[i * step];
+                }
             }
         }
     };
@@ -612,7 +650,8 @@ static int device_cus()
 // within the 63 memory operations a wave can have outstanding. Measured on
 // 64 x 2^20 FP32 at depth 8 (normalized rows): forward / backward 8 / 8:
 // 1.142 ms, 10 / 12: 1.094 ms, 12 / 16: 1.076 ms (4 rows per step as before
-// the normalization: 1.208 ms).
+// the normalization: 1.208 ms). With the factors packed in pairs (the
+// pcrGauss second pass): 3 loads + 1 store forward, 1 + 1 backward.
 #ifndef TRID_STRIDED_U
 #define TRID_STRIDED_U 8
 #endif
@@ -622,11 +661,19 @@ static int device_cus()
 #ifndef TRID_STRIDED_UB
 #define TRID_STRIDED_UB 16
 #endif
+#ifndef TRID_STRIDED_UP
+#define TRID_STRIDED_UP 15
+#endif
+#ifndef TRID_STRIDED_UBP
+#define TRID_STRIDED_UBP 31
+#endif
 
+// F2: workspace for packed factor pairs (one pair per row of the blocks,
+// pair-aligned) for the direct kernel of normalized rows, or null.
 template <typename T, bool NORM>
 static hipError_t launch_strided(const T* A, const T* B, const T* C, const T* D, T* FC, T* FD,
                                  T* X, int64_t block_len, int64_t blocks, int64_t chains,
-                                 int64_t stride, hipStream_t stream)
+                                 int64_t stride, hipStream_t stream, T* F2 = nullptr)
 {
     const int64_t total = blocks * chains;  // two lanes each
     if (total <= 0) return hipSuccess;
@@ -647,6 +694,20 @@ static hipError_t launch_strided(const T* A, const T* B, const T* C, const T* D,
     constexpr int U = NORM ? TRID_STRIDED_UN : TRID_STRIDED_U;
     static_assert((NORM ? 3 : 4) * U + 2 * U <= 63 && 3 * TRID_STRIDED_UB <= 63,
                   "outstanding memory operations of a wave");
+    // packed factors: 256 registers, one wave per SIMD, so only while the
+    // grid has at most one wave per SIMD (64 x 2^20 FP32, depth 2 / 4 / 8:
+    // 24.1 -> 20.5 ms, 5.67 -> 5.41 ms, 1.033 -> 1.008 ms)
+    if constexpr (NORM) {
+        if (F2 && grid <= 4 * (int64_t)device_cus()) {
+            static_assert(4 * TRID_STRIDED_UP <= 63 && 2 * TRID_STRIDED_UBP <= 63,
+                          "outstanding memory operations of a wave");
+            hipLaunchKernelGGL(
+                (thomas_strided_kernel<T, TRID_STRIDED_UP, TRID_STRIDED_UBP, true, true>),
+                dim3((unsigned)grid), dim3(kWave), 0, stream, A, B, C, D, FC, FD, X, block_len,
+                blocks, chains, stride, F2);
+            return hipGetLastError();
+        }
+    }
     hipLaunchKernelGGL((thomas_strided_kernel<T, U, TRID_STRIDED_UB, NORM>), dim3((unsigned)grid),
                        dim3(kWave), 0, stream, A, B, C, D, FC, FD, X, block_len, blocks, chains,
                        stride);
@@ -698,22 +759,23 @@ hipError_t launch_pcr_thomas(const PcrThomasArgs<T>& g)
         return hipErrorInvalidValue;
     }
     // two passes through the workspace (3 arrays of n*N elements: the
-    // normalized rows)
+    // normalized rows; then n*N factor pairs, pair-aligned)
     const int64_t total = g.n * g.N;
     T* A2 = g.ws;
     T* C2 = A2 + total;
     T* D2 = C2 + total;
+    T* F2 = g.ws + pcrg_pairs_offset(total);
     const int G = g.G;
     const int nt = TRID_PCRG_THREADS;
     const int max_kpt = sizeof(T) == 4 ? 12 : 10;
     int tile = TRID_PCRG_TILE_MIN;
     if (tile < TRID_PCRG_TILE_PER_G * G) tile = TRID_PCRG_TILE_PER_G * G;
     while (tile > G && (tile + 2 * (G - 1) > max_kpt * nt ||
-                        (size_t)(tile + 2 * (G - 1)) * sizeof(Eq<T>) > 160 * 1024))
+                        pcrg_lds_bytes<T>(tile + 2 * (G - 1)) > 160 * 1024))
         tile /= 2;
     const int len = tile + 2 * (G - 1);
     const int kpt = (len + nt - 1) / nt;
-    const size_t lds = (size_t)len * sizeof(Eq<T>);
+    const size_t lds = pcrg_lds_bytes<T>(len);
     if (lds > 160 * 1024 || kpt > max_kpt) return hipErrorInvalidConfiguration;
     const int64_t tiles = (g.n + tile - 1) / tile;
     if (tiles > 0x7fffffffLL / g.N) return hipErrorInvalidConfiguration;
@@ -738,7 +800,8 @@ hipError_t launch_pcr_thomas(const PcrThomasArgs<T>& g)
         else return hipErrorInvalidValue;
         if (e != hipSuccess) return e;
     }
-    return launch_strided<T, true>(A2, nullptr, C2, D2, C2, D2, g.x, g.n, g.N, G, G, g.stream);
+    return launch_strided<T, true>(A2, nullptr, C2, D2, C2, D2, g.x, g.n, g.N, G, G, g.stream,
+                                   F2);
 }
 
 template hipError_t launch_pcr_thomas<float>(const PcrThomasArgs<float>&);

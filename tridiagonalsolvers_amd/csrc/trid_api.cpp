@@ -93,7 +93,7 @@ template <typename T>
 size_t plan_ws_bytes(const trid::Plan& p, int method, int64_t n, int64_t N)
 {
     if (p.family == trid::Family::PcrThomas && p.two_pass)
-        return (size_t)3 * (size_t)n * (size_t)N * sizeof(T);
+        return (size_t)trid::pcrg_ws_elems(n * N) * sizeof(T);
     if (p.family == trid::Family::Part)
         return trid::part_ws_bytes<T>(method == TRID_CR ? TRID_CR : TRID_CRPCR, n, N);
     return 0;

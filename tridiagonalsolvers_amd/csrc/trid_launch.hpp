@@ -71,9 +71,13 @@ struct PcrThomasArgs {
     int G;          // subsystems; PCR depth log2(G)
     int pad;        // in-LDS form: pad period in elements, 0 = none
     int two_pass;   // 1: global PCR pass + strided Thomas pass through `ws`
-    T* ws;          // two-pass workspace: 3 * n * N elements
+    T* ws;          // two-pass workspace: pcrg_ws_elems(n * N) elements
     hipStream_t stream;
 };
+// Two-pass pcrGauss workspace: the normalized rows (3 arrays of n*N
+// elements), then n*N factor pairs starting at an even element offset.
+inline int64_t pcrg_pairs_offset(int64_t total) { return (3 * total + 1) & ~int64_t(1); }
+inline int64_t pcrg_ws_elems(int64_t total) { return pcrg_pairs_offset(total) + 2 * total; }
 
 template <typename T>
 hipError_t launch_pcr_thomas(const PcrThomasArgs<T>& g);
