@@ -731,7 +731,9 @@ __device__ __forceinline__ void wg_items(const RaggedArgs& g, int lo, int hi, in
     __shared__ __align__(16) char smem[wg_lds_bytes<MODE, WAVES>()];
     __shared__ int s_item[2];
     __shared__ RaggedBucket s_bk[kRaggedMaxBuckets];
-    const int wave = threadIdx.x / kWave;
+    // the wave index is uniform in the wave: held in a scalar register, so
+    // that the group, the system's base and size derived from it are too
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
     const int lane = threadIdx.x & (kWave - 1);
     char* area = smem + (size_t)wave * ragged_wave_bytes<MODE>();
     char* xchg_base = smem + (size_t)WAVES * ragged_wave_bytes<MODE>();
@@ -771,7 +773,7 @@ __global__ __launch_bounds__(kWgThreads) void ragged_wide_kernel(RaggedArgs g)
     __shared__ __align__(16) char smem[kWideLdsBytes];
     __shared__ int s_item[2];
     __shared__ RaggedBucket s_bk[kRaggedMaxBuckets];
-    const int wave = threadIdx.x / kWave;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));  // uniform
     const int lane = threadIdx.x & (kWave - 1);
     lds_poison_wg(smem, (unsigned)sizeof(smem));
     lds_poison_wg(s_bk, (unsigned)sizeof(s_bk));

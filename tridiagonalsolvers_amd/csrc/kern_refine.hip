@@ -132,7 +132,7 @@ __global__ __launch_bounds__(1024) void refine_wg_kernel(const double* __restric
 {
     extern __shared__ __align__(16) char trid_smem[];
     const int W = blockDim.x / kWave;
-    const int w = threadIdx.x / kWave;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));  // uniform
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t base = (int64_t)blockIdx.x * n;
     const int64_t j0 = (int64_t)(w * kWave + lane) * M;
