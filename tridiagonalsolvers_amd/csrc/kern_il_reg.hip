@@ -40,6 +40,10 @@ namespace {
 #ifndef TRID_IL_SYSTEMS_F32
 #define TRID_IL_SYSTEMS_F32 16
 #endif
+// Contiguous bytes of every row that the workgroups on one XCD take together
+// (xcd_span_group).
+constexpr int kIlSpanBytes = 1024;
+
 template <typename T, int MI>
 struct IlGeom {
     static constexpr int S = sizeof(T) == 4 ? TRID_IL_SYSTEMS_F32 : 8;
@@ -52,21 +56,21 @@ struct IlGeom {
     static constexpr int SP = LC * GP + EV;                   // system pitch
     static constexpr int AP = S * SP;                         // array pitch
     static constexpr int P = 128 / (S * (int)sizeof(T));      // workgroups per 128-byte line
+    static constexpr int Q = xcd_span(kIlSpanBytes, S * (int)sizeof(T));  // workgroups per span
     static_assert(LC * NC == kWave && P * S * (int)sizeof(T) == 128, "chunk geometry");
 };
 
 // Workgroup -> group of S systems. A row of the group is 1/P of a 128-byte
 // line; the other parts belong to the neighbouring groups. Workgroups are
-// dealt to the 8 XCDs round-robin, so b, b + 8, .., b + 8 (P-1) run at about
-// the same time on one XCD: they take the P parts of the same lines, and
-// every read of a line after the first hits that XCD's L2. (The grid is a
-// multiple of 8 P so the map is one-to-one.)
-template <int P>
-__device__ __forceinline__ int64_t il_group(int64_t b)
-{
-    const int64_t x = b % 8, k = b / 8;
-    return P * ((k / P) * 8 + x) + (k % P);
-}
+// dealt to the 8 XCDs round-robin, so b, b + 8, .., b + 8 (Q-1) run at about
+// the same time on one XCD: they take Q consecutive groups, i.e. the same
+// Q * S * sizeof(T) contiguous bytes of every row. Every read of a line after
+// the first hits that XCD's L2, and HBM sees whole 1 KiB pieces of each row
+// (rows of an interleaved batch lie stride * sizeof(T) apart, a DRAM page
+// or more). Measured on 65,536 x 1,024 (FP32 / FP64): Q = P (one 128-byte
+// line): 449 / 937 us; 256 B: 437 / 892 us; 512 B: 403 / 796 us; 1 KiB:
+// 381 / 770 us; 2 KiB: 395 / 798 us. (The grid is a multiple of 8 Q so the
+// map is one-to-one: xcd_span_group.)
 
 template <typename T, int MI>
 __device__ __forceinline__ int il_addr(int sys, int row_in_chunk)
@@ -114,7 +118,7 @@ __global__ __launch_bounds__((IlGeom<T, MI>::S * kWave)) void il_reg_kernel(
     const int t = threadIdx.x;
     const int w = t / kWave;
     const int lane = t & (kWave - 1);
-    const int64_t s0 = il_group<G::P>(blockIdx.x) * G::S;
+    const int64_t s0 = xcd_span_group<G::Q>(blockIdx.x) * G::S;
     if (s0 >= N) return;  // padding workgroup: the whole block leaves
     const T* P[4] = {A, B, C, D};
 
@@ -211,7 +215,7 @@ hipError_t launch_il_m(const InterleavedArgs<T>& g)
 {
     using G = IlGeom<T, MI>;
     const int64_t groups = (g.N + G::S - 1) / G::S;
-    constexpr int64_t kMul = 8 * G::P;  // il_group is one-to-one on multiples of this
+    constexpr int64_t kMul = 8 * G::Q;  // xcd_span_group is one-to-one on multiples of this
     const int64_t grid = (groups + kMul - 1) / kMul * kMul;
     if (grid > 0x7fffffffLL) return hipErrorInvalidConfiguration;
     const int vec = (g.stride % G::EV == 0 && il_aligned(g.a) && il_aligned(g.b) &&

@@ -309,6 +309,25 @@ __device__ __forceinline__ void lds_poison_wave(void* p, unsigned bytes, int lan
     }
 }
 
+// Workgroup b of a grid padded to a multiple of 8 Q -> the column group it
+// takes, for kernels whose consecutive column groups are contiguous bytes of
+// every row (interleaved batches). The 8 XCDs take the workgroups in turn,
+// so b, b + 8, .., b + 8 (Q-1) run at about the same time on one XCD: they
+// take Q consecutive groups, and HBM sees their rows as whole spans of Q
+// groups instead of pieces of one. (Measured neutral for the lane-per-system
+// pivoting and elimination kernels, which keep the plain order.)
+template <int Q>
+__device__ __forceinline__ int64_t xcd_span_group(int64_t b)
+{
+    const int64_t x = b % 8, k = b / 8;
+    return Q * ((k / Q) * 8 + x) + (k % Q);
+}
+// Groups of `group_bytes` bytes per row that make a span of `span_bytes`.
+constexpr int xcd_span(int span_bytes, int group_bytes)
+{
+    return span_bytes / group_bytes > 1 ? span_bytes / group_bytes : 1;
+}
+
 template <int V>
 struct Log2 {
     static constexpr int value = 1 + Log2<V / 2>::value;
