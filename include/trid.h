@@ -324,7 +324,7 @@ int trid_generate_f64(int kind, uint64_t seed, double* a, double* b, double* c, 
  */
 typedef struct trid_bucket_cfg {
     int32_t size;      /* padded system size (power of two, 2..16384)           */
-    int32_t per_lane;  /* unknowns per lane: <= 8 (16 for size 16384)          */
+    int32_t per_lane;  /* unknowns per lane: <= 8 (16 for sizes 8192, 16384)   */
     int32_t lanes;     /* lanes per system: size/per_lane (<= 64 below 1024)    */
     int32_t rest;      /* CR -> PCR switch size, power of two <= lanes          */
 } trid_bucket_cfg;

@@ -189,6 +189,15 @@ def test_ragged_plan_rejects_bad_sizes_without_a_device(trid):
     bad = [dict(size=64, per_lane=8, lanes=8, rest=16)]                    # rest > lanes
     with pytest.raises(trid.TridError):
         R.RaggedPlan(np.array([0, 64], dtype=np.int64), table=bad)
+    # 8192-unknown systems run at 16 unknowns per lane (8 waves of 64 lanes);
+    # below, 16 per lane needs at least two waves
+    for bad in ([dict(size=8192, per_lane=8, lanes=1024, rest=512)],
+                [dict(size=8192, per_lane=16, lanes=256, rest=256)],
+                [dict(size=1024, per_lane=16, lanes=64, rest=64)]):
+        n = bad[0]["size"]
+        with pytest.raises(trid.TridError) as e:
+            R.RaggedPlan(np.array([0, n], dtype=np.int64), table=bad)
+        assert e.value.status == 1
 
 
 def test_autotune_candidates_are_valid_configs():
@@ -199,10 +208,13 @@ def test_autotune_candidates_are_valid_configs():
         assert cs, size
         for c in cs:
             assert c["per_lane"] * c["lanes"] == size
-            assert c["per_lane"] <= (16 if size > 8192 else 8)
+            assert c["per_lane"] <= (16 if size >= 2048 else 8)
             assert 2 <= c["rest"] <= c["lanes"] <= 1024
             if size <= 512:
                 assert c["lanes"] <= 64
+            else:
+                assert c["lanes"] >= 128
+                assert c["per_lane"] == 16 if size >= 8192 else c["per_lane"] in (8, 16)
 
 
 def test_log_uniform_sizes_are_seeded_and_in_range():

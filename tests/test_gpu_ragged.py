@@ -163,12 +163,12 @@ def test_ragged_fp64_mode_limits(oracle, trid, rg):
 def test_every_tuner_candidate_is_accurate(oracle, trid, rg, mode):
     import torch
     from tridiagonalsolvers_amd.autotune import candidates
-    for size in (4, 64, 512, 1024, 8192, 16384):
+    for size in (4, 64, 512, 1024, 2048, 4096, 8192, 16384):
         if mode == "fp64" and size > 8192:
             continue
         n = size - (size // 5)   # not a power of two: padded
         n = max(n, 2)
-        sizes = [n] * 12
+        sizes = [n] * 13         # the last item of a bucket partly filled
         offs, host, dev = _batch(oracle, rg, sizes, seed=size)
         for cand in candidates(size):
             x = torch.full_like(dev[0], float("nan"))

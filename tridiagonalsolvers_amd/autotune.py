@@ -31,12 +31,16 @@ def candidates(size: int):
                     r *= 2
             m *= 2
     else:
-        m = 16 if size > 8192 else 8
-        lanes = size // m
-        w = lanes // 64
-        for r in sorted({2, max(2, w // 2), w, 2 * w, 4 * w, 8 * w, 16 * w, 32 * w, 64 * w}):
-            if 2 <= r <= lanes:
-                out.append(dict(size=size, per_lane=m, lanes=lanes, rest=r))
+        # workgroup items of at least 2 waves: 16 unknowns per lane from 8192
+        # on, 8 or 16 below (the 16-per-lane form from 2048 on)
+        for m in ((16,) if size >= 8192 else (8, 16)):
+            lanes = size // m
+            if lanes < 128:
+                continue
+            w = lanes // 64
+            for r in sorted({2, max(2, w // 2), w, 2 * w, 4 * w, 8 * w, 16 * w, 32 * w, 64 * w}):
+                if 2 <= r <= lanes:
+                    out.append(dict(size=size, per_lane=m, lanes=lanes, rest=r))
     return out
 
 

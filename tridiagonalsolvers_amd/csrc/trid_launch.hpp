@@ -202,7 +202,7 @@ struct RaggedArgs {
     const RaggedBucket* buckets;
     int* counters;            // device, 3 queue counters (wide, 16-wave, <= 8-wave items)
     int n_wide_items;         // items of 8193..16384-unknown systems (the first items)
-    int n_full_items;         // end of the 16-wave workgroup items (8192 unknowns)
+    int n_full_items;         // end of the items at 16 per lane in 8-wave workgroups
     int n_wg_items;           // end of the workgroup items (<= 8 waves after n_full_items)
     int n_wave_items;
     int n_buckets;            // <= kRaggedMaxBuckets
@@ -233,8 +233,8 @@ struct RaggedHostPlan {
     std::vector<RaggedItem> items;  // wide items, workgroup items, then wave items
     std::vector<RaggedBucket> buckets;
     std::vector<int64_t> bucket_systems;
-    int n_wide_items = 0;           // of the n_wg_items, those of 16 unknowns per lane
-    int n_full_items = 0;           // end of the 16-wave workgroup items
+    int n_wide_items = 0;           // of the n_wg_items, those above 8192 unknowns
+    int n_full_items = 0;           // end of the 16-per-lane 8-wave items
     int n_wg_items = 0;
     int n_wave_items = 0;
 };

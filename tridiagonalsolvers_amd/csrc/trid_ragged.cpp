@@ -69,6 +69,10 @@ RaggedBucket ragged_default_bucket(int n_pad)
         b.per_lane = 16;
         b.lanes = n_pad / 16;
         b.rest = b.lanes / 64;
+    } else if (n_pad == 8192) {
+        b.per_lane = 16;
+        b.lanes = 512;
+        b.rest = 512;  // one PCR equation per lane of the workgroup
     } else if (n_pad <= 512) {
         b.per_lane = n_pad / 2 < 8 ? n_pad / 2 : 8;
         b.lanes = n_pad / b.per_lane;
@@ -94,11 +98,11 @@ int ragged_check_bucket(const RaggedBucket& b)
         if (b.lanes < 2 || b.lanes > 64 || b.per_lane > 8 || b.rest > b.lanes)
             return TRID_EINVAL;
     } else {
-        // workgroup items: 8 unknowns per lane (16 above 8192), W = lanes/64
-        // waves, W <= 16
-        if (b.per_lane != (b.n_pad > 8192 ? 16 : 8) || b.lanes < 128 || b.lanes > 1024 ||
-            b.rest > b.lanes)
-            return TRID_EINVAL;
+        // workgroup items of W = lanes/64 >= 2 waves: 16 unknowns per lane
+        // from 8192 on (8 waves; 16 waves above 8192); 8 or 16 below (W <= 8)
+        const bool m_ok = b.n_pad >= 8192 ? b.per_lane == 16
+                                          : (b.per_lane == 8 || b.per_lane == 16);
+        if (!m_ok || b.lanes < 128 || b.lanes > 1024 || b.rest > b.lanes) return TRID_EINVAL;
     }
     return TRID_OK;
 }
@@ -129,7 +133,10 @@ int ragged_build_host(const int64_t* offsets, int64_t N, const trid_bucket_cfg* 
     for (int64_t s = 0; s < N; ++s) out->order[s] = (int32_t)s;
     std::stable_sort(out->order.begin(), out->order.end(),
                      [&](int32_t x, int32_t y) { return bucket_of[x] > bucket_of[y]; });
-    std::vector<RaggedItem> wg_items, wave_items;
+    // workgroup items by kernel: wide (above 8192 unknowns), 16 unknowns per
+    // lane (8-wave workgroups, one per CU), 8 per lane (8-wave workgroups, two
+    // per CU); each list largest bucket first
+    std::vector<RaggedItem> wide_items, full_items, half_items, wave_items;
     out->buckets.clear();
     out->bucket_systems.clear();
     int64_t pos = 0;
@@ -141,29 +148,30 @@ int ragged_build_host(const int64_t* offsets, int64_t N, const trid_bucket_cfg* 
         const int bi = (int)out->buckets.size();
         out->buckets.push_back(b);
         out->bucket_systems.push_back(end - pos);
-        // wave items: 64/L systems; workgroup items: one system of 16 waves,
-        // or 8/W systems of W <= 8 waves (a 512-thread workgroup)
+        // wave items: 64/L systems; workgroup items: one wide system, or
+        // 8/W systems of W <= 8 waves (a 512-thread workgroup)
         const int W = b.lanes / 64;
-        const int per_item = b.n_pad <= 512 ? 64 / b.lanes : (W >= 16 ? 1 : 8 / W);
+        const int per_item = b.n_pad <= 512 ? 64 / b.lanes : (b.n_pad > 8192 ? 1 : 8 / W);
+        std::vector<RaggedItem>& dst = b.n_pad <= 512  ? wave_items
+                                       : b.n_pad > 8192 ? wide_items
+                                       : b.per_lane == 16 ? full_items
+                                                          : half_items;
         for (int64_t f = pos; f < end; f += per_item) {
             RaggedItem it{};
             it.bucket = bi;
             it.first = (int32_t)f;
             it.count = (int32_t)std::min<int64_t>(per_item, end - f);
-            (b.n_pad <= 512 ? wave_items : wg_items).push_back(it);
+            dst.push_back(it);
         }
         pos = end;
     }
-    out->n_wg_items = (int)wg_items.size();
-    out->n_wide_items = 0;  // the largest buckets come first: wide, 16-wave, <= 8-wave
-    out->n_full_items = 0;
-    for (const RaggedItem& it : wg_items) {
-        const RaggedBucket& b = out->buckets[it.bucket];
-        if (b.per_lane == 16) ++out->n_wide_items;
-        if (b.lanes >= 1024) ++out->n_full_items;
-    }
+    out->n_wide_items = (int)wide_items.size();
+    out->n_full_items = out->n_wide_items + (int)full_items.size();
+    out->n_wg_items = out->n_full_items + (int)half_items.size();
     out->n_wave_items = (int)wave_items.size();
-    out->items = wg_items;
+    out->items = wide_items;
+    out->items.insert(out->items.end(), full_items.begin(), full_items.end());
+    out->items.insert(out->items.end(), half_items.begin(), half_items.end());
     out->items.insert(out->items.end(), wave_items.begin(), wave_items.end());
     if (out->buckets.size() > (size_t)kRaggedMaxBuckets) return TRID_EINVAL;
     out->sys.resize(N);
