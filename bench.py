@@ -45,10 +45,11 @@ def parse():
                    help="uniform workload: requests of the served leg added to the line "
                         "(the request server with Poisson arrivals and resident ADI "
                         "grids); 0 = no served leg")
-    p.add_argument("--ragged-mode", default="refine", choices=["refine", "fp64"],
-                   help="ragged: FP32 solve + 2 FP64 refinements (the mixed-precision "
-                        "workload), or one FP64 solve per system (both are timed; this "
-                        "picks the reported value)")
+    p.add_argument("--ragged-mode", default="tol", choices=["tol", "refine", "fp64"],
+                   help="ragged: FP32 solve + FP64 refinements until every system's "
+                        "relative residual is <= 1e-12 (at most 4; the mixed-precision "
+                        "workload), FP32 solve + exactly 2 FP64 refinements, or one FP64 "
+                        "solve per system (all are timed; this picks the reported value)")
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--method", default="crpcr")
@@ -645,8 +646,9 @@ def bench_ragged(args):
     """C4: 100,000 systems, sizes round(exp(U[ln 16, ln 8192])) (seed 1), FP64
     data; one graph replay of the persistent ragged kernel against plain FP64
     launches, one per distinct size, on the same data (sorted by size so each
-    size is contiguous). The ragged kernel is timed in both of its modes (one
-    FP64 solve; FP32 solve + 2 FP64 refinements) and with one refinement;
+    size is contiguous). The ragged kernel is timed in each of its modes: FP32
+    solve + FP64 refinements to a relative residual of 1e-12 (at most 4),
+    FP32 solve + exactly 2 or 1 FP64 refinements, one FP64 solve;
     --ragged-mode picks the reported value."""
     import numpy as np
     import torch
@@ -668,23 +670,23 @@ def bench_ragged(args):
     T.generate(0, 1, a, b, c, d, total, 1, stream=stream)
     plan_refine = R.RaggedPlan(offs)
     plan_fp64 = R.RaggedPlan(offs, table="default-fp64")
-    plan = plan_fp64 if args.ragged_mode == "fp64" else plan_refine
-    info = plan.info()
+    info = (plan_fp64 if args.ragged_mode == "fp64" else plan_refine).info()
     h = T.Handle()
     uniq, first = np.unique(sizes, return_index=True)
     runs = list(zip(uniq.tolist(), first.tolist(),
                     np.diff(np.append(first, N)).tolist()))
-
-    x_other = torch.empty_like(a)
-    fp64 = args.ragged_mode == "fp64"
-
-    other = plan_refine if fp64 else plan_fp64
-
-    def ragged_step():
-        plan.solve(a, b, c, d, x, stream=stream, fp64=fp64)
-
-    def other_step():
-        other.solve(a, b, c, d, x_other, stream=stream, fp64=not fp64)
+    TOL, TOL_ITERS = 1e-12, 4
+    names = {"tol": f"refine to 1e-12 (FP32 solve + FP64 refinements until every system's "
+                    f"relative residual is <= {TOL:g}, at most {TOL_ITERS})",
+             "refine": "refine (FP32 solve + 2 FP64 refinements)",
+             "refine1": "refine (FP32 solve + 1 FP64 refinement)",
+             "fp64": "fp64 (one FP64 solve)"}
+    steps_of = {
+        "tol": lambda xo: plan_refine.solve(a, b, c, d, xo, iters=TOL_ITERS, tol=TOL,
+                                            stream=stream),
+        "refine": lambda xo: plan_refine.solve(a, b, c, d, xo, iters=2, stream=stream),
+        "refine1": lambda xo: plan_refine.solve(a, b, c, d, xo, iters=1, stream=stream),
+        "fp64": lambda xo: plan_fp64.solve(a, b, c, d, xo, stream=stream, fp64=True)}
 
     def plain_step():
         for n, f, cnt in runs:
@@ -707,29 +709,21 @@ def bench_ragged(args):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps, e0.elapsed_time(e1) / steps
 
-    x_one = torch.empty_like(a)
-
-    def one_step():
-        plan_refine.solve(a, b, c, d, x_one, iters=1, stream=stream)
-
-    wall_r, gpu_r = timed(ragged_step, args.steps, args.warmup)
-    wall_o, gpu_o = timed(other_step, args.steps, args.warmup)
-    wall_1, gpu_1 = timed(one_step, args.steps, args.warmup)
-    wall_p, gpu_p = timed(plain_step, max(1, args.steps // 5), 1)
     d_offs = torch.from_numpy(offs).to(dev)
     rr = torch.empty(N, dtype=torch.float64, device=dev)
-    R.residual(a, b, c, d, x, d_offs, N, rr, stream=stream)
-    rr_r = float(rr.max())
-    R.residual(a, b, c, d, x_other, d_offs, N, rr, stream=stream)
-    rr_o = float(rr.max())
-    R.residual(a, b, c, d, x_one, d_offs, N, rr, stream=stream)
-    rr_1 = float(rr.max())
-    names = {True: "fp64 (one FP64 solve)", False: "refine (FP32 solve + 2 FP64 refinements)"}
-    modes = {names[fp64]: {"value": total / wall_r, "kernel_ms": gpu_r, "max_rel_residual": rr_r},
-             names[not fp64]: {"value": total / wall_o, "kernel_ms": gpu_o,
-                               "max_rel_residual": rr_o},
-             "refine (FP32 solve + 1 FP64 refinement)": {
-                 "value": total / wall_1, "kernel_ms": gpu_1, "max_rel_residual": rr_1}}
+    modes, timing = {}, {}
+    main = args.ragged_mode
+    for m in [main] + [k for k in steps_of if k != main]:
+        xo = x if m == main else torch.empty_like(a)
+        wall_m, gpu_m = timed(lambda: steps_of[m](xo), args.steps, args.warmup)
+        R.residual(a, b, c, d, xo, d_offs, N, rr, stream=stream)
+        timing[m] = (wall_m, gpu_m)
+        modes[names[m]] = {"value": total / wall_m, "kernel_ms": gpu_m,
+                           "max_rel_residual": float(rr.max())}
+    wall_r, gpu_r = timing[main]
+    rr_r = modes[names[main]]["max_rel_residual"]
+    fp64 = main == "fp64"
+    wall_p, gpu_p = timed(plain_step, max(1, args.steps // 5), 1)
     R.residual(a, b, c, d, x_plain, d_offs, N, rr, stream=stream)
     rr_p = float(rr.max())
     alg = 5 * 8 * total
@@ -739,7 +733,7 @@ def bench_ragged(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded device generator, FP64; sizes log-uniform in [16, 8192], seed 1)",
         "config": {"workload": f"ragged f64 {N} systems, sizes log-uniform [16, 8192], "
-                               f"{names[fp64]}, one graph launch",
+                               f"{names[main]}, one graph launch",
                    "systems": N, "unknowns": total, "plan": info},
         "modes": modes,
         "kernel_ms": {"ragged_graph": gpu_r, "plain_per_size": gpu_p},
@@ -748,7 +742,7 @@ def bench_ragged(args):
                      "frac": alg / (gpu_r * 1e-3) / 1e9 / PEAK_HBM_GBS,
                      "traffic": traffic_from_profiles(
                          f"ragged f64 {N} systems, sizes log-uniform [16, 8192], "
-                         f"{names[fp64]}, one graph launch"),
+                         f"{names[main]}, one graph launch"),
                      "note": "algorithmic 5*8 B per unknown (FP64 in/out)"
                              + ("" if fp64 else "; refinement re-reads are not credited")},
         "plain_per_size": {"value": total / wall_p, "launches": len(runs),

@@ -373,6 +373,18 @@ int trid_ragged_plan_info(trid_ragged_plan plan, trid_ragged_info* info,
 int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* b,
                           const double* c, const double* d, double* x, int iters, int flags,
                           void* stream);
+/* The refine mode to a residual target: after the FP32 solve a system takes
+ * FP64 refinements until max_i |d_i - (A x)_i| <= tol * max_i |d_i| (its
+ * relative residual, the FP64 residual of the current x; max|r| is taken
+ * from the residual rounded to FP32 and scaled by 1 + 2^-23, an upper bound)
+ * or it has taken maxIters (0..8) of them. The decision is made per system,
+ * so a system's result does not depend on the other systems of the batch;
+ * a workgroup stops once all of its systems have. tol = 0: exactly maxIters
+ * refinements (trid_ragged_solve_f64). TRID_EINVAL for a negative or
+ * non-finite tol, or with TRID_RAGGED_FP64. */
+int trid_ragged_solve_tol_f64(trid_ragged_plan plan, const double* a, const double* b,
+                              const double* c, const double* d, double* x, int maxIters,
+                              double tol, int flags, void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* Resident ADI grids (2D heat equation, Peaceman-Rachford)                 */

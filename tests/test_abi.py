@@ -200,6 +200,14 @@ def test_ragged_plan_rejects_bad_sizes_without_a_device(trid):
         assert e.value.status == 1
 
 
+def test_ragged_tolerance_arguments_without_a_device(trid):
+    lib = trid.lib()
+    null = ctypes.c_void_p(0)
+    for tol, flags in ((-1.0, 0), (float("nan"), 0), (float("inf"), 0), (1e-12, 2)):
+        assert lib.trid_ragged_solve_tol_f64(null, null, null, null, null, null, 2,
+                                             ctypes.c_double(tol), flags, null) == 1
+
+
 def test_autotune_candidates_are_valid_configs():
     from tridiagonalsolvers_amd.autotune import candidates
     for k in range(1, 15):

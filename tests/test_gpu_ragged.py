@@ -26,6 +26,8 @@ def _table(mode):
 def _solve(plan, dev, x, mode, **kw):
     if mode == "fp64":
         plan.solve(*dev, x, fp64=True, **kw)
+    elif mode == "tol":     # refine to relative residual <= 1e-12, at most 4 steps
+        plan.solve(*dev, x, iters=4, tol=1e-12, **kw)
     else:
         plan.solve(*dev, x, iters=2, **kw)
 
@@ -115,6 +117,44 @@ def test_ragged_alone_equals_in_the_mix(oracle, trid, rg, mode):
         assert np.array_equal(xs.cpu().numpy(), xm[o:e]), (s, e - o)
 
 
+def test_ragged_refines_to_a_residual_target(oracle, trid, rg):
+    """trid_ragged_solve_tol_f64: every system ends at or below the target
+    (the oracle's long-double residual), a target no system meets gives the
+    fixed-count result bit for bit, a target every FP32 solve meets gives the
+    unrefined result bit for bit, and each system decides for itself: alone
+    and in the mix its result is the same."""
+    import torch
+    sizes = np.concatenate([EDGE_SIZES, rg.log_uniform_sizes(1500, 16, 8192, seed=31)])
+    for kind in ("dominant", "weak"):
+        offs, host, dev = _batch(oracle, rg, sizes, seed=17, kind=kind)
+        plan = rg.RaggedPlan(offs)
+
+        def run(**kw):
+            x = torch.full_like(dev[0], float("nan"))
+            plan.solve(*dev, x, graph=False, **kw)
+            torch.cuda.synchronize()
+            return x.cpu().numpy()
+
+        for tol in (1e-12, 1e-9):
+            x = run(iters=8, tol=tol)
+            res = oracle.residual(*host, x, offsets=offs)
+            print(f"{kind} tol {tol:g}: max rel residual {float(np.max(res)):.2e}", flush=True)
+            assert float(np.max(res)) <= tol, (kind, tol, float(np.max(res)))
+        assert np.array_equal(run(iters=2, tol=1e-300), run(iters=2))
+        assert np.array_equal(run(iters=2, tol=1.0), run(iters=0))
+        xm = run(iters=8, tol=1e-12)
+        plan.close()
+        for s in list(range(len(EDGE_SIZES))) + [40, 300, 900, 1400]:
+            o, e = int(offs[s]), int(offs[s + 1])
+            one = [v[o:e].contiguous() for v in dev]
+            xs = torch.full_like(one[0], float("nan"))
+            p1 = rg.RaggedPlan(np.array([0, e - o], dtype=np.int64))
+            p1.solve(*one, xs, iters=8, tol=1e-12, graph=False)
+            torch.cuda.synchronize()
+            p1.close()
+            assert np.array_equal(xs.cpu().numpy(), xm[o:e]), (kind, s, e - o)
+
+
 def test_ragged_graph_replays_match_eager(oracle, trid, rg):
     import torch
     sizes = rg.log_uniform_sizes(2000, 16, 8192, seed=21)
@@ -179,7 +219,7 @@ def test_every_tuner_candidate_is_accurate(oracle, trid, rg, mode):
             _check_ragged(oracle, offs, host, x.cpu().numpy(), 1e-11)
 
 
-@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("mode", MODES + ["tol"])
 def test_ragged_100k_full_size(oracle, trid, rg, mode):
     """The configured ragged workload at its full size: 100,000 systems with
     sizes log-uniform in [16, 8192] (the benchmark's batch), each mode with

@@ -36,7 +36,7 @@ EXPORTS = (
     "trid_solve_pivoting_interleaved_f32", "trid_solve_pivoting_interleaved_f64",
     "trid_gtsv_f32", "trid_gtsv_f64",
     "trid_ragged_plan_create", "trid_ragged_plan_destroy", "trid_ragged_plan_info",
-    "trid_ragged_solve_f64", "trid_bucket_table_load",
+    "trid_ragged_solve_f64", "trid_ragged_solve_tol_f64", "trid_bucket_table_load",
     "trid_server_create", "trid_server_destroy", "trid_server_register_operator",
     "trid_server_release_operator", "trid_server_submit", "trid_server_wait",
     "trid_server_cancel", "trid_server_get_stats", "trid_server_host_alloc",
@@ -196,6 +196,8 @@ def lib() -> ctypes.CDLL:
     L.trid_bucket_table_load.restype = i32
     L.trid_ragged_solve_f64.argtypes = [P, P, P, P, P, P, i32, i32, P]
     L.trid_ragged_solve_f64.restype = i32
+    L.trid_ragged_solve_tol_f64.argtypes = [P, P, P, P, P, P, i32, ctypes.c_double, i32, P]
+    L.trid_ragged_solve_tol_f64.restype = i32
     L.trid_server_create.argtypes = [ctypes.POINTER(P), ctypes.POINTER(ServerOpts)]
     L.trid_server_destroy.argtypes = [P]
     L.trid_server_register_operator.argtypes = [P, P, i64, P, P, P, ctypes.POINTER(ctypes.c_int32)]

@@ -76,7 +76,10 @@ constexpr int kStageWaveBytes = RStage<8>::kBytes;  // 5120
 
 // Kernel modes: FP32 solve + FP64 refinement, or one FP64 solve (systems up
 // to 8192 unknowns).
-enum RaggedMode { kRefine = 0, kDirect = 1 };
+// kRefineTol: the refine mode with the residual check (a system stops
+// refining at the target); a kernel of its own, so that the fixed-count
+// kernels keep their registers.
+enum RaggedMode { kRefine = 0, kDirect = 1, kRefineTol = 2 };
 
 // The refine mode keeps each system's FP64 a and c in LDS across its
 // passes (and b, d in registers), so the coefficients are read from HBM once:
@@ -109,18 +112,24 @@ using Home = HomeT<(M > 8 ? 16 : 8)>;
 template <int MODE, int M>
 constexpr size_t ragged_wave_bytes()
 {
-    return MODE == kRefine ? (size_t)Home<M>::kWaveBytes
+    return MODE != kDirect ? (size_t)Home<M>::kWaveBytes
                            : (size_t)(M > 8 ? RStage<M>::kBytes : kStageWaveBytes);
 }
+// LDS of one group of W waves (one system): the exchange area of wg_core,
+// then 2 W boundary unknowns (the first and last unknown of every wave).
+template <typename T, int M>
+constexpr size_t ragged_group_bytes(int W)
+{
+    return wg_core_lds_bytes<T, M>(W) + (size_t)2 * W * sizeof(double);
+}
 // Exchange areas of the workgroup items of a WAVES-wave workgroup at M
-// unknowns per lane: WAVES/W groups of W waves, each with the exchange area
-// of wg_core and 2 W boundary unknowns (wg_core_lds_bytes is linear in W, so
-// one group of all the waves is the largest).
+// unknowns per lane: WAVES/W groups of W waves (ragged_group_bytes is linear
+// in W, so one group of all the waves is the largest).
 template <int MODE, int WAVES, int M>
 constexpr size_t ragged_xchg_bytes()
 {
-    return MODE == kDirect ? wg_core_lds_bytes<double, M>(WAVES) + 2 * WAVES * sizeof(double)
-                           : wg_core_lds_bytes<float, M>(WAVES) + 2 * WAVES * sizeof(double);
+    return MODE == kDirect ? ragged_group_bytes<double, M>(WAVES)
+                           : ragged_group_bytes<float, M>(WAVES);
 }
 template <int MODE, int WAVES, int M>
 constexpr size_t wg_lds_bytes()
@@ -133,7 +142,7 @@ constexpr size_t kWgSmallLds = 2 * sizeof(int) + kRaggedMaxBuckets * sizeof(Ragg
 // exchange area and 2 W boundary unknowns
 constexpr size_t kWideStageBytes = (size_t)kWgWaves * RStage<16>::kBytes;
 constexpr size_t kWideLdsBytes =
-    kWideStageBytes + wg_core_lds_bytes<float, 16>(16) + 2 * 16 * sizeof(double);
+    kWideStageBytes + ragged_group_bytes<float, 16>(16);
 static_assert(wg_lds_bytes<kDirect, kFullWaves, 16>() + kWgSmallLds <= 160 * 1024,
               "LDS of the FP64 8192-unknown items");
 static_assert(wg_lds_bytes<kRefine, kFullWaves, 16>() + kWgSmallLds <= 160 * 1024,
@@ -288,6 +297,104 @@ __device__ __forceinline__ void direct_rows(const RaggedArgs& g, int64_t base, i
         e[s] = Eq<double>{j0 + s > 0 ? va[s] : 0.0, vb[s], j0 + s < n - 1 ? vc[s] : 0.0, vd[s]};
 }
 
+// ---- residual check (refine mode with a tolerance) -------------------------
+// A system stops taking corrections once max|r| <= tol max|d| over its rows
+// (r = d - A x in FP64, the residual of the current x). The check compares
+// FP32 bounds: rb >= max|r| (each |fl(r)| scaled by 1 + 2^-23, an upper
+// bound for normal r; a NaN counts as infinite, so it fails) and
+// dl <= max|d| (rounded toward zero), so a passing system meets the target
+// within the rounding of one FP64 product. Checked from the second pass on
+// (the first has x = 0).
+
+// The lane's rb from the FP32 right-hand sides of its rows (rows past the
+// system hold 0).
+template <int M>
+__device__ __forceinline__ float rhs_bound(const Eq<float> (&e)[M])
+{
+    float m = 0.0f;
+#pragma unroll
+    for (int s = 0; s < M; ++s) {
+        const float v = fabsf(e[s].d);
+        m = fmaxf(m, v != v ? __builtin_huge_valf() : v);
+    }
+    // m + m 2^-23 >= m + ulp(m) >= |r| (|fl(r) - r| <= 2^-24 |r|), and
+    // m + ulp(m) is representable, so the rounded product stays above |r|
+    return m * (1.0f + 0x1p-23f);
+}
+// The lane's dl (rows past the system or a dead lane: 0).
+template <int M>
+__device__ __forceinline__ float d_bound(const double (&vd)[M], int j0, int n, bool live)
+{
+    double m = 0.0;
+#pragma unroll
+    for (int s = 0; s < M; ++s)
+        if (live && j0 + s < n) m = fmax(m, fabs(vd[s]));
+    return __double2float_rz(m);
+}
+// v of the lane CTRL's DPP row shift names (lanes without a source keep v).
+template <int CTRL>
+__device__ __forceinline__ float dpp_keep(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
+                                                      CTRL, 0xf, 0xf, false));
+}
+// Largest v of the wave, uniform: prefix maxima within rows of 16 lanes by
+// DPP row shifts, then the four row maxima.
+__device__ __forceinline__ float wave_max(float v)
+{
+    v = fmaxf(v, dpp_keep<0x111>(v));  // row_shr:1
+    v = fmaxf(v, dpp_keep<0x112>(v));  // row_shr:2
+    v = fmaxf(v, dpp_keep<0x114>(v));  // row_shr:4
+    v = fmaxf(v, dpp_keep<0x118>(v));  // row_shr:8
+    const int b = __float_as_int(v);
+    return fmaxf(fmaxf(__int_as_float(__builtin_amdgcn_readlane(b, 15)),
+                       __int_as_float(__builtin_amdgcn_readlane(b, 31))),
+                 fmaxf(__int_as_float(__builtin_amdgcn_readlane(b, 47)),
+                       __int_as_float(__builtin_amdgcn_readlane(b, 63))));
+}
+// Largest v over the aligned segment of L lanes (L a power of two <= 64).
+__device__ __forceinline__ float seg_max(float v, int L)
+{
+    for (int k = 1; k < L; k <<= 1) v = fmaxf(v, __shfl_xor(v, k));
+    return v;
+}
+// Wave items: the check of pass `step` for the systems of the wave (L lanes
+// each); returns whether every system of the wave is done.
+__device__ __forceinline__ bool wave_check(float rb, float& dl, int step, int L, double tol,
+                                           bool& done)
+{
+    rb = seg_max(rb, L);
+    if (step == 1) dl = seg_max(dl, L);
+    done = done || (double)rb <= tol * (double)dl;
+    return __all(done);
+}
+// Workgroup items: the check of pass `step` (every thread of the workgroup
+// calls it). Wave v publishes its bounds in chk[v] (rb) and chk[16 + v]
+// (dl, on the first check); after one barrier lane k < waves reads wave k's,
+// the groups of W lanes take their maxima (one system each), and a ballot
+// gives every lane both its own group's result and whether all are done.
+__device__ __forceinline__ bool wg_check(float rb, float dl, int step, int wave, int waves,
+                                         int W, int grp, double tol, float* chk, bool& done)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    rb = wave_max(rb);
+    if (step == 1) dl = wave_max(dl);
+    if (lane == 0) {
+        chk[wave] = rb;
+        if (step == 1) chk[16 + wave] = dl;
+    }
+    __syncthreads();
+    const bool mine = lane < waves;
+    float r = mine ? chk[lane] : 0.0f;
+    float d = mine ? chk[16 + lane] : 0.0f;
+    r = seg_max(r, W);
+    d = seg_max(d, W);
+    const bool ok = !mine || (double)r <= tol * (double)d;
+    const uint64_t bal = __ballot(ok);
+    done = done || ((bal >> (grp * W)) & 1);
+    return bal == ~0ull;
+}
+
 // ---- resident refine mode ---------------------------------------------------
 // Column element q of lane ll (unknown q*L + ll of the segment, as
 // fetch_cols loads it) into the row layout of a home array.
@@ -423,7 +530,7 @@ __device__ __forceinline__ void scatter_resident(double* __restrict__ X, int64_t
     __builtin_amdgcn_wave_barrier();
 }
 
-template <int M>
+template <int M, bool TOL>
 __device__ void wave_item_resident(const RaggedArgs& g, const RaggedItem& it,
                                    const RaggedBucket& bk, int lane, char* home)
 {
@@ -447,29 +554,41 @@ __device__ void wave_item_resident(const RaggedArgs& g, const RaggedItem& it,
     for (int s = 0; s < M; ++s) x[s] = 0.0;
     Eq<float> e[M];
     float xf[M];
+    float dl = 0.0f;
+    bool done = false;
     for (int step = 0; step <= g.iters; ++step) {
         const double xl = lane_step_or_zero<-1>(x[M - 1], ll >= 1);
         const double xr = lane_step_or_zero<1>(x[0], ll + 1 < L);
         resident_rows<M>(home, lane, vb, vd, x, xl, xr, j0, n, live, e);
+        if constexpr (TOL) {
+            if (step > 0) {
+                if (step == 1) dl = d_bound<M>(vd, j0, n, live);
+                if (wave_check(rhs_bound<M>(e), dl, step, L, g.tol, done)) break;  // uniform
+            }
+        }
         wave_core<float, M, 1>(e, xf, lane, ll, L, bk.rest);
+        if (!done) {
 #pragma unroll
-        for (int s = 0; s < M; ++s) x[s] += (double)xf[s];
+            for (int s = 0; s < M; ++s) x[s] += (double)xf[s];
+        }
     }
     scatter_resident<M>(g.x, base, n, L, ll, lane, live, home, x);
 }
 
 // Every wave of the workgroup calls this (wg_core has barriers). M unknowns
 // per lane: 8 (items of 1024..4096 unknowns), 16 (8192).
-template <int M>
+template <int M, bool TOL>
 __device__ void wg_item_resident(const RaggedArgs& g, const RaggedItem& it,
-                                 const RaggedBucket& bk, int wave, int lane, char* xchg_base,
-                                 char* home)
+                                 const RaggedBucket& bk, int wave, int waves, int lane,
+                                 char* xchg_base, char* home, float* chk)
 {
     asm volatile("" : "+v"(lane));  // formed per item (see wide_item)
-    const int W = bk.lanes / kWave;
+    // uniform in the workgroup: held in scalar registers, as are the group and
+    // the residual check's decisions derived from it
+    const int W = __builtin_amdgcn_readfirstlane(bk.lanes / kWave);
     const int grp = wave / W;
     const int w = wave % W;
-    const size_t grp_bytes = wg_core_lds_bytes<float, M>(W) + (size_t)2 * W * sizeof(double);
+    const size_t grp_bytes = ragged_group_bytes<float, M>(W);
     char* xchg = xchg_base + grp * grp_bytes;
     double* edge = reinterpret_cast<double*>(xchg + wg_core_lds_bytes<float, M>(W));
     const bool live = grp < it.count;
@@ -490,6 +609,7 @@ __device__ void wg_item_resident(const RaggedArgs& g, const RaggedItem& it,
     for (int s = 0; s < M; ++s) x[s] = 0.0;
     Eq<float> e[M];
     float xf[M];
+    bool done = false;
     for (int step = 0; step <= g.iters; ++step) {
         // lane-derived values (masks, permute addresses) are cheap: left
         // visible, the compiler keeps them all across the passes and spills
@@ -506,11 +626,23 @@ __device__ void wg_item_resident(const RaggedArgs& g, const RaggedItem& it,
             xr = ln < kWave - 1 ? xr0 : (w + 1 < W ? edge[2 * w + 2] : 0.0);
         }
         resident_rows<M>(home, ln, vb, vd, x, xl, xr, jj, nn, live, e);
+        if constexpr (TOL) {
+            if (step > 0) {
+                const float dl = step == 1 ? d_bound<M>(vd, jj, nn, live) : 0.0f;
+                if (wg_check(rhs_bound<M>(e), dl, step, wave, waves, W, grp, g.tol, chk, done))
+                    break;  // every system of the item
+            }
+        }
         wg_core<float, M>(e, xf, w, W, ln, bk.rest, xchg);
+        if (!done) {
 #pragma unroll
-        for (int s = 0; s < M; ++s) x[s] += (double)xf[s];
+            for (int s = 0; s < M; ++s) x[s] += (double)xf[s];
+        }
     }
-    scatter_resident<M>(g.x, base + wb, n - wb, kWave, lane, lane, live, home, x);
+    // the lane index formed again here, not kept across the passes
+    int ls = (int)threadIdx.x & (kWave - 1);
+    asm volatile("" : "+v"(ls));
+    scatter_resident<M>(g.x, base + wb, n - wb, kWave, ls, ls, live, home, x);
     __syncthreads();  // exchange areas are reused by the next item
 }
 
@@ -527,7 +659,8 @@ template <int M>
 __device__ __forceinline__ void ragged_rows(const RaggedArgs& g, const double (&x)[M],
                                             double x_left, double x_right, int64_t base,
                                             int nb, int j0, int n, int L, int ll,
-                                            int lane, bool live, char* stage, Eq<float> (&e)[M])
+                                            int lane, bool live, char* stage, Eq<float> (&e)[M],
+                                            double& rmax)
 {
     double r[M], v[M];
     auto use_b = [&]() {
@@ -553,7 +686,9 @@ __device__ __forceinline__ void ragged_rows(const RaggedArgs& g, const double (&
             const double xr = s < M - 1 ? x[s < M - 1 ? s + 1 : 0] : x_right;
             r[s] = fma(-cr, xr, r[s]);
             e[s].c = (float)cr;
-            e[s].d = (live && j0 + s < n) ? (float)r[s] : 0.0f;
+            const bool row = live && j0 + s < n;
+            e[s].d = row ? (float)r[s] : 0.0f;
+            if (row) rmax = fmax(rmax, fabs(r[s]));
         }
     };
     // see gather4 for why
@@ -605,8 +740,9 @@ __device__ __forceinline__ void ragged_rows(const RaggedArgs& g, const double (&
 }
 
 
+template <bool TOL>
 __device__ void wide_item(const RaggedArgs& g, const RaggedItem& it, const RaggedBucket& bk,
-                          int wave, int lane, char* xchg_base, char* stage)
+                          int wave, int lane, char* xchg_base, char* stage, float* chk)
 {
     // Lane-derived addresses are cheap to form; left visible, the compiler
     // forms every one of them once for the whole persistent loop and spills
@@ -615,7 +751,7 @@ __device__ void wide_item(const RaggedArgs& g, const RaggedItem& it, const Ragge
     constexpr int M = 16;
     const int W = bk.lanes / kWave;  // 16: one system per item
     const int w = wave;
-    const size_t grp_bytes = wg_core_lds_bytes<float, M>(W) + (size_t)2 * W * sizeof(double);
+    const size_t grp_bytes = ragged_group_bytes<float, M>(W);
     (void)grp_bytes;
     char* xchg = xchg_base;
     double* edge = reinterpret_cast<double*>(xchg + wg_core_lds_bytes<float, M>(W));
@@ -631,6 +767,8 @@ __device__ void wide_item(const RaggedArgs& g, const RaggedItem& it, const Ragge
     for (int s = 0; s < M; ++s) x[s] = 0.0;
     Eq<float> e[M];
     float xf[M];
+    float dl = 0.0f;
+    bool done = false;
     for (int step = 0; step <= g.iters; ++step) {
         int ln = lane;
         asm volatile("" : "+v"(ln));
@@ -644,10 +782,18 @@ __device__ void wide_item(const RaggedArgs& g, const RaggedItem& it, const Ragge
             xl = ln > 0 ? xl0 : (w > 0 ? edge[2 * w - 1] : 0.0);
             xr = ln < kWave - 1 ? xr0 : (w + 1 < W ? edge[2 * w + 2] : 0.0);
         }
-        ragged_rows<M>(g, x, xl, xr, base + wb, n - wb, j0, n, kWave, ln, ln, live, stage, e);
+        double dmax = 0.0;
+        ragged_rows<M>(g, x, xl, xr, base + wb, n - wb, j0, n, kWave, ln, ln, live, stage, e,
+                       dmax);
+        if constexpr (TOL) {
+            if (step == 0) dl = __double2float_rz(dmax);  // x = 0: the rows' |r| are |d|
+            else if (wg_check(rhs_bound<M>(e), dl, step, w, W, W, 0, g.tol, chk, done)) break;
+        }
         wg_core<float, M>(e, xf, w, W, ln, bk.rest, xchg);
+        if (!done) {
 #pragma unroll
-        for (int s = 0; s < M; ++s) x[s] += (double)xf[s];
+            for (int s = 0; s < M; ++s) x[s] += (double)xf[s];
+        }
     }
     scatter_rows<M>(g.x, base + wb, n - wb, kWave, lane, lane, live, stage, x);
     __syncthreads();  // exchange areas are reused by the next item
@@ -688,7 +834,7 @@ __device__ void wg_item_direct(const RaggedArgs& g, const RaggedItem& it, const 
     const int W = bk.lanes / kWave;
     const int grp = wave / W;
     const int w = wave % W;
-    const size_t grp_bytes = wg_core_lds_bytes<double, M>(W) + (size_t)2 * W * sizeof(double);
+    const size_t grp_bytes = ragged_group_bytes<double, M>(W);
     char* xchg = xchg_base + grp * grp_bytes;
     const bool live = grp < it.count;
     int64_t base = 0;
@@ -746,6 +892,7 @@ template <int MODE, int WAVES, int M>
 __device__ __forceinline__ void wg_items(const RaggedArgs& g, int lo, int hi, int* ctr)
 {
     __shared__ __align__(16) char smem[wg_lds_bytes<MODE, WAVES, M>()];
+    __shared__ float s_chk[32];  // residual check: per-wave bounds (refine mode)
     __shared__ int s_item[2];
     __shared__ RaggedBucket s_bk[kRaggedMaxBuckets];
     // the wave index is uniform in the wave: held in a scalar register, so
@@ -760,8 +907,9 @@ __device__ __forceinline__ void wg_items(const RaggedArgs& g, int lo, int hi, in
     drain_queue(lo, hi, ctr, s_item, [&](int item) {
         const RaggedItem it = g.items[item];
         const RaggedBucket bk = s_bk[it.bucket];
-        if constexpr (MODE == kRefine)
-            wg_item_resident<M>(g, it, bk, wave, lane, xchg_base, area);
+        if constexpr (MODE != kDirect)
+            wg_item_resident<M, MODE == kRefineTol>(g, it, bk, wave, WAVES, lane, xchg_base,
+                                                    area, s_chk);
         else wg_item_direct<M>(g, it, bk, wave, lane, xchg_base, area);
     });
 }
@@ -789,10 +937,12 @@ void ragged_half_kernel(RaggedArgs g)
     wg_items<MODE, kHalfWaves, 8>(g, g.n_full_items, g.n_wg_items, &g.counters[2]);
 }
 
-// Wide items [0, n_wide_items) (refine mode).
+// Wide items [0, n_wide_items) (refine mode; TOL: with the residual check).
+template <bool TOL>
 __global__ __launch_bounds__(kWgThreads) void ragged_wide_kernel(RaggedArgs g)
 {
     __shared__ __align__(16) char smem[kWideLdsBytes];
+    __shared__ float s_chk[32];  // residual check: per-wave bounds
     __shared__ int s_item[2];
     __shared__ RaggedBucket s_bk[kRaggedMaxBuckets];
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));  // uniform
@@ -802,8 +952,8 @@ __global__ __launch_bounds__(kWgThreads) void ragged_wide_kernel(RaggedArgs g)
     if (threadIdx.x < g.n_buckets) s_bk[threadIdx.x] = g.buckets[threadIdx.x];
     drain_queue(0, g.n_wide_items, &g.counters[0], s_item, [&](int item) {
         const RaggedItem it = g.items[item];
-        wide_item(g, it, s_bk[it.bucket], wave, lane, smem + kWideStageBytes,
-                  smem + (size_t)wave * RStage<16>::kBytes);
+        wide_item<TOL>(g, it, s_bk[it.bucket], wave, lane, smem + kWideStageBytes,
+                  smem + (size_t)wave * RStage<16>::kBytes, s_chk);
     });
 }
 
@@ -829,12 +979,13 @@ __global__ __launch_bounds__(kWaveThreads) void ragged_wave_kernel(RaggedArgs g)
          witem += waves_total) {
         const RaggedItem it = g.items[g.n_wg_items + witem];
         const RaggedBucket bk = s_bk[it.bucket];
-        if constexpr (MODE == kRefine) {
+        if constexpr (MODE != kDirect) {
+            constexpr bool TOL = MODE == kRefineTol;
             switch (bk.per_lane) {
-                case 1: wave_item_resident<1>(g, it, bk, lane, area); break;
-                case 2: wave_item_resident<2>(g, it, bk, lane, area); break;
-                case 4: wave_item_resident<4>(g, it, bk, lane, area); break;
-                case 8: wave_item_resident<8>(g, it, bk, lane, area); break;
+                case 1: wave_item_resident<1, TOL>(g, it, bk, lane, area); break;
+                case 2: wave_item_resident<2, TOL>(g, it, bk, lane, area); break;
+                case 4: wave_item_resident<4, TOL>(g, it, bk, lane, area); break;
+                case 8: wave_item_resident<8, TOL>(g, it, bk, lane, area); break;
                 default: break;
             }
         } else {
@@ -871,73 +1022,86 @@ int clamp_grid(int64_t work, int slots)
 
 RaggedGrids ragged_grids(int device, const RaggedArgs& g)
 {
-    // workgroup slots of each kernel on each device (the current device when
-    // first asked), found once
+    // workgroup slots of each kernel in each mode (refine, FP64, refine with
+    // the residual check) on each device (the current device when first
+    // asked), found once: [mode][wide, 8192, half, wave]
     static std::mutex mu;
-    static int cache[64][7];
-    int slots[7];
+    static int cache[64][3][4];
+    static bool known[64];
+    const int dv = device >= 0 && device < 64 ? device : 0;
+    const int mode = g.direct ? kDirect : (g.tol > 0.0 ? kRefineTol : kRefine);
+    int slots[4];
     {
         std::lock_guard<std::mutex> lk(mu);
-        int* c = cache[device >= 0 && device < 64 ? device : 0];
-        if (!c[0]) {
-            c[1] = slots_of((const void*)ragged_wg_kernel<kRefine>, kFullThreads, device);
-            c[2] = slots_of((const void*)ragged_wg_kernel<kDirect>, kFullThreads, device);
-            c[3] = slots_of((const void*)ragged_wide_kernel, kWgThreads, device);
-            c[4] = slots_of((const void*)ragged_wave_kernel<kRefine>, kWaveThreads, device);
-            c[5] = slots_of((const void*)ragged_half_kernel<kRefine>, kHalfThreads, device);
-            c[6] = slots_of((const void*)ragged_half_kernel<kDirect>, kHalfThreads, device);
-            c[0] = slots_of((const void*)ragged_wave_kernel<kDirect>, kWaveThreads, device);
+        int (*c)[4] = cache[dv];
+        if (!known[dv]) {
+            auto fill = [&](int m, const void* wg, const void* half, const void* wave) {
+                c[m][1] = slots_of(wg, kFullThreads, device);
+                c[m][2] = slots_of(half, kHalfThreads, device);
+                c[m][3] = slots_of(wave, kWaveThreads, device);
+            };
+            fill(kRefine, (const void*)ragged_wg_kernel<kRefine>,
+                 (const void*)ragged_half_kernel<kRefine>, (const void*)ragged_wave_kernel<kRefine>);
+            fill(kDirect, (const void*)ragged_wg_kernel<kDirect>,
+                 (const void*)ragged_half_kernel<kDirect>, (const void*)ragged_wave_kernel<kDirect>);
+            fill(kRefineTol, (const void*)ragged_wg_kernel<kRefineTol>,
+                 (const void*)ragged_half_kernel<kRefineTol>,
+                 (const void*)ragged_wave_kernel<kRefineTol>);
+            c[kRefine][0] = slots_of((const void*)ragged_wide_kernel<false>, kWgThreads, device);
+            c[kDirect][0] = 1;  // no FP64 form of the wide items
+            c[kRefineTol][0] = slots_of((const void*)ragged_wide_kernel<true>, kWgThreads, device);
+            known[dv] = true;
         }
-        for (int k = 0; k < 7; ++k) slots[k] = c[k];
+        for (int k = 0; k < 4; ++k) slots[k] = c[mode][k];
     }
     RaggedGrids r{};
-    r.wide = clamp_grid(g.n_wide_items, slots[3]);
-    r.wg = clamp_grid(g.n_full_items - g.n_wide_items, g.direct ? slots[2] : slots[1]);
-    r.half = clamp_grid(g.n_wg_items - g.n_full_items, g.direct ? slots[6] : slots[5]);
-    r.wave = clamp_grid((g.n_wave_items + kWaveWaves - 1) / kWaveWaves,
-                        g.direct ? slots[0] : slots[4]);
+    r.wide = clamp_grid(g.n_wide_items, slots[0]);
+    r.wg = clamp_grid(g.n_full_items - g.n_wide_items, slots[1]);
+    r.half = clamp_grid(g.n_wg_items - g.n_full_items, slots[2]);
+    r.wave = clamp_grid((g.n_wave_items + kWaveWaves - 1) / kWaveWaves, slots[3]);
     return r;
+}
+
+template <int MODE>
+static hipError_t launch_ragged_mode(const RaggedArgs& g, const RaggedGrids& grid,
+                                     hipStream_t stream)
+{
+    hipError_t e = hipSuccess;
+    if constexpr (MODE != kDirect) {
+        if (g.n_wide_items > 0) {
+            hipLaunchKernelGGL((ragged_wide_kernel<MODE == kRefineTol>), dim3((unsigned)grid.wide),
+                               dim3(kWgThreads), 0, stream, g);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+    }
+    if (g.n_full_items > g.n_wide_items) {
+        hipLaunchKernelGGL(ragged_wg_kernel<MODE>, dim3((unsigned)grid.wg), dim3(kFullThreads), 0,
+                           stream, g);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (g.n_wg_items > g.n_full_items) {
+        hipLaunchKernelGGL(ragged_half_kernel<MODE>, dim3((unsigned)grid.half),
+                           dim3(kHalfThreads), 0, stream, g);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (g.n_wave_items > 0) {
+        hipLaunchKernelGGL(ragged_wave_kernel<MODE>, dim3((unsigned)grid.wave),
+                           dim3(kWaveThreads), 0, stream, g);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_ragged(const RaggedArgs& g, const RaggedGrids& grid, hipStream_t stream)
 {
     if (g.direct && g.n_wide_items > 0) return hipErrorInvalidValue;  // no 16384 FP64 form
+    if (g.direct && g.tol != 0.0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(ragged_reset_kernel, dim3(1), dim3(64), 0, stream, g.counters);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (g.n_wide_items > 0) {
-        hipLaunchKernelGGL(ragged_wide_kernel, dim3((unsigned)grid.wide), dim3(kWgThreads), 0,
-                           stream, g);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
-    if (g.n_full_items > g.n_wide_items) {
-        if (g.direct)
-            hipLaunchKernelGGL(ragged_wg_kernel<kDirect>, dim3((unsigned)grid.wg),
-                               dim3(kFullThreads), 0, stream, g);
-        else
-            hipLaunchKernelGGL(ragged_wg_kernel<kRefine>, dim3((unsigned)grid.wg),
-                               dim3(kFullThreads), 0, stream, g);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
-    if (g.n_wg_items > g.n_full_items) {
-        if (g.direct)
-            hipLaunchKernelGGL(ragged_half_kernel<kDirect>, dim3((unsigned)grid.half),
-                               dim3(kHalfThreads), 0, stream, g);
-        else
-            hipLaunchKernelGGL(ragged_half_kernel<kRefine>, dim3((unsigned)grid.half),
-                               dim3(kHalfThreads), 0, stream, g);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
-    if (g.n_wave_items > 0) {
-        if (g.direct)
-            hipLaunchKernelGGL(ragged_wave_kernel<kDirect>, dim3((unsigned)grid.wave),
-                               dim3(kWaveThreads), 0, stream, g);
-        else
-            hipLaunchKernelGGL(ragged_wave_kernel<kRefine>, dim3((unsigned)grid.wave),
-                               dim3(kWaveThreads), 0, stream, g);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
-    return hipSuccess;
+    if (g.direct) return launch_ragged_mode<kDirect>(g, grid, stream);
+    if (g.tol > 0.0) return launch_ragged_mode<kRefineTol>(g, grid, stream);
+    return launch_ragged_mode<kRefine>(g, grid, stream);
 }
 
 }  // namespace trid

@@ -200,17 +200,19 @@ struct RaggedArgs {
     const RaggedSys* sys;     // device, numSystems, grouped by bucket
     const RaggedItem* items;  // device: wide items, workgroup items, then wave items
     const RaggedBucket* buckets;
-    int* counters;            // device, 3 queue counters (wide, 16-wave, <= 8-wave items)
+    int* counters;            // device, 3 queue counters (wide, 16-per-lane, 8-per-lane items)
     int n_wide_items;         // items of 8193..16384-unknown systems (the first items)
     int n_full_items;         // end of the items at 16 per lane in 8-wave workgroups
     int n_wg_items;           // end of the workgroup items (<= 8 waves after n_full_items)
     int n_wave_items;
     int n_buckets;            // <= kRaggedMaxBuckets
-    int iters;
+    int iters;                // refinement steps (the most, when tol > 0)
     int direct;               // 1: one FP64 solve per system (no FP32 stage)
+    double tol;               // > 0: a system takes no more refinements once
+                              // max|d - A x| <= tol max|d| (0: exactly `iters`)
 };
 
-// Grids of the four persistent kernels (wide items, 16-wave and <= 8-wave
+// Grids of the four persistent kernels (wide items, 16- and 8-per-lane
 // workgroup items, wave items) for a batch: every slot of the device, or
 // fewer when the batch has fewer items.
 struct RaggedGrids {

@@ -29,8 +29,9 @@ struct trid_ragged_plan_s {
     int n_wg_items = 0;
     int n_wave_items = 0;
     int n_buckets = 0;
-    trid::RaggedGrids grids{};  // of the refine mode and of the FP64 mode
-    trid::RaggedGrids grids_direct{};
+    trid::RaggedGrids grids{};  // of the refine mode, the FP64 mode and the
+    trid::RaggedGrids grids_direct{};  // refine mode with the residual check
+    trid::RaggedGrids grids_tol{};
     std::vector<trid::RaggedBucket> buckets;  // host copy (info)
     std::vector<int64_t> bucket_systems;
     char* dev = nullptr;  // one device block: systems, items, buckets, counters
@@ -43,6 +44,7 @@ struct trid_ragged_plan_s {
     hipGraphExec_t exec = nullptr;
     const void* key[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     int key_iters = -1;  // iters + 16 * (FP64 mode)
+    double key_tol = 0.0;
 };
 
 namespace {
@@ -429,6 +431,9 @@ int trid_ragged_plan_create(trid_ragged_plan* out, const int64_t* offsets, int64
     g.n_wg_items = p->n_wg_items;
     g.n_wave_items = p->n_wave_items;
     p->grids = trid::ragged_grids(p->device, g);
+    g.tol = 1.0;
+    p->grids_tol = trid::ragged_grids(p->device, g);
+    g.tol = 0.0;
     g.direct = 1;
     p->grids_direct = trid::ragged_grids(p->device, g);
     p->grid = std::max(std::max(p->grids.wg, p->grids.half),
@@ -469,8 +474,10 @@ int trid_ragged_plan_info(trid_ragged_plan plan, trid_ragged_info* info, trid_bu
     return TRID_OK;
 }
 
-int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* b, const double* c,
-                          const double* d, double* x, int iters, int flags, void* stream)
+namespace {
+
+int ragged_solve(trid_ragged_plan plan, const double* a, const double* b, const double* c,
+                 const double* d, double* x, int iters, double tol, int flags, void* stream)
 {
     if (!plan || !a || !b || !c || !d || !x || iters < 0 || iters > 8) return TRID_EINVAL;
     if (flags & ~(TRID_RAGGED_NO_GRAPH | TRID_RAGGED_FP64)) return TRID_EINVAL;
@@ -494,7 +501,9 @@ int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* 
     g.n_buckets = plan->n_buckets;
     g.iters = iters;
     g.direct = direct;
-    const trid::RaggedGrids& grids = direct ? plan->grids_direct : plan->grids;
+    g.tol = tol;
+    const trid::RaggedGrids& grids =
+        direct ? plan->grids_direct : (tol > 0.0 ? plan->grids_tol : plan->grids);
     hipStream_t s = static_cast<hipStream_t>(stream);
     if ((flags & TRID_RAGGED_NO_GRAPH) || s == nullptr) {
         hipError_t e = trid::launch_ragged(g, grids, s);
@@ -502,7 +511,7 @@ int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* 
     }
     const void* key[5] = {a, b, c, d, x};
     const int key_mode = iters + 16 * direct;
-    const bool same = plan->exec && plan->key_iters == key_mode &&
+    const bool same = plan->exec && plan->key_iters == key_mode && plan->key_tol == tol &&
                       std::memcmp(key, plan->key, sizeof(key)) == 0;
     if (!same) {
         if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
@@ -525,8 +534,26 @@ int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* 
         }
         std::memcpy(plan->key, key, sizeof(key));
         plan->key_iters = key_mode;
+        plan->key_tol = tol;
     }
     return hipGraphLaunch(plan->exec, s) == hipSuccess ? TRID_OK : TRID_EHIP;
+}
+
+}  // namespace
+
+int trid_ragged_solve_f64(trid_ragged_plan plan, const double* a, const double* b, const double* c,
+                          const double* d, double* x, int iters, int flags, void* stream)
+{
+    return ragged_solve(plan, a, b, c, d, x, iters, 0.0, flags, stream);
+}
+
+int trid_ragged_solve_tol_f64(trid_ragged_plan plan, const double* a, const double* b,
+                              const double* c, const double* d, double* x, int maxIters,
+                              double tol, int flags, void* stream)
+{
+    // tol: finite and >= 0 (0: exactly maxIters refinements); refine mode only
+    if (!(tol >= 0.0) || tol > 1e308 || (flags & TRID_RAGGED_FP64)) return TRID_EINVAL;
+    return ragged_solve(plan, a, b, c, d, x, maxIters, tol, flags, stream);
 }
 
 }  // extern "C"

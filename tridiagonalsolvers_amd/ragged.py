@@ -79,13 +79,21 @@ class RaggedPlan:
         return out
 
     def solve(self, a, b, c, d, x, iters: int | None = None, stream=None, graph: bool = True,
-              fp64: bool = False) -> None:
+              fp64: bool = False, tol: float = 0.0) -> None:
         """FP32 solve + `iters` FP64 refinements (default 2), or with fp64=True
-        one FP64 solve (iters must then be 0 or None; systems up to 8192)."""
+        one FP64 solve (iters must then be 0 or None; systems up to 8192).
+        tol > 0: a system stops refining once its relative residual
+        max|d - A x| / max|d| is at most tol (`iters` is then the most it
+        takes; trid_ragged_solve_tol_f64)."""
         if iters is None:
             iters = 0 if fp64 else 2
         flags = (0 if graph else NO_GRAPH) | (FP64 if fp64 else 0)
         ptrs = _dev_ptrs((a, b, c, d, x), int(self.offsets[-1]), "f64", "ragged solve")
+        if tol:
+            st = lib().trid_ragged_solve_tol_f64(self._h, *ptrs, int(iters), float(tol), flags,
+                                                 ctypes.c_void_p(_stream_ptr(stream)))
+            check(st, "trid_ragged_solve_tol_f64")
+            return
         st = lib().trid_ragged_solve_f64(self._h, *ptrs, int(iters), flags,
                                          ctypes.c_void_p(_stream_ptr(stream)))
         check(st, "trid_ragged_solve_f64")
