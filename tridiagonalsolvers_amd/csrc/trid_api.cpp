@@ -377,8 +377,9 @@ int interleaved_impl(trid_context* h, const T* a, const T* b, const T* c, const 
     if (n > INT64_MAX / stride / (int64_t)sizeof(T)) return TRID_EINVAL;
     int st = check_device();
     if (st != TRID_OK) return st;
-    // the register form (512 < n <= 1024 and nearby) needs no workspace; the
-    // two-ended elimination keeps its c' factors in n * stride elements
+    // the register form (256 < n <= 1024 in FP32, 128 < n <= 1024 in FP64)
+    // needs no workspace; the two-ended elimination keeps its c' factors in
+    // n * stride elements
     if (!trid::interleaved_reg_takes<T>(n, N)) {
         st = ensure_ws(h, (size_t)n * (size_t)stride * sizeof(T));
         if (st != TRID_OK) return st;

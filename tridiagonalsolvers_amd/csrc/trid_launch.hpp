@@ -100,7 +100,8 @@ struct InterleavedArgs {
 
 template <typename T>
 hipError_t launch_interleaved(const InterleavedArgs<T>& g);
-// The register-core form for 512 < n <= 1024 (kern_il_reg.hip); *taken is
+// The register-core form for 256 (FP32) / 128 (FP64) < n <= 1024
+// (kern_il_reg.hip); *taken is
 // false when the request is outside its range and nothing was launched.
 template <typename T>
 hipError_t launch_interleaved_reg(const InterleavedArgs<T>& g, bool* taken);
