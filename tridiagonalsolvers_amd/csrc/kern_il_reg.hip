@@ -40,13 +40,26 @@ namespace {
 #ifndef TRID_IL_SYSTEMS_F32
 #define TRID_IL_SYSTEMS_F32 16
 #endif
+// ... and 8 at 8 or 4 unknowns per lane, where 16 systems' LDS (99 / 132 KB)
+// leaves one workgroup per CU and 8 systems' (50 / 66 KB) two or three
+// (131,072 x 512: 348 us against 379-386 us; at 16 per lane 8 systems take
+// 419-421 us against 381-387 us, and FP64 at 4 systems 769-809 us against
+// 661-772 us)
+#ifndef TRID_IL_SYSTEMS_F32_SHORT
+#define TRID_IL_SYSTEMS_F32_SHORT 8
+#endif
+#ifndef TRID_IL_SYSTEMS_F64
+#define TRID_IL_SYSTEMS_F64 8
+#endif
 // Contiguous bytes of every row that the workgroups on one XCD take together
 // (xcd_span_group).
 constexpr int kIlSpanBytes = 1024;
 
 template <typename T, int MI>
 struct IlGeom {
-    static constexpr int S = sizeof(T) == 4 ? TRID_IL_SYSTEMS_F32 : 8;
+    static constexpr int S = sizeof(T) == 8 ? TRID_IL_SYSTEMS_F64
+                             : MI >= 16     ? TRID_IL_SYSTEMS_F32
+                                            : TRID_IL_SYSTEMS_F32_SHORT;
     static constexpr int EV = 16 / (int)sizeof(T);           // elements per vector
     static constexpr int VPR = S / EV;                        // vectors per row
     static constexpr int RC = S * kWave / VPR;                // rows per chunk
@@ -230,8 +243,8 @@ hipError_t launch_il_m(const InterleavedArgs<T>& g)
 // One wave per system: 16, 8 or 4 unknowns per lane for n up to 1024, 512
 // or 256; shorter systems stay with the lane-per-system elimination. The
 // bounds are measured (2^26 unknowns, against that elimination): FP32
-// n = 512: 452 / 492 us, n = 256 at 4 per lane: 532 / 460 us (its padded
-// LDS rows leave one workgroup per CU), so FP32 starts above 256; FP64
+// n = 512: 352 / 492 us, n = 256 at 4 per lane: 536 / 469 us (also with 8
+// systems per workgroup), so FP32 starts above 256; FP64
 // n = 512: 647 / 945 us, n = 200: 622 / 984 us, n = 129: 1012 / 1069 us.
 template <typename T>
 bool interleaved_reg_takes(int64_t n, int64_t N)
