@@ -47,7 +47,7 @@ def parse():
                         "grids); 0 = no served leg")
     p.add_argument("--ragged-mode", default="tol", choices=["tol", "refine", "fp64"],
                    help="ragged: FP32 solve + FP64 refinements until every system's "
-                        "relative residual is <= 1e-12 (at most 4; the mixed-precision "
+                        "relative residual is <= 1e-12 (one or two corrections; the mixed-precision "
                         "workload), FP32 solve + exactly 2 FP64 refinements, or one FP64 "
                         "solve per system (all are timed; this picks the reported value)")
     p.add_argument("--steps", type=int, default=50)
@@ -647,7 +647,7 @@ def bench_ragged(args):
     data; one graph replay of the persistent ragged kernel against plain FP64
     launches, one per distinct size, on the same data (sorted by size so each
     size is contiguous). The ragged kernel is timed in each of its modes: FP32
-    solve + FP64 refinements to a relative residual of 1e-12 (at most 4),
+    solve + FP64 refinements to a relative residual of 1e-12 (at most 2),
     FP32 solve + exactly 2 or 1 FP64 refinements, one FP64 solve;
     --ragged-mode picks the reported value."""
     import numpy as np
@@ -675,7 +675,7 @@ def bench_ragged(args):
     uniq, first = np.unique(sizes, return_index=True)
     runs = list(zip(uniq.tolist(), first.tolist(),
                     np.diff(np.append(first, N)).tolist()))
-    TOL, TOL_ITERS = 1e-12, 4
+    TOL, TOL_ITERS = 1e-12, 2
     names = {"tol": f"refine to 1e-12 (FP32 solve + FP64 refinements until every system's "
                     f"relative residual is <= {TOL:g}, at most {TOL_ITERS})",
              "refine": "refine (FP32 solve + 2 FP64 refinements)",

@@ -26,8 +26,8 @@ def _table(mode):
 def _solve(plan, dev, x, mode, **kw):
     if mode == "fp64":
         plan.solve(*dev, x, fp64=True, **kw)
-    elif mode == "tol":     # refine to relative residual <= 1e-12, at most 4 steps
-        plan.solve(*dev, x, iters=4, tol=1e-12, **kw)
+    elif mode == "tol":     # refine to relative residual <= 1e-12, one or two steps
+        plan.solve(*dev, x, iters=2, tol=1e-12, **kw)
     else:
         plan.solve(*dev, x, iters=2, **kw)
 
