@@ -10,7 +10,16 @@ header is newer. Packed FP32 instructions are disabled
 ``_build_lds_poison/``) is the same library compiled with
 ``-DTRID_LDS_POISON``: every LDS staging and exchange area is filled with
 NaN before each use, so a read of an element no thread wrote shows up as
-NaN in the parity tests.
+NaN in the parity tests. The ``asan`` and ``tsan`` variants
+(``_san/libtrid_asan.so``, ``_san/libtrid_tsan.so``) sanitize the host code
+only (AddressSanitizer + UndefinedBehaviorSanitizer, or ThreadSanitizer; the
+device code is the release code), for the CPU tests run with the runtime
+preloaded, e.g.::
+
+    ASAN_OPTIONS=detect_leaks=0 \
+    LD_PRELOAD=/opt/rocm/llvm/lib/clang/22/lib/linux/libclang_rt.asan-x86_64.so \
+    TRID_LIBRARY=tridiagonalsolvers_amd/_san/libtrid_asan.so \
+    python -m pytest tests -m "not gpu"
 """
 from __future__ import annotations
 
@@ -27,9 +36,18 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtrid.so")
 # debug variant: every LDS staging / exchange area filled with NaN before use
 LIB_LDS_POISON = os.path.join(HERE, "libtrid_lds_poison.so")
+_ASAN = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined"]
+_TSAN = ["-Xarch_host", "-fsanitize=thread"]
+_NOFP = ["-Xarch_host", "-fno-omit-frame-pointer"]
+# variant: (library, object directory, compile flags, link flags)
 VARIANTS = {
-    "release": (LIB, BUILD, []),
-    "lds_poison": (LIB_LDS_POISON, os.path.join(HERE, "_build_lds_poison"), ["-DTRID_LDS_POISON"]),
+    "release": (LIB, BUILD, [], []),
+    "lds_poison": (LIB_LDS_POISON, os.path.join(HERE, "_build_lds_poison"), ["-DTRID_LDS_POISON"],
+                   []),
+    "asan": (os.path.join(HERE, "_san", "libtrid_asan.so"), os.path.join(HERE, "_build_asan"),
+             _ASAN + _NOFP + ["-Xarch_host", "-fno-sanitize-recover=undefined"], _ASAN),
+    "tsan": (os.path.join(HERE, "_san", "libtrid_tsan.so"), os.path.join(HERE, "_build_tsan"),
+             _TSAN + _NOFP, _TSAN),
 }
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -66,16 +84,18 @@ def _compile(src: str, force: bool, build_dir: str = BUILD, defines=()) -> str:
 
 
 def build(force: bool = True, jobs: int = 8, variant: str = "release") -> str:
-    """Compile and link libtrid.so (variant "release") or the LDS-poisoning
-    debug library libtrid_lds_poison.so ("lds_poison", -DTRID_LDS_POISON).
-    ``force`` recompiles every object."""
-    lib_path, build_dir, defines = VARIANTS[variant]
+    """Compile and link libtrid.so (variant "release"), the LDS-poisoning
+    debug library libtrid_lds_poison.so ("lds_poison", -DTRID_LDS_POISON) or
+    a host-sanitized library ("asan", "tsan"). ``force`` recompiles every
+    object."""
+    lib_path, build_dir, defines, link_flags = VARIANTS[variant]
     os.makedirs(build_dir, exist_ok=True)
+    os.makedirs(os.path.dirname(lib_path), exist_ok=True)
     srcs = _sources()
     with cf.ThreadPoolExecutor(max_workers=min(jobs, len(srcs))) as ex:
         objs = list(ex.map(lambda s: _compile(s, force, build_dir, defines), srcs))
     tmp = lib_path + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *link_flags, "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
@@ -84,5 +104,8 @@ def build(force: bool = True, jobs: int = 8, variant: str = "release") -> str:
 
 
 if __name__ == "__main__":
-    print(build(force="--force" in sys.argv,
-                variant="lds_poison" if "--lds-poison" in sys.argv else "release"))
+    variant = "release"
+    for v in VARIANTS:
+        if "--" + v.replace("_", "-") in sys.argv:
+            variant = v
+    print(build(force="--force" in sys.argv, variant=variant))
