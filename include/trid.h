@@ -66,7 +66,7 @@ typedef struct trid_opts {
                            2..1024; above 64 a workgroup of lanes/64 waves
                            holds one system), 0 = plan                        */
     int32_t per_lane;   /* register kernels: unknowns per lane (1..16; 2..16
-                           FP32 / 2..8 FP64 when lanes > 64)                  */
+                           when lanes > 64, FP64 at 16 with <= 512 lanes)     */
     int32_t io_mode;    /* register kernels, dense batches: 0 = plan,
                            1 per-lane non-temporal, 2 per-lane cached,
                            3 wave-coalesced through LDS, 4 as 3 non-temporal  */

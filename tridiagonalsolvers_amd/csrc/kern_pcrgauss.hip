@@ -653,22 +653,6 @@ __global__ __launch_bounds__(kStagedWaves * 64) void strided_staged_kernel(
     }
 }
 
-// Compute units of the current device (cached per device).
-static int device_cus()
-{
-    static int cached[64] = {0};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-    if (cached[dev] == 0) {
-        int cus = 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
-                hipSuccess || cus <= 0)
-            cus = 256;
-        cached[dev] = cus;
-    }
-    return cached[dev];
-}
-
 // Steps of loads in flight per lane of the direct kernel, forward (4 loads,
 // 3 when normalized, and 2 stores per step) and backward (2 loads, 1 store),
 // within the 63 memory operations a wave can have outstanding. Measured on

@@ -240,13 +240,27 @@ constexpr size_t wg_core_lds_bytes(int W)
            (size_t)kWave * W * wg_red_bytes_per_lane<T>() + (size_t)kWave * W * sizeof(T);
 }
 
+// Workgroup barrier of the multi-wave core. DMA: the caller has global->LDS
+// copies in flight (global_load_lds into an LDS area the core never
+// touches). __syncthreads() would wait for them: its release fence covers
+// LDS writes, and those copies are LDS writes counted by vmcnt. This form
+// waits for the wave's own LDS operations only (ds_* and ds_bpermute, all
+// counted by lgkmcnt) and then meets the other waves; the "memory" clobber
+// keeps the compiler from moving memory accesses across it.
+template <bool DMA>
+__device__ __forceinline__ void wg_barrier()
+{
+    if constexpr (DMA) asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    else __syncthreads();
+}
+
 // Multi-wave form: W waves of one workgroup hold one system, wave w owning
 // unknowns [w*64*M, (w+1)*64*M) (see kern_reg_mw.hip). Every thread of the
 // workgroup must call it (it contains barriers). `xchg` is LDS of
 // wg_core_lds_bytes(W): W*(log2(M)+6) prefix equations, the reduced-system
 // area and 64*W values; the caller's staging area may alias it (a barrier is
-// taken before first use).
-template <typename T, int M>
+// taken before first use). DMA: see wg_barrier.
+template <typename T, int M, bool DMA = false>
 __device__ __forceinline__ void wg_core(Eq<T> (&e)[M], T (&xv)[M], int w, int W, int lane,
                                         int rest, char* xchg)
 {
@@ -258,12 +272,12 @@ __device__ __forceinline__ void wg_core(Eq<T> (&e)[M], T (&xv)[M], int w, int W,
                                  (size_t)kWave * W * wg_red_bytes_per_lane<T>());  // [64*W]
 
     const Eq<T> top0 = e[M - 1];  // level-0 top equation (lane 63's chain restarts here)
-    __syncthreads();              // staging rows become the exchange area
+    wg_barrier<DMA>();              // staging rows become the exchange area
     if constexpr (kLdsPoison) {
         // the W waves of this system fill its exchange area (other groups of
         // the workgroup fill theirs; every thread reaches the barrier)
         lds_fill_nan(xchg, (unsigned)wg_core_lds_bytes<T, M>(W), w * kWave + lane, W * kWave);
-        __syncthreads();
+        wg_barrier<DMA>();
     }
 
     // ---- in-lane CR levels (lane 63's top slot is provisional) ------------
@@ -297,7 +311,7 @@ __device__ __forceinline__ void wg_core(Eq<T> (&e)[M], T (&xv)[M], int w, int W,
     // prefix of cross-lane level j: lane 2^(j-1)-1 after level j-1 (final)
     if (((lane + 1) & lane) == 0 && lane + 1 <= (hs >> 1))
         pre[w * NPRE + LOGM + __builtin_ctz((unsigned)(lane + 1))] = e[M - 1];
-    __syncthreads();
+    wg_barrier<DMA>();
 
     // ---- redo lane 63's chain with the next wave's prefixes ---------------
     if (w + 1 < W) {
@@ -322,7 +336,7 @@ __device__ __forceinline__ void wg_core(Eq<T> (&e)[M], T (&xv)[M], int w, int W,
     if (rest <= W) {
         // ---- wave-top system: W equations, solved by wave 0 ---------------
         if (lane == kWave - 1) red[w] = e[M - 1];
-        __syncthreads();
+        wg_barrier<DMA>();
         if (w == 0) {
             const int L = W;
             const int ll = lane & (L - 1);
@@ -358,7 +372,7 @@ __device__ __forceinline__ void wg_core(Eq<T> (&e)[M], T (&xv)[M], int w, int W,
             }
             if (lane < W) xs[lane] = xt;
         }
-        __syncthreads();
+        wg_barrier<DMA>();
         if (lane == kWave - 1) xv[M - 1] = xs[w];
         xprevw = w > 0 ? xs[w - 1] : T(0);
     } else {
@@ -389,7 +403,7 @@ __device__ __forceinline__ void wg_core(Eq<T> (&e)[M], T (&xv)[M], int w, int W,
             me = normalize_eq(e[M - 1]);
             put(0, q, me);
         }
-        __syncthreads();
+        wg_barrier<DMA>();
         int cb = 0;  // buffer holding the current rows
         for (int delta = 1; delta < (rest >> 1); delta <<= 1) {
             if (pact) {
@@ -399,7 +413,7 @@ __device__ __forceinline__ void wg_core(Eq<T> (&e)[M], T (&xv)[M], int w, int W,
                 me = reduce_neq(me, lft, rgt);
                 put(cb ^ 1, q, me);
             }
-            __syncthreads();
+            wg_barrier<DMA>();
             cb ^= 1;
         }
         if (pact) {
@@ -412,7 +426,7 @@ __device__ __forceinline__ void wg_core(Eq<T> (&e)[M], T (&xv)[M], int w, int W,
             xv[M - 1] = tdiv(me.d - cm * p.d, T(1) - cm * cp);
             xs[q] = xv[M - 1];
         }
-        __syncthreads();
+        wg_barrier<DMA>();
         xprevw = w > 0 ? xs[(w * kWave) / LS - 1] : T(0);
     }
 

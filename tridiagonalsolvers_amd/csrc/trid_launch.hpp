@@ -4,11 +4,29 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <vector>
 
 #include "trid.h"
 
 namespace trid {
+
+// Compute units of the current device, cached per device (256 when the
+// query fails).
+inline int device_cus()
+{
+    static std::atomic<int> cached[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int cus = cached[dev].load(std::memory_order_relaxed);
+    if (cus == 0) {
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                hipSuccess || cus <= 0)
+            cus = 256;
+        cached[dev].store(cus, std::memory_order_relaxed);
+    }
+    return cus;
+}
 
 // Register-resident CR / CR-PCR / PCR (kern_reg.hip).
 template <typename T>
