@@ -133,31 +133,36 @@ __global__ __launch_bounds__(512) void reg_mw512_kernel(const T* __restrict__ A,
     reg_mw_body<T, M, IO>(A, B, C, D, X, n, rest, vec, N, bs);
 }
 
-// ---- persistent FP64 form with the next system's a, c copied ahead ---------
+// ---- persistent form with the next system's a, c copied ahead -------------
 //
-// FP64 systems of exactly kPfWaves * 64 * 16 = 8192 unknowns fill a CU's
-// registers, so one workgroup runs per CU and, in the kernels above, the
-// CU's memory pipe idles while the system is solved. Here each workgroup
-// walks the systems blockIdx.x, blockIdx.x + gridDim.x, ... and, while it
-// solves one, copies the next one's a and c (2 x 64 KiB) from HBM straight
-// into LDS (global_load_lds_dwordx4: 16 bytes a lane, no registers). b and d
-// are loaded after the wait, coalesced, and transposed through the same LDS
-// area (free by then); x is transposed through the exchange area and stored
-// coalesced, as many waves at a time as that area holds.
+// Systems of exactly kPfWaves * 64 * 16 = 8192 unknowns at 16 per lane: in
+// FP64 one system fills a CU's registers, in FP32 two do, so in the kernels
+// above the CU's memory pipe idles while its systems are solved. Here each
+// workgroup walks the systems blockIdx.x, blockIdx.x + gridDim.x, ... and,
+// while it solves one, copies the next one's a and c (2 x 64 KiB FP64,
+// 2 x 32 KiB FP32) from HBM straight into LDS (global_load_lds_dwordx4:
+// 16 bytes a lane, no registers). b and d are loaded after the wait,
+// coalesced, and transposed through the same LDS area (free by then); x is
+// transposed through the exchange area and stored coalesced, as many waves
+// at a time as that area holds.
 //
 // Copy layout: copy instruction k of wave w moves the k-th KiB of the wave's
-// block (64 lanes x 16 doubles = 8 KiB) in lane order, so the LDS holds the
-// block as it is in HBM and lane l reads its row (128 bytes) at 128 l. Those
-// 16-byte reads meet 8-way bank conflicts, but they overlap the b, d loads.
-// Measured on 4,096 x 8,192 FP64 (kernel average): this form 275 us against
-// 334 us for reg_mw512_kernel; rows swizzled within each KiB (conflict-free
-// reads, 16-byte pieces out of order within each instruction) 281 us; x by
-// per-lane 16-byte stores (lanes 128 bytes apart) 551-565 us.
+// block (64 lanes x 16 elements) in lane order, so the LDS holds the block
+// as it is in HBM and lane l reads its row at l * 16 * sizeof(T). Those
+// 16-byte reads meet bank conflicts, but they overlap the b, d loads.
+// Measured on 4,096 x 8,192 (kernel average): FP64 275-279 us against 334 us
+// for reg_mw512_kernel, FP32 111 us against 119 us for reg_mw_kernel (the
+// time of a plain stream of the same bytes); FP64 with the rows swizzled
+// within each KiB (conflict-free reads, 16-byte pieces out of order within
+// each copy instruction) 281 us; x by per-lane 16-byte stores (lanes 128
+// bytes apart) 551-565 us.
 constexpr int kPfWaves = 8;
 constexpr int kPfM = 16;
 constexpr int kPfN = kPfWaves * kWave * kPfM;  // 8192 unknowns
-constexpr size_t kPfArrayBytes = (size_t)kPfN * sizeof(double);  // 64 KiB per array
-constexpr size_t kPfWaveBytes = (size_t)kWave * kPfM * sizeof(double);  // 8 KiB
+template <typename T>
+constexpr size_t kPfArrayBytes = (size_t)kPfN * sizeof(T);  // 64 KiB FP64 / 32 KiB FP32
+template <typename T>
+constexpr size_t kPfWaveBytes = (size_t)kWave * kPfM * sizeof(T);  // 8 / 4 KiB
 
 // s_waitcnt immediate (gfx9 encoding) for vmcnt(0) alone: vmcnt bits 3:0
 // and 15:14 zero, expcnt (6:4) and lgkmcnt (11:8) at their maxima
@@ -165,16 +170,18 @@ constexpr int kWaitVmcnt0 = 0x0F70;
 
 // Issue the copies of system `sys`'s a and c (wave w's blocks) into `pf`:
 // a at pf, c at pf + kPfArrayBytes, wave w's block at w * kPfWaveBytes.
-__device__ __forceinline__ void pf_copy(const double* __restrict__ A,
-                                        const double* __restrict__ C, int64_t sys, int w,
-                                        int lane, char* pf)
+template <typename T>
+__device__ __forceinline__ void pf_copy(const T* __restrict__ A, const T* __restrict__ C,
+                                        int64_t sys, int w, int lane, char* pf)
 {
+    constexpr int E = 16 / (int)sizeof(T);                  // elements a lane copies
+    constexpr int K = (int)(kPfWaveBytes<T> / 1024);        // KiB of a wave's block
     const int64_t wbase = sys * kPfN + (int64_t)w * kWave * kPfM;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int64_t el = wbase + 128 * k + 2 * lane;  // 16 bytes: two doubles a lane
-        char* dst_a = pf + (size_t)w * kPfWaveBytes + (size_t)k * 1024;
-        char* dst_c = dst_a + kPfArrayBytes;
+    for (int k = 0; k < K; ++k) {
+        const int64_t el = wbase + (int64_t)kWave * E * k + E * lane;
+        char* dst_a = pf + (size_t)w * kPfWaveBytes<T> + (size_t)k * 1024;
+        char* dst_c = dst_a + kPfArrayBytes<T>;
         __builtin_amdgcn_global_load_lds(
             (const void*)(A + el), (__attribute__((address_space(3))) void*)dst_a, 16, 0, 0);
         __builtin_amdgcn_global_load_lds(
@@ -183,28 +190,38 @@ __device__ __forceinline__ void pf_copy(const double* __restrict__ A,
 }
 
 // LDS of the persistent kernel: the exchange area, and the total
-constexpr size_t pf_xchg_bytes() { return wg_core_lds_bytes<double, kPfM>(kPfWaves); }
-constexpr size_t pf_lds_bytes() { return 2 * kPfArrayBytes + pf_xchg_bytes(); }
+template <typename T>
+constexpr size_t pf_xchg_bytes() { return wg_core_lds_bytes<T, kPfM>(kPfWaves); }
+template <typename T>
+constexpr size_t pf_lds_bytes() { return 2 * kPfArrayBytes<T> + pf_xchg_bytes<T>(); }
+// workgroups per CU: FP64 fills the registers (256 a lane); FP32 takes two
+// (128 registers a lane, twice 81,152 bytes of LDS)
+template <typename T>
+constexpr int kPfPerCu = sizeof(T) == 8 ? 1 : 2;
 
-__global__ __launch_bounds__(kPfWaves * kWave) void reg_mw_pf_kernel(
-    const double* __restrict__ A, const double* __restrict__ B, const double* __restrict__ C,
-    const double* __restrict__ D, double* __restrict__ X, int rest, int64_t N)
+template <typename T>
+__global__ __launch_bounds__(kPfWaves * kWave)
+__attribute__((amdgpu_waves_per_eu(2 * kPfPerCu<T>))) void reg_mw_pf_kernel(
+    const T* __restrict__ A, const T* __restrict__ B, const T* __restrict__ C,
+    const T* __restrict__ D, T* __restrict__ X, int rest, int64_t N)
 {
-    using T = double;
     constexpr int M = kPfM, W = kPfWaves;
     using S = Stage<T, M>;
-    static_assert(2 * kPfArrayBytes >= (size_t)W * S::kWaveBytes,
+    static_assert(2 * kPfArrayBytes<T> >= (size_t)W * S::kWaveBytes,
                   "the b, d staging rows fit in the copy area");
+    static_assert(kPfPerCu<T> * pf_lds_bytes<T>() <= 160 * 1024, "LDS of the CU's workgroups");
     // Two static LDS variables: the compiler gives each its own alias scope,
     // so it can tell the exchange area's accesses from the copies' target and
     // does not make them wait for the copies in flight.
-    __shared__ __align__(16) char pf[2 * kPfArrayBytes];  // a, c copies; b, d staging
-    __shared__ __align__(16) char xchg[pf_xchg_bytes()];  // the core's exchange area
-    const int w = threadIdx.x / kWave;
+    __shared__ __align__(16) char pf[2 * kPfArrayBytes<T>];  // a, c copies; b, d staging
+    __shared__ __align__(16) char xchg[pf_xchg_bytes<T>()];  // the core's exchange area
+    // the wave index is uniform: a scalar register, and so are the wave's
+    // base offsets derived from it
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
     const int lane = threadIdx.x & (kWave - 1);
 
     int64_t sys = blockIdx.x;
-    pf_copy(A, C, sys, w, lane, pf);
+    pf_copy<T>(A, C, sys, w, lane, pf);
     for (; sys < N; sys += gridDim.x) {
         const int64_t wbase = sys * kPfN + (int64_t)w * kWave * M;
         // this system's copies have landed (every wave's): s_waitcnt vmcnt(0)
@@ -215,11 +232,12 @@ __global__ __launch_bounds__(kPfWaves * kWave) void reg_mw_pf_kernel(
         // a, c rows from the copies
         T ta[M], tb[M], tc[M], td[M];
         {
-            const char* ra = pf + (size_t)w * kPfWaveBytes + (size_t)lane * (kPfM * sizeof(T));
+            const char* ra =
+                pf + (size_t)w * kPfWaveBytes<T> + (size_t)lane * (kPfM * sizeof(T));
 #pragma unroll
             for (int q = 0; q < S::Q; ++q) {
                 const u32x4 va = *reinterpret_cast<const u32x4*>(ra + 16 * q);
-                const u32x4 vc = *reinterpret_cast<const u32x4*>(ra + kPfArrayBytes + 16 * q);
+                const u32x4 vc = *reinterpret_cast<const u32x4*>(ra + kPfArrayBytes<T> + 16 * q);
                 __builtin_memcpy(&ta[q * S::E], &va, 16);
                 __builtin_memcpy(&tc[q * S::E], &vc, 16);
             }
@@ -257,7 +275,7 @@ __global__ __launch_bounds__(kPfWaves * kWave) void reg_mw_pf_kernel(
             }
         }
         wg_barrier<true>();  // staging done: the next copies may land
-        if (sys + gridDim.x < N) pf_copy(A, C, sys + gridDim.x, w, lane, pf);
+        if (sys + gridDim.x < N) pf_copy<T>(A, C, sys + gridDim.x, w, lane, pf);
 
         Eq<T> e[M];
 #pragma unroll
@@ -267,7 +285,7 @@ __global__ __launch_bounds__(kPfWaves * kWave) void reg_mw_pf_kernel(
         T xv[M];
         wg_core<T, M, true>(e, xv, w, W, lane, rest, xchg);
         // x through the exchange area, as many waves at a time as it holds
-        constexpr int kPer = (int)(pf_xchg_bytes() / S::kWaveBytes);
+        constexpr int kPer = (int)(pf_xchg_bytes<T>() / S::kWaveBytes);
         static_assert(kPer >= 1, "one wave's staging rows fit the exchange area");
         wg_barrier<true>();  // every wave is past its last exchange-area read
         for (int r = 0; r < W; r += kPer) {
@@ -279,26 +297,21 @@ __global__ __launch_bounds__(kPfWaves * kWave) void reg_mw_pf_kernel(
     }
 }
 
-// The persistent form serves dense FP64 batches of 8192-unknown systems
-// with 16-byte aligned arrays, at 16 unknowns per lane.
-static bool pf_form(const RegArgs<double>& g)
+// The persistent form serves dense batches of 8192-unknown systems with
+// 16-byte aligned arrays, at 16 unknowns per lane.
+template <typename T>
+static bool pf_form(const RegArgs<T>& g)
 {
     return g.n == kPfN && g.M == kPfM && g.L == kPfWaves * kWave && g.vec &&
            (g.bs == 0 || g.bs == g.n) && g.io >= 2;
 }
 
-static int pf_grid(int64_t N)
+template <typename T>
+static hipError_t launch_pf(const RegArgs<T>& g)
 {
-    const int cus = device_cus();
-    return (int)(N < cus ? N : cus);
-}
-
-static hipError_t launch_pf(const RegArgs<double>& g)
-{
-    const int grid = pf_grid(g.N);
-    if (grid <= 0) return hipErrorInvalidDevice;
-    static_assert(pf_lds_bytes() <= 160 * 1024, "copy area + exchange area fit the LDS");
-    hipLaunchKernelGGL(reg_mw_pf_kernel, dim3((unsigned)grid), dim3(kPfWaves * kWave), 0,
+    const int64_t slots = (int64_t)kPfPerCu<T> * device_cus();
+    const int grid = (int)(g.N < slots ? g.N : slots);
+    hipLaunchKernelGGL(reg_mw_pf_kernel<T>, dim3((unsigned)grid), dim3(kPfWaves * kWave), 0,
                        g.stream, g.a, g.b, g.c, g.d, g.x, g.rest, g.N);
     return hipGetLastError();
 }
@@ -314,7 +327,7 @@ size_t mw_lds_bytes(int W, int io, int vec)
 template <typename T, int M, int IO>
 static hipError_t launch_mw_io(const RegArgs<T>& g)
 {
-    if constexpr (sizeof(T) == 8 && M == kPfM && IO >= 2) {
+    if constexpr (M == kPfM && IO >= 2) {
         if (pf_form(g)) return launch_pf(g);
     }
     const int W = g.L / kWave;
