@@ -164,10 +164,6 @@ constexpr size_t kPfArrayBytes = (size_t)kPfN * sizeof(T);  // 64 KiB FP64 / 32 
 template <typename T>
 constexpr size_t kPfWaveBytes = (size_t)kWave * kPfM * sizeof(T);  // 8 / 4 KiB
 
-// s_waitcnt immediate (gfx9 encoding) for vmcnt(0) alone: vmcnt bits 3:0
-// and 15:14 zero, expcnt (6:4) and lgkmcnt (11:8) at their maxima
-constexpr int kWaitVmcnt0 = 0x0F70;
-
 // Issue the copies of system `sys`'s a and c (wave w's blocks) into `pf`:
 // a at pf, c at pf + kPfArrayBytes, wave w's block at w * kPfWaveBytes.
 template <typename T>

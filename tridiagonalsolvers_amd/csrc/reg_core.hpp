@@ -254,6 +254,12 @@ __device__ __forceinline__ void wg_barrier()
     else __syncthreads();
 }
 
+// s_waitcnt immediate (gfx9 encoding) for vmcnt(0) alone: vmcnt bits 3:0
+// and 15:14 zero, expcnt (6:4) and lgkmcnt (11:8) at their maxima. Given to
+// __builtin_amdgcn_s_waitcnt, the wait is one the compiler's own wait
+// insertion sees (global->LDS copies it then knows to have landed).
+constexpr int kWaitVmcnt0 = 0x0F70;
+
 // Multi-wave form: W waves of one workgroup hold one system, wave w owning
 // unknowns [w*64*M, (w+1)*64*M) (see kern_reg_mw.hip). Every thread of the
 // workgroup must call it (it contains barriers). `xchg` is LDS of
