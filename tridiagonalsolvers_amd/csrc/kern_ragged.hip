@@ -383,7 +383,7 @@ __device__ __forceinline__ bool wg_check(float rb, float dl, int step, int wave,
         chk[wave] = rb;
         if (step == 1) chk[16 + wave] = dl;
     }
-    __syncthreads();
+    wg_barrier<true>();
     const bool mine = lane < waves;
     float r = mine ? chk[lane] : 0.0f;
     float d = mine ? chk[16 + lane] : 0.0f;
@@ -619,7 +619,7 @@ __device__ void wg_item_resident(const RaggedArgs& g, const RaggedItem& it,
         if (step > 0) {
             if (ln == 0) edge[2 * w] = x[0];
             if (ln == kWave - 1) edge[2 * w + 1] = x[M - 1];
-            __syncthreads();
+            wg_barrier<true>();
             const double xl0 = lane_read(x[M - 1], (ln - 1) & (kWave - 1));
             const double xr0 = lane_read(x[0], (ln + 1) & (kWave - 1));
             xl = ln > 0 ? xl0 : (w > 0 ? edge[2 * w - 1] : 0.0);
@@ -633,7 +633,7 @@ __device__ void wg_item_resident(const RaggedArgs& g, const RaggedItem& it,
                     break;  // every system of the item
             }
         }
-        wg_core<float, M>(e, xf, w, W, ln, bk.rest, xchg);
+        wg_core<float, M, true>(e, xf, w, W, ln, bk.rest, xchg);
         if (!done) {
 #pragma unroll
             for (int s = 0; s < M; ++s) x[s] += (double)xf[s];
@@ -643,7 +643,7 @@ __device__ void wg_item_resident(const RaggedArgs& g, const RaggedItem& it,
     int ls = (int)threadIdx.x & (kWave - 1);
     asm volatile("" : "+v"(ls));
     scatter_resident<M>(g.x, base + wb, n - wb, kWave, ls, ls, live, home, x);
-    __syncthreads();  // exchange areas are reused by the next item
+    wg_barrier<true>();  // exchange areas are reused by the next item
 }
 
 // ---- wide items (refine mode, 8193..16384 unknowns) ---------------------------
@@ -776,7 +776,7 @@ __device__ void wide_item(const RaggedArgs& g, const RaggedItem& it, const Ragge
         if (step > 0) {
             if (ln == 0) edge[2 * w] = x[0];
             if (ln == kWave - 1) edge[2 * w + 1] = x[M - 1];
-            __syncthreads();
+            wg_barrier<true>();
             const double xl0 = lane_read(x[M - 1], (ln - 1) & (kWave - 1));
             const double xr0 = lane_read(x[0], (ln + 1) & (kWave - 1));
             xl = ln > 0 ? xl0 : (w > 0 ? edge[2 * w - 1] : 0.0);
@@ -789,14 +789,14 @@ __device__ void wide_item(const RaggedArgs& g, const RaggedItem& it, const Ragge
             if (step == 0) dl = __double2float_rz(dmax);  // x = 0: the rows' |r| are |d|
             else if (wg_check(rhs_bound<M>(e), dl, step, w, W, W, 0, g.tol, chk, done)) break;
         }
-        wg_core<float, M>(e, xf, w, W, ln, bk.rest, xchg);
+        wg_core<float, M, true>(e, xf, w, W, ln, bk.rest, xchg);
         if (!done) {
 #pragma unroll
             for (int s = 0; s < M; ++s) x[s] += (double)xf[s];
         }
     }
     scatter_rows<M>(g.x, base + wb, n - wb, kWave, lane, lane, live, stage, x);
-    __syncthreads();  // exchange areas are reused by the next item
+    wg_barrier<true>();  // exchange areas are reused by the next item
 }
 
 // ---- FP64 mode items (one solve) -------------------------------------------------
@@ -850,15 +850,19 @@ __device__ void wg_item_direct(const RaggedArgs& g, const RaggedItem& it, const 
     double x[M];
     Eq<double> e[M];
     direct_rows<M>(g, base + wb, n - wb, j0, n, kWave, lane, lane, live, stage, e);
-    wg_core<double, M>(e, x, w, W, lane, bk.rest, xchg);
+    wg_core<double, M, true>(e, x, w, W, lane, bk.rest, xchg);
     scatter_rows<M>(g.x, base + wb, n - wb, kWave, lane, lane, live, stage, x);
-    __syncthreads();  // exchange areas are reused by the next item
+    wg_barrier<true>();  // exchange areas are reused by the next item
 }
 
 // Items [lo, hi) from the atomic queue counter `ctr`; every thread of the
 // workgroup calls it, `body(item)` ends with a workgroup barrier. The next
 // item's queue slot is claimed when an item starts and published when it
-// ends, so the atomic's latency hides behind the item.
+// ends, so the atomic's latency hides behind the item. The barriers of the
+// workgroup items order LDS only (wg_barrier<true>): an item's x stores
+// (rows no other item touches) and the next slot's atomic stay in flight
+// across them instead of being waited for at every barrier, as
+// __syncthreads() would (its release fence waits for all memory operations).
 template <typename F>
 __device__ __forceinline__ void drain_queue(int lo, int hi, int* ctr, int* s_item, F&& body)
 {
@@ -872,7 +876,7 @@ __device__ __forceinline__ void drain_queue(int lo, int hi, int* ctr, int* s_ite
         // body ends with a barrier, after which every thread has read
         // s_item[(k - 1) & 1]; slot k & 1 was last read before that barrier
         if (threadIdx.x == 0) s_item[k & 1] = next;
-        __syncthreads();
+        wg_barrier<true>();
         item = s_item[k & 1];
     }
 }
